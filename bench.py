@@ -1,0 +1,183 @@
+#!/usr/bin/env python3
+"""Gate-bootstrapping throughput on one MI355X: batched vs one gate per launch.
+
+Workload (default): one launch of 4096 independent gate bootstraps with the
+default 128-bit parameters (n=630, N=1024, k=1, l=3, Bgbit=7, keyswitch t=8,
+basebit=2), gate type uniform over NAND/AND/OR/XOR/XNOR/NOR, uniform input
+bits, one cloud key from seed 0; keys and ciphertexts resident in HBM when the
+timed region starts. A step is one such launch (blind rotation + sample
+extraction + keyswitch for all 4096 gates).
+
+Prints one JSON line (see the keys below). `value` is batched gates/s.
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+FLOP_PER_BOOTSTRAP_P128 = 630 * 258048  # n*[(k+1)(l+1)(5 M log2 M + 6 M) + 8 (k+1) kpl M], M = 512
+FP64_PEAK_TFLOPS = 78.6
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--batch", type=int, default=4096)
+    ap.add_argument("--per-gate", type=int, default=64, help="one-gate-per-launch samples")
+    ap.add_argument("--cpu-gates", type=int, default=500, help="reference CPU sample size (0 = skip)")
+    ap.add_argument("--seed", type=int, default=0)
+    return ap.parse_args()
+
+
+def cpu_baseline(count, seed):
+    """Reference spqlios-FMA CPU path (oracle/_ref, built from the reference's
+    own sources) timed on this host; falls back to the oracle port."""
+    harness = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+    if count <= 0:
+        return None
+    if os.path.exists(harness):
+        out = subprocess.run([harness, "bench", str(seed), str(count), "1"], capture_output=True, text=True,
+                             timeout=600, check=True).stdout.strip().splitlines()[-1]
+        r = json.loads(out)
+        return {"value": round(r["gates_per_s"], 3), "unit": "gates/s", "cores": 1, "kind": "reference",
+                "sample": f"{count} bootsNAND on fresh encryptions, P128 keys from seed {seed}, "
+                          f"reference spqlios-FMA build, 1 thread",
+                "p50_ms": r["p50_ms"], "p99_ms": r["p99_ms"]}
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_py as O
+    P = O.p128()
+    keys = O.keygen(P, seed)
+    n = 4
+    a = np.stack([O.boots_encrypt(keys, 1) for _ in range(n)])
+    t0 = time.perf_counter()
+    O.gates(keys, np.zeros(n, np.int32), a, a, threads=1)
+    dt = time.perf_counter() - t0
+    return {"value": round(n / dt, 3), "unit": "gates/s", "cores": 1, "kind": "port",
+            "sample": f"{n} NAND gates, exact-integer oracle port, 1 thread"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: independent gates shard across GPUs with a key replica each "
+              "and no exchange, so no multi-GPU run is made here.", file=sys.stderr)
+        sys.exit(2)
+    import tfhe_amd as T
+
+    if T.lib().tfhe_amd_device_count() < 1:
+        raise SystemExit("no GPU visible to libtfhe_amd.so")
+    P = T.default_params(110)
+    sk = T.SecretKey(P, seed=args.seed)
+    sk.upload()
+    B = args.batch
+    rng = np.random.default_rng(args.seed + 1)
+    ops = rng.integers(0, 6, size=B).astype(np.int32)
+    bits = rng.integers(0, 2, size=(2, B)).astype(np.int32)
+    a, b = sk.encrypt(bits[0]), sk.encrypt(bits[1])
+    w = P.n + 1
+    pool = np.concatenate([a, b, np.zeros((B, w), np.int32)])
+    idx = np.arange(B, dtype=np.int32)
+    d_pool = T.DeviceArray.from_numpy(pool)
+    d_ops = T.DeviceArray.from_numpy(ops)
+    d_i0, d_i1 = T.DeviceArray.from_numpy(idx), T.DeviceArray.from_numpy(idx + B)
+    d_o = T.DeviceArray.from_numpy(idx + 2 * B)
+
+    def step():
+        T.gates_device(sk.cloud, d_ops, d_i0, d_i1, d_i0, d_o, d_pool)
+
+    print(f"[bench] warmup {args.warmup} x {B} gates", file=sys.stderr, flush=True)
+    for _ in range(args.warmup):
+        step()
+    T.synchronize()
+    T.profile_read()
+    T.profile_enable(True)
+    T.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    T.synchronize()
+    dt = time.perf_counter() - t0
+    br_ms, ks_ms, launches = T.profile_read()
+    T.profile_enable(False)
+    out = d_pool.numpy()[2 * B:]
+    dec = sk.decrypt(out)
+    truth = {0: lambda x, y: 1 - (x & y), 1: lambda x, y: x & y, 2: lambda x, y: x | y, 3: lambda x, y: x ^ y,
+             4: lambda x, y: 1 - (x ^ y), 5: lambda x, y: 1 - (x | y)}
+    exp = np.array([truth[o](x, y) for o, x, y in zip(ops, bits[0], bits[1])])
+    errors = int((dec != exp).sum())
+    ms_per_step = 1e3 * dt / args.steps
+    batched = B * args.steps / dt
+    print(f"[bench] batched {batched:.1f} gates/s, {ms_per_step:.3f} ms/step, errors {errors}", file=sys.stderr,
+          flush=True)
+
+    # one gate per launch (same kernels, count = 1, synchronised each time)
+    lat = []
+    one = T.DeviceArray.from_numpy(np.zeros(1, np.int32))
+    for g in range(args.per_gate):
+        d1_ops = T.DeviceArray.from_numpy(ops[g:g + 1])
+        d1_i0 = T.DeviceArray.from_numpy(np.array([g], np.int32))
+        d1_i1 = T.DeviceArray.from_numpy(np.array([g + B], np.int32))
+        d1_o = T.DeviceArray.from_numpy(np.array([2 * B + g], np.int32))
+        T.synchronize()
+        t1 = time.perf_counter()
+        T.gates_device(sk.cloud, d1_ops, d1_i0, d1_i1, d1_i0, d1_o, d_pool)
+        T.synchronize()
+        lat.append(time.perf_counter() - t1)
+    one.free()
+    lat_ms = np.array(lat) * 1e3
+    per_gate = 1e3 / lat_ms.mean()
+    print(f"[bench] per-gate {per_gate:.1f} gates/s, p50 {np.percentile(lat_ms, 50):.3f} ms", file=sys.stderr,
+          flush=True)
+
+    br_avg_s = br_ms / max(launches, 1) / 1e3
+    achieved_tflops = FLOP_PER_BOOTSTRAP_P128 * B / br_avg_s / 1e12 if br_avg_s > 0 else None
+    res = {
+        "metric": "gates/s bootstrapped on 1 MI355X, batched vs per-gate; p50/p99 latency",
+        "value": round(batched, 2),
+        "unit": "gates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"{B} independent gate bootstraps per launch (NAND/AND/OR/XOR/XNOR/NOR uniform), "
+                               f"P128 n=630 N=1024 k=1 l=3 Bgbit=7 ks t=8 basebit=2, one cloud key (seed {args.seed})",
+                   "batch": B, "params": "P128"},
+        "per_gate": {"gates_per_s": round(per_gate, 2), "samples": args.per_gate,
+                     "p50_ms": round(float(np.percentile(lat_ms, 50)), 4),
+                     "p99_ms": round(float(np.percentile(lat_ms, 99)), 4)},
+        "batched_over_per_gate": round(batched / per_gate, 2),
+        "batch_latency_ms": {"p50": round(ms_per_step, 4), "p99": round(ms_per_step, 4)},
+        "decrypt_errors": errors,
+        "kernel_ms": {"blind_rotate_avg": round(br_ms / max(launches, 1), 4),
+                      "keyswitch_avg": round(ks_ms / max(launches, 1), 4)},
+        "roofline": {"bound": "mfma", "achieved": round(achieved_tflops, 3) if achieved_tflops else None,
+                     "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(achieved_tflops / FP64_PEAK_TFLOPS, 4) if achieved_tflops else None,
+                     "traffic": None,
+                     "kernel": "br_kernel (blind rotation, FP64)",
+                     "note": "FP64 peak: MI355X vector and matrix FP64 are both 78.6 TFLOP/s"},
+    }
+    try:
+        res["cpu_baseline"] = cpu_baseline(args.cpu_gates, args.seed)
+    except Exception as e:  # the GPU result stands on its own
+        res["cpu_baseline"] = {"error": str(e)}
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,266 @@
+/* tfhe_amd — MI355X gate-bootstrapping engine with the tfhe/tfhe C gate API.
+ *
+ * Drop-in boundary: the declarations below keep the names, argument meaning
+ * and struct layouts of the reference's public headers (src/include/ in
+ * tfhe/tfhe; each group cites the header it replaces), so C/C++ code written
+ * against libtfhe compiles and links against libtfhe_amd.so unchanged. The
+ * bootstrapping and keyswitch run on the GPU; key generation, encryption and
+ * decryption stay on the host, as in the reference.
+ *
+ * Error behaviour: like the reference, the void gate API aborts the process
+ * on a fatal error (after printing the reason to stderr, as die_dramatically
+ * does in libtfhe/tfhe_gate_bootstrapping.cpp:11-15). The added batch and
+ * device entry points return a status code instead (0 = success) and record
+ * the reason for tfhe_amd_last_error().
+ */
+#ifndef TFHE_AMD_H
+#define TFHE_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+#define TFHE_AMD_EXPORT extern "C" __attribute__((visibility("default")))
+#else
+#define TFHE_AMD_EXPORT __attribute__((visibility("default")))
+#endif
+
+/* ---- types: include/tfhe_core.h:37 and the struct headers ------------- */
+typedef int32_t Torus32;
+
+typedef struct LweParams LweParams;
+typedef struct LweKey LweKey;
+typedef struct LweSample LweSample;
+typedef struct LweKeySwitchKey LweKeySwitchKey;
+typedef struct TLweParams TLweParams;
+typedef struct TLweKey TLweKey;
+typedef struct TLweSample TLweSample;
+typedef struct TLweSampleFFT TLweSampleFFT;
+typedef struct TGswParams TGswParams;
+typedef struct TGswKey TGswKey;
+typedef struct TGswSample TGswSample;
+typedef struct TGswSampleFFT TGswSampleFFT;
+typedef struct LweBootstrappingKey LweBootstrappingKey;
+typedef struct LweBootstrappingKeyFFT LweBootstrappingKeyFFT;
+typedef struct IntPolynomial IntPolynomial;
+typedef struct TorusPolynomial TorusPolynomial;
+typedef struct TFheGateBootstrappingParameterSet TFheGateBootstrappingParameterSet;
+typedef struct TFheGateBootstrappingCloudKeySet TFheGateBootstrappingCloudKeySet;
+typedef struct TFheGateBootstrappingSecretKeySet TFheGateBootstrappingSecretKeySet;
+
+/* opaque device-resident key (bootstrapping key in FP64 transform form + keyswitch key) */
+typedef struct tfhe_amd_device_key tfhe_amd_device_key;
+
+/* include/lweparams.h:13-27 */
+struct LweParams { int32_t n; double alpha_min; double alpha_max; };
+/* include/tlwe.h:10-28 */
+struct TLweParams { int32_t N; int32_t k; double alpha_min; double alpha_max; LweParams extracted_lweparams; };
+/* include/tgsw.h:10-32 */
+struct TGswParams {
+    int32_t l; int32_t Bgbit; int32_t Bg; int32_t halfBg; uint32_t maskMod;
+    const TLweParams* tlwe_params; int32_t kpl; Torus32* h; uint32_t offset;
+};
+/* include/tfhe_gate_bootstrapping_structures.h:9-24 */
+struct TFheGateBootstrappingParameterSet {
+    int32_t ks_t; int32_t ks_basebit; const LweParams* in_out_params; const TGswParams* tgsw_params;
+};
+/* include/lwekey.h:10-20 */
+struct LweKey { const LweParams* params; int32_t* key; };
+/* include/lwesamples.h:9-20 */
+struct LweSample { Torus32* a; Torus32 b; double current_variance; };
+/* include/polynomials.h:10-34 */
+struct IntPolynomial { int32_t N; int32_t* coefs; };
+struct TorusPolynomial { int32_t N; Torus32* coefsT; };
+/* include/tlwe.h:30-63 */
+struct TLweKey { const TLweParams* params; IntPolynomial* key; };
+struct TLweSample { TorusPolynomial* a; TorusPolynomial* b; double current_variance; int32_t k; };
+/* include/tgsw.h:35-76 */
+struct TGswKey { const TGswParams* params; const TLweParams* tlwe_params; IntPolynomial* key; TLweKey tlwe_key; };
+struct TGswSample { TLweSample* all_sample; TLweSample** bloc_sample; int32_t k; int32_t l; };
+/* include/tgsw.h:78-96. There is no host transform-domain copy in this
+ * library (all_samples/sample are NULL): the sample is row `row` of the
+ * device-resident key of `owner`. */
+struct TGswSampleFFT {
+    TLweSampleFFT* all_samples; TLweSampleFFT** sample; int32_t k; int32_t l;
+    const LweBootstrappingKeyFFT* owner; int32_t row;
+};
+/* include/lwekeyswitch.h:11-28 (ks[i][j][h] = ks0_raw[(i*t+j)*base+h]) */
+struct LweKeySwitchKey {
+    int32_t n; int32_t t; int32_t basebit; int32_t base; const LweParams* out_params;
+    LweSample* ks0_raw; LweSample** ks1_raw; LweSample*** ks;
+};
+/* include/lwebootstrappingkey.h:10-33 */
+struct LweBootstrappingKey {
+    const LweParams* in_out_params; const TGswParams* bk_params; const TLweParams* accum_params;
+    const LweParams* extract_params; TGswSample* bk; LweKeySwitchKey* ks;
+};
+/* include/lwebootstrappingkey.h:36-59; dkey is filled on first use (or by
+ * tfhe_amd_cloud_key_upload) and points at the HBM-resident key, built from
+ * the coefficient-domain key bk_coef. */
+struct LweBootstrappingKeyFFT {
+    const LweParams* in_out_params; const TGswParams* bk_params; const TLweParams* accum_params;
+    const LweParams* extract_params; const TGswSampleFFT* bkFFT; const LweKeySwitchKey* ks;
+    tfhe_amd_device_key* dkey; const LweBootstrappingKey* bk_coef;
+};
+/* include/tfhe_gate_bootstrapping_structures.h:26-62 */
+struct TFheGateBootstrappingCloudKeySet {
+    const TFheGateBootstrappingParameterSet* params; const LweBootstrappingKey* bk; const LweBootstrappingKeyFFT* bkFFT;
+};
+struct TFheGateBootstrappingSecretKeySet {
+    const TFheGateBootstrappingParameterSet* params; const LweKey* lwe_key; const TGswKey* tgsw_key;
+    TFheGateBootstrappingCloudKeySet cloud;
+};
+
+/* ---- include/tfhe_gate_bootstrapping_functions.h:14-87 ----------------- */
+TFHE_AMD_EXPORT TFheGateBootstrappingParameterSet* new_default_gate_bootstrapping_parameters(int32_t minimum_lambda);
+TFHE_AMD_EXPORT TFheGateBootstrappingSecretKeySet* new_random_gate_bootstrapping_secret_keyset(const TFheGateBootstrappingParameterSet* params);
+TFHE_AMD_EXPORT void delete_gate_bootstrapping_parameters(TFheGateBootstrappingParameterSet* params);
+TFHE_AMD_EXPORT void delete_gate_bootstrapping_secret_keyset(TFheGateBootstrappingSecretKeySet* keyset);
+TFHE_AMD_EXPORT void delete_gate_bootstrapping_cloud_keyset(TFheGateBootstrappingCloudKeySet* keyset);
+TFHE_AMD_EXPORT LweSample* new_gate_bootstrapping_ciphertext(const TFheGateBootstrappingParameterSet* params);
+TFHE_AMD_EXPORT LweSample* new_gate_bootstrapping_ciphertext_array(int32_t nbelems, const TFheGateBootstrappingParameterSet* params);
+TFHE_AMD_EXPORT void delete_gate_bootstrapping_ciphertext(LweSample* sample);
+TFHE_AMD_EXPORT void delete_gate_bootstrapping_ciphertext_array(int32_t nbelems, LweSample* samples);
+TFHE_AMD_EXPORT void bootsSymEncrypt(LweSample* result, int32_t message, const TFheGateBootstrappingSecretKeySet* params);
+TFHE_AMD_EXPORT int32_t bootsSymDecrypt(const LweSample* sample, const TFheGateBootstrappingSecretKeySet* params);
+TFHE_AMD_EXPORT void bootsCONSTANT(LweSample* result, int32_t value, const TFheGateBootstrappingCloudKeySet* bk);
+TFHE_AMD_EXPORT void bootsNAND(LweSample* result, const LweSample* ca, const LweSample* cb, const TFheGateBootstrappingCloudKeySet* bk);
+TFHE_AMD_EXPORT void bootsOR(LweSample* result, const LweSample* ca, const LweSample* cb, const TFheGateBootstrappingCloudKeySet* bk);
+TFHE_AMD_EXPORT void bootsAND(LweSample* result, const LweSample* ca, const LweSample* cb, const TFheGateBootstrappingCloudKeySet* bk);
+TFHE_AMD_EXPORT void bootsXOR(LweSample* result, const LweSample* ca, const LweSample* cb, const TFheGateBootstrappingCloudKeySet* bk);
+TFHE_AMD_EXPORT void bootsXNOR(LweSample* result, const LweSample* ca, const LweSample* cb, const TFheGateBootstrappingCloudKeySet* bk);
+TFHE_AMD_EXPORT void bootsNOT(LweSample* result, const LweSample* ca, const TFheGateBootstrappingCloudKeySet* bk);
+TFHE_AMD_EXPORT void bootsCOPY(LweSample* result, const LweSample* ca, const TFheGateBootstrappingCloudKeySet* bk);
+TFHE_AMD_EXPORT void bootsNOR(LweSample* result, const LweSample* ca, const LweSample* cb, const TFheGateBootstrappingCloudKeySet* bk);
+TFHE_AMD_EXPORT void bootsANDNY(LweSample* result, const LweSample* ca, const LweSample* cb, const TFheGateBootstrappingCloudKeySet* bk);
+TFHE_AMD_EXPORT void bootsANDYN(LweSample* result, const LweSample* ca, const LweSample* cb, const TFheGateBootstrappingCloudKeySet* bk);
+TFHE_AMD_EXPORT void bootsORNY(LweSample* result, const LweSample* ca, const LweSample* cb, const TFheGateBootstrappingCloudKeySet* bk);
+TFHE_AMD_EXPORT void bootsORYN(LweSample* result, const LweSample* ca, const LweSample* cb, const TFheGateBootstrappingCloudKeySet* bk);
+TFHE_AMD_EXPORT void bootsMUX(LweSample* result, const LweSample* a, const LweSample* b, const LweSample* c, const TFheGateBootstrappingCloudKeySet* bk);
+
+/* ---- include/tfhe.h:34-45 (seed + bootstrapping subsystem) ------------- */
+TFHE_AMD_EXPORT void tfhe_random_generator_setSeed(uint32_t* values, int32_t size);
+TFHE_AMD_EXPORT void tfhe_blindRotate_FFT(TLweSample* accum, const TGswSampleFFT* bk, const int32_t* bara, const int32_t n, const TGswParams* bk_params);
+TFHE_AMD_EXPORT void tfhe_blindRotateAndExtract_FFT(LweSample* result, const TorusPolynomial* v, const TGswSampleFFT* bk, const int32_t barb, const int32_t* bara, const int32_t n, const TGswParams* bk_params);
+TFHE_AMD_EXPORT void tfhe_bootstrap_woKS_FFT(LweSample* result, const LweBootstrappingKeyFFT* bk, Torus32 mu, const LweSample* x);
+TFHE_AMD_EXPORT void tfhe_bootstrap_FFT(LweSample* result, const LweBootstrappingKeyFFT* bk, Torus32 mu, const LweSample* x);
+
+/* ---- include/lwe-functions.h:17-79 (host LWE ops + device keyswitch) ---- */
+TFHE_AMD_EXPORT void lweKeyGen(LweKey* result);
+TFHE_AMD_EXPORT void lweSymEncrypt(LweSample* result, Torus32 message, double alpha, const LweKey* key);
+TFHE_AMD_EXPORT Torus32 lwePhase(const LweSample* sample, const LweKey* key);
+TFHE_AMD_EXPORT void lweNoiselessTrivial(LweSample* result, Torus32 mu, const LweParams* params);
+TFHE_AMD_EXPORT void lweKeySwitch(LweSample* result, const LweKeySwitchKey* ks, const LweSample* sample);
+
+/* ---- include/numeric_functions.h:24-52 ---------------------------------- */
+TFHE_AMD_EXPORT Torus32 dtot32(double d);
+TFHE_AMD_EXPORT double t32tod(Torus32 x);
+TFHE_AMD_EXPORT int32_t modSwitchFromTorus32(Torus32 phase, int32_t Msize);
+TFHE_AMD_EXPORT Torus32 modSwitchToTorus32(int32_t mu, int32_t Msize);
+
+/* ---- allocators used by callers of the subsystem entry points
+ *      (include/lwesamples.h, include/lweparams.h, include/tlwe.h, include/polynomials.h) */
+TFHE_AMD_EXPORT LweParams* new_LweParams(int32_t n, double alpha_min, double alpha_max);
+TFHE_AMD_EXPORT void delete_LweParams(LweParams* obj);
+TFHE_AMD_EXPORT TLweParams* new_TLweParams(int32_t N, int32_t k, double alpha_min, double alpha_max);
+TFHE_AMD_EXPORT void delete_TLweParams(TLweParams* obj);
+TFHE_AMD_EXPORT TGswParams* new_TGswParams(int32_t l, int32_t Bgbit, const TLweParams* tlwe_params);
+TFHE_AMD_EXPORT void delete_TGswParams(TGswParams* obj);
+TFHE_AMD_EXPORT LweSample* new_LweSample(const LweParams* params);
+TFHE_AMD_EXPORT LweSample* new_LweSample_array(int32_t nbelts, const LweParams* params);
+TFHE_AMD_EXPORT void delete_LweSample(LweSample* obj);
+TFHE_AMD_EXPORT void delete_LweSample_array(int32_t nbelts, LweSample* obj);
+TFHE_AMD_EXPORT TLweSample* new_TLweSample(const TLweParams* params);
+TFHE_AMD_EXPORT void delete_TLweSample(TLweSample* obj);
+TFHE_AMD_EXPORT TorusPolynomial* new_TorusPolynomial(int32_t N);
+TFHE_AMD_EXPORT void delete_TorusPolynomial(TorusPolynomial* obj);
+
+/* ======================= additions (no reference counterpart) =========== */
+
+/* Gate codes for bootsGate_batch / the device API. */
+enum {
+    TFHE_AMD_NAND = 0, TFHE_AMD_AND = 1, TFHE_AMD_OR = 2, TFHE_AMD_XOR = 3, TFHE_AMD_XNOR = 4, TFHE_AMD_NOR = 5,
+    TFHE_AMD_ANDNY = 6, TFHE_AMD_ANDYN = 7, TFHE_AMD_ORNY = 8, TFHE_AMD_ORYN = 9, TFHE_AMD_MUX = 10
+};
+
+/* Parameter set from explicit values (the reference builds these only from
+ * C++ constructors, libtfhe/tfhe_gate_bootstrapping.cpp:51-76). */
+TFHE_AMD_EXPORT TFheGateBootstrappingParameterSet* new_gate_bootstrapping_parameters(
+    int32_t n, int32_t N, int32_t k, int32_t bk_l, int32_t bk_Bgbit, int32_t ks_t, int32_t ks_basebit,
+    double ks_stdev, double bk_stdev, double max_stdev);
+
+/* Cloud key from coefficient-domain arrays (e.g. keys generated by another
+ * implementation): bk int32 [n][(k+1)l][k+1][N], ksk int32 [k*N][t][base][n+1]. */
+TFHE_AMD_EXPORT TFheGateBootstrappingCloudKeySet* new_gate_bootstrapping_cloud_keyset_from_arrays(
+    const TFheGateBootstrappingParameterSet* params, const int32_t* bk, const int32_t* ksk);
+
+/* Batched gates: count independent gates, gate_op[g] in TFHE_AMD_*; in2 is
+ * only read for MUX and may be NULL otherwise. All inputs are read before any
+ * output is written, so outputs may alias inputs. Returns 0 on success. */
+TFHE_AMD_EXPORT int32_t bootsGate_batch(int32_t count, const int32_t* gate_op, LweSample* const* out,
+                                        const LweSample* const* in0, const LweSample* const* in1,
+                                        const LweSample* const* in2, const TFheGateBootstrappingCloudKeySet* ck);
+
+/* ---- bulk host helpers --------------------------------------------------- */
+/* bootsSymEncrypt of count bits into int32 [count][n+1] (a[0..n), b). */
+TFHE_AMD_EXPORT int32_t tfhe_amd_encrypt_bits(const TFheGateBootstrappingSecretKeySet* key, int32_t count,
+                                              const int32_t* bits, int32_t* out);
+/* lwePhase of count samples int32 [count][n+1]. */
+TFHE_AMD_EXPORT int32_t tfhe_amd_phases(const TFheGateBootstrappingSecretKeySet* key, int32_t count,
+                                        const int32_t* cts, int32_t* phases);
+/* Key material in flat layouts: lwe_key [n], tlwe_key [k*N],
+ * bk [n][(k+1)l][k+1][N], ksk [k*N][t][base][n+1]; NULL pointers are skipped. */
+TFHE_AMD_EXPORT int32_t tfhe_amd_export_keys(const TFheGateBootstrappingSecretKeySet* key, int32_t* lwe_key,
+                                             int32_t* tlwe_key, int32_t* bk, int32_t* ksk);
+/* &key->cloud, for callers without the struct layout. */
+TFHE_AMD_EXPORT const TFheGateBootstrappingCloudKeySet* tfhe_amd_cloud_of(const TFheGateBootstrappingSecretKeySet* key);
+
+/* ---- device-resident API (keys and ciphertexts stay in HBM) ------------- */
+/* Uploads (or returns the already uploaded) device key of a cloud key set. */
+TFHE_AMD_EXPORT int32_t tfhe_amd_cloud_key_upload(const TFheGateBootstrappingCloudKeySet* ck);
+/* Frees the device copy; the host key stays valid and is re-uploaded on next use. */
+TFHE_AMD_EXPORT void tfhe_amd_cloud_key_release(const TFheGateBootstrappingCloudKeySet* ck);
+/* Bytes of HBM the device key of ck occupies (0 if not uploaded). */
+TFHE_AMD_EXPORT size_t tfhe_amd_cloud_key_device_bytes(const TFheGateBootstrappingCloudKeySet* ck);
+
+/* Device ciphertext pool layout: int32 [slots][n+1] (a[0..n), b).
+ * Gate g reads pool[in0[g]], pool[in1[g]] (and pool[in2[g]] for MUX) and
+ * writes pool[out[g]]. All index arrays and the pool are device pointers;
+ * stream is a hipStream_t (NULL = the library's stream). Asynchronous. */
+TFHE_AMD_EXPORT int32_t tfhe_amd_gates_device(const TFheGateBootstrappingCloudKeySet* ck, int32_t count,
+                                              const int32_t* d_op, const int32_t* d_in0, const int32_t* d_in1,
+                                              const int32_t* d_in2, const int32_t* d_out, int32_t* d_pool,
+                                              void* stream);
+/* Same, bootstrapping only (no keyswitch): writes extracted samples int32
+ * [count][k*N+1] to d_ext. For MUX gates both halves are summed with +1/8. */
+TFHE_AMD_EXPORT int32_t tfhe_amd_gates_device_woks(const TFheGateBootstrappingCloudKeySet* ck, int32_t count,
+                                                   const int32_t* d_op, const int32_t* d_in0, const int32_t* d_in1,
+                                                   const int32_t* d_in2, const int32_t* d_pool, int32_t* d_ext,
+                                                   void* stream);
+/* Keyswitch count extracted samples int32 [count][k*N+1] into [count][n+1]. */
+TFHE_AMD_EXPORT int32_t tfhe_amd_keyswitch_device(const TFheGateBootstrappingCloudKeySet* ck, int32_t count,
+                                                  const int32_t* d_in, int32_t* d_out, void* stream);
+
+/* Launch configuration (autotuner hook). mode: 0 = automatic, 1 = one
+ * wavefront per bootstrap, 2 = two wavefronts per bootstrap. */
+TFHE_AMD_EXPORT void tfhe_amd_set_launch_config(int32_t br_mode, int32_t br_blocks_per_cu, int32_t ks_gates_per_block);
+TFHE_AMD_EXPORT int32_t tfhe_amd_get_launch_config(int32_t* br_mode, int32_t* br_blocks_per_cu, int32_t* ks_gates_per_block);
+
+/* Per-launch timing: when enabled, every device gate batch records HIP
+ * events on its stream around the blind-rotation kernel and around the
+ * keyswitch kernels. profile_read waits for them and returns the summed
+ * milliseconds and the number of batches since the last read. */
+TFHE_AMD_EXPORT void tfhe_amd_profile_enable(int32_t on);
+TFHE_AMD_EXPORT int32_t tfhe_amd_profile_read(double* br_ms, double* ks_ms, int32_t* launches);
+
+/* Device memory helpers for callers without a HIP runtime of their own. */
+TFHE_AMD_EXPORT void* tfhe_amd_malloc(size_t bytes);
+TFHE_AMD_EXPORT int32_t tfhe_amd_free(void* ptr);
+TFHE_AMD_EXPORT int32_t tfhe_amd_memcpy_h2d(void* dst, const void* src, size_t bytes);
+TFHE_AMD_EXPORT int32_t tfhe_amd_memcpy_d2h(void* dst, const void* src, size_t bytes);
+TFHE_AMD_EXPORT int32_t tfhe_amd_synchronize(void);
+TFHE_AMD_EXPORT void* tfhe_amd_stream(void);
+TFHE_AMD_EXPORT int32_t tfhe_amd_device_count(void);
+TFHE_AMD_EXPORT const char* tfhe_amd_last_error(void);
+
+#endif

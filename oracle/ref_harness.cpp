@@ -1,0 +1,206 @@
+// Test infrastructure only (never linked into the product).
+//
+// A driver around the reference library built from its own sources by
+// oracle/Makefile (`make -C oracle ref`). It uses only the reference's public
+// C API and struct layouts:
+//   keygen  : new_random_gate_bootstrapping_secret_keyset
+//             (libtfhe/tfhe_gate_bootstrapping.cpp:97-109)
+//   gates   : boots* (libtfhe/boot-gates.cpp:34-370)
+//   keyswitch: lweKeySwitch (libtfhe/lwe-keyswitch-functions.cpp:228-238)
+//
+// Commands (all binary outputs are little-endian int32 unless stated):
+//   keys  SEED OUTDIR           dump lwe_key, tlwe_key, bk (coefficient domain), ksk
+//   gates SEED COUNT OPS OUTFILE  encrypt random bits (global RNG, in order) and run gates
+//   ks    SEED COUNT OUTFILE    keyswitch random extracted-dimension samples
+//   bench SEED COUNT THREADS    time bootsNAND on fresh encryptions; prints one JSON line
+#include <tfhe.h>
+#include <tfhe_io.h>
+#include <lwesamples.h>
+#include <lwekey.h>
+#include <lweparams.h>
+#include <tlwe.h>
+#include <tgsw.h>
+#include <polynomials.h>
+#include <lwekeyswitch.h>
+#include <lwebootstrappingkey.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <random>
+#include <string>
+#include <thread>
+#include <vector>
+
+// Gate codes shared with the oracle and the product library.
+enum { G_NAND = 0, G_AND, G_OR, G_XOR, G_XNOR, G_NOR, G_ANDNY, G_ANDYN, G_ORNY, G_ORYN, G_MUX };
+
+static TFheGateBootstrappingSecretKeySet* make_keys(uint32_t seed) {
+    uint32_t s[1] = {seed};
+    tfhe_random_generator_setSeed(s, 1);
+    TFheGateBootstrappingParameterSet* params = new_default_gate_bootstrapping_parameters(110);
+    return new_random_gate_bootstrapping_secret_keyset(params);
+}
+
+static void write_all(FILE* f, const void* p, size_t bytes) {
+    if (fwrite(p, 1, bytes, f) != bytes) { perror("fwrite"); exit(2); }
+}
+
+static void dump_keys(const TFheGateBootstrappingSecretKeySet* key, const std::string& dir) {
+    const TFheGateBootstrappingParameterSet* P = key->params;
+    const int n = P->in_out_params->n;
+    const TGswParams* gp = P->tgsw_params;
+    const int N = gp->tlwe_params->N, k = gp->tlwe_params->k, kpl = gp->kpl;
+    FILE* f = fopen((dir + "/lwe_key.bin").c_str(), "wb");
+    write_all(f, key->lwe_key->key, sizeof(int32_t) * n);
+    fclose(f);
+    f = fopen((dir + "/tlwe_key.bin").c_str(), "wb");
+    for (int i = 0; i < k; i++) write_all(f, key->tgsw_key->tlwe_key.key[i].coefs, sizeof(int32_t) * N);
+    fclose(f);
+    const LweBootstrappingKey* bk = key->cloud.bk;
+    f = fopen((dir + "/bk.bin").c_str(), "wb");
+    for (int i = 0; i < n; i++)
+        for (int r = 0; r < kpl; r++)
+            for (int c = 0; c <= k; c++)
+                write_all(f, bk->bk[i].all_sample[r].a[c].coefsT, sizeof(int32_t) * N);
+    fclose(f);
+    const LweKeySwitchKey* ks = bk->ks;
+    f = fopen((dir + "/ksk.bin").c_str(), "wb");
+    const int rows = ks->n * ks->t * ks->base;
+    for (int r = 0; r < rows; r++) {
+        write_all(f, ks->ks0_raw[r].a, sizeof(int32_t) * n);
+        write_all(f, &ks->ks0_raw[r].b, sizeof(int32_t));
+    }
+    fclose(f);
+    f = fopen((dir + "/params.txt").c_str(), "w");
+    fprintf(f, "n=%d N=%d k=%d l=%d Bgbit=%d ks_t=%d ks_basebit=%d\n", n, N, k, gp->l, gp->Bgbit, P->ks_t, P->ks_basebit);
+    fclose(f);
+}
+
+static void run_gate(int op, LweSample* out, const LweSample* a, const LweSample* b, const LweSample* c,
+                     const TFheGateBootstrappingCloudKeySet* ck) {
+    switch (op) {
+        case G_NAND: bootsNAND(out, a, b, ck); break;
+        case G_AND: bootsAND(out, a, b, ck); break;
+        case G_OR: bootsOR(out, a, b, ck); break;
+        case G_XOR: bootsXOR(out, a, b, ck); break;
+        case G_XNOR: bootsXNOR(out, a, b, ck); break;
+        case G_NOR: bootsNOR(out, a, b, ck); break;
+        case G_ANDNY: bootsANDNY(out, a, b, ck); break;
+        case G_ANDYN: bootsANDYN(out, a, b, ck); break;
+        case G_ORNY: bootsORNY(out, a, b, ck); break;
+        case G_ORYN: bootsORYN(out, a, b, ck); break;
+        case G_MUX: bootsMUX(out, a, b, c, ck); break;
+        default: fprintf(stderr, "bad op %d\n", op); exit(2);
+    }
+}
+
+static void put_sample(FILE* f, const LweSample* s, int n) {
+    write_all(f, s->a, sizeof(int32_t) * n);
+    write_all(f, &s->b, sizeof(int32_t));
+}
+
+int main(int argc, char** argv) {
+    if (argc < 3) {
+        fprintf(stderr, "usage: ref_harness keys|gates|ks|bench SEED ...\n");
+        return 2;
+    }
+    const std::string cmd = argv[1];
+    const uint32_t seed = (uint32_t)strtoul(argv[2], nullptr, 10);
+    TFheGateBootstrappingSecretKeySet* key = make_keys(seed);
+    const TFheGateBootstrappingParameterSet* P = key->params;
+    const int n = P->in_out_params->n;
+    // input bits and gate codes come from a separate generator so that the
+    // library's global generator is consumed only by key generation and encryption
+    std::mt19937_64 bitgen(0x5eed0000ULL + seed);
+
+    if (cmd == "keys" && argc >= 4) {
+        dump_keys(key, argv[3]);
+        return 0;
+    }
+    if (cmd == "gates" && argc >= 6) {
+        const int count = atoi(argv[3]);
+        const std::string ops = argv[4];  // comma-free list of op codes as letters 'a'+code, or "rand6"
+        FILE* f = fopen(argv[5], "wb");
+        int32_t hdr[2] = {count, n};
+        write_all(f, hdr, sizeof hdr);
+        LweSample* ct = new_gate_bootstrapping_ciphertext_array(4, P);
+        for (int g = 0; g < count; g++) {
+            int op;
+            if (ops == "rand6") op = (int)(bitgen() % 6);
+            else op = ops[g % ops.size()] - 'a';
+            int32_t bits[3];
+            for (int j = 0; j < 3; j++) bits[j] = (int32_t)(bitgen() & 1);
+            const int nin = (op == G_MUX) ? 3 : 2;
+            for (int j = 0; j < nin; j++) bootsSymEncrypt(&ct[j], bits[j], key);
+            if (nin == 2) lweNoiselessTrivial(&ct[2], 0, P->in_out_params);
+            auto t0 = std::chrono::steady_clock::now();
+            run_gate(op, &ct[3], &ct[0], &ct[1], &ct[2], &key->cloud);
+            auto t1 = std::chrono::steady_clock::now();
+            int32_t rec[4] = {op, bits[0], bits[1], bits[2]};
+            write_all(f, rec, sizeof rec);
+            for (int j = 0; j < 4; j++) put_sample(f, &ct[j], n);
+            int32_t ph = lwePhase(&ct[3], key->lwe_key);
+            write_all(f, &ph, sizeof ph);
+            int64_t ns = std::chrono::duration_cast<std::chrono::nanoseconds>(t1 - t0).count();
+            write_all(f, &ns, sizeof ns);
+        }
+        fclose(f);
+        return 0;
+    }
+    if (cmd == "ks" && argc >= 5) {
+        const int count = atoi(argv[3]);
+        const LweKeySwitchKey* ks = key->cloud.bk->ks;
+        const LweParams* ext = &P->tgsw_params->tlwe_params->extracted_lweparams;
+        LweSample* in = new_LweSample(ext);
+        LweSample* out = new_LweSample(P->in_out_params);
+        FILE* f = fopen(argv[4], "wb");
+        int32_t hdr[3] = {count, ext->n, n};
+        write_all(f, hdr, sizeof hdr);
+        for (int g = 0; g < count; g++) {
+            for (int i = 0; i < ext->n; i++) in->a[i] = (int32_t)(uint32_t)bitgen();
+            in->b = (int32_t)(uint32_t)bitgen();
+            lweKeySwitch(out, ks, in);
+            put_sample(f, in, ext->n);
+            put_sample(f, out, n);
+        }
+        fclose(f);
+        return 0;
+    }
+    if (cmd == "bench" && argc >= 5) {
+        const int count = atoi(argv[3]);
+        const int threads = std::max(1, atoi(argv[4]));
+        std::vector<LweSample*> ins;
+        for (int g = 0; g < 2 * count; g++) {
+            LweSample* s = new_gate_bootstrapping_ciphertext(P);
+            bootsSymEncrypt(s, (int32_t)(bitgen() & 1), key);
+            ins.push_back(s);
+        }
+        std::vector<double> lat(count);
+        auto work = [&](int tid) {
+            LweSample* out = new_gate_bootstrapping_ciphertext(P);
+            for (int g = tid; g < count; g += threads) {
+                auto t0 = std::chrono::steady_clock::now();
+                bootsNAND(out, ins[2 * g], ins[2 * g + 1], &key->cloud);
+                auto t1 = std::chrono::steady_clock::now();
+                lat[g] = std::chrono::duration<double>(t1 - t0).count();
+            }
+            delete_gate_bootstrapping_ciphertext(out);
+        };
+        auto t0 = std::chrono::steady_clock::now();
+        std::vector<std::thread> th;
+        for (int t = 0; t < threads; t++) th.emplace_back(work, t);
+        for (auto& t : th) t.join();
+        double wall = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        std::vector<double> s = lat;
+        std::sort(s.begin(), s.end());
+        printf("{\"gates\": %d, \"threads\": %d, \"wall_s\": %.6f, \"gates_per_s\": %.3f, "
+               "\"p50_ms\": %.4f, \"p99_ms\": %.4f}\n",
+               count, threads, wall, count / wall, 1e3 * s[count / 2], 1e3 * s[(size_t)(0.99 * (count - 1))]);
+        return 0;
+    }
+    fprintf(stderr, "bad command\n");
+    return 2;
+}

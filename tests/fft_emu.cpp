@@ -1,0 +1,113 @@
+// Lane-by-lane CPU emulation of the wavefront negacyclic transform in
+// tfhe_amd/csrc/negacyclic_fft.h (the same functions the GPU kernels call),
+// checked against the exact negacyclic product mod 2^32.
+// Usage: fft_emu N TRIALS SEED DIGIT_BITS   -> prints "maxerr <e> mismatches <m>"
+#include <cstdio>
+#include <cstdlib>
+#include <cstdint>
+#include <random>
+#include <vector>
+#include <cmath>
+
+#include "../tfhe_amd/csrc/negacyclic_fft.h"
+
+using namespace tfhe_amd;
+
+template <int M>
+struct Emu {
+    using F = Fft<M>;
+    static constexpr int R = F::R;
+    std::vector<c64> tw;
+    Emu() : tw(TwiddleTable<M>::COUNT) { build_twiddles<M>(tw.data()); }
+    const c64* tw1(int lane) const { return tw.data() + lane * (R + 8); }
+    const c64* tw2(int lane) const { return tw.data() + lane * (R + 8) + R; }
+    const c64* twist() const { return tw.data() + 64 * (R + 8); }
+    const c64* itwist() const { return tw.data() + 64 * (R + 8) + R; }
+
+    // poly (length 2M, as doubles) -> frequency registers x[64][R]
+    void forward(const double* p, c64 (*x)[R]) {
+        std::vector<c64> lds(F::LDS_C64);
+        for (int b = 0; b < 64; b++)
+            for (int a = 0; a < R; a++) x[b][a] = c64{p[64 * a + b], p[64 * a + b + M]};
+        for (int l = 0; l < 64; l++) F::fwd1(x[l], l, lds.data(), twist(), tw1(l));
+        for (int l = 0; l < 64; l++) F::fwd2(x[l], l, lds.data(), tw2(l)[1]);
+        for (int l = 0; l < 64; l++) F::fwd3(x[l], l, lds.data());
+        for (int l = 0; l < 64; l++) F::fwd4(x[l], l, lds.data());
+    }
+    void inverse(c64 (*x)[R], double* p) {
+        std::vector<c64> lds(F::LDS_C64);
+        for (int l = 0; l < 64; l++) F::inv1(x[l], l, lds.data());
+        for (int l = 0; l < 64; l++) F::inv2(x[l], l, lds.data(), tw2(l)[1]);
+        for (int l = 0; l < 64; l++) F::inv3(x[l], l, lds.data());
+        for (int l = 0; l < 64; l++) F::inv4(x[l], l, lds.data(), tw1(l), itwist());
+        for (int b = 0; b < 64; b++)
+            for (int a = 0; a < R; a++) {
+                p[64 * a + b] = x[b][a].re;
+                p[64 * a + b + M] = x[b][a].im;
+            }
+    }
+};
+
+template <int M>
+int run(int trials, unsigned seed, int dbits) {
+    const int N = 2 * M;
+    Emu<M> E;
+    std::mt19937_64 rng(seed);
+    double maxerr = 0;
+    long mismatches = 0;
+    static c64 X[64][M / 64], Y[64][M / 64];
+    std::vector<double> pd(N), pt(N), out(N);
+    std::vector<int32_t> d(N), t(N);
+    for (int tr = 0; tr < trials; tr++) {
+        const int32_t half = 1 << (dbits - 1);
+        for (int j = 0; j < N; j++) {
+            d[j] = (int32_t)(rng() % (uint64_t)(2 * half)) - half;
+            t[j] = (int32_t)(uint32_t)rng();
+            pd[j] = d[j];
+            pt[j] = t[j];
+        }
+        // special cases: a monomial and a constant
+        if (tr == 0) { for (int j = 0; j < N; j++) { d[j] = (j == 5); pd[j] = d[j]; } }
+        if (tr == 1) { for (int j = 0; j < N; j++) { t[j] = 1000 * (j == N - 1); pt[j] = t[j]; } }
+        E.forward(pd.data(), X);
+        E.forward(pt.data(), Y);
+        for (int l = 0; l < 64; l++)
+            for (int s = 0; s < M / 64; s++) X[l][s] = cmul(X[l][s], Y[l][s]);
+        E.inverse(X, out.data());
+        // exact product mod 2^32
+        std::vector<uint32_t> ex(N, 0);
+        for (int i = 0; i < N; i++) {
+            if (!d[i]) continue;
+            for (int j = 0; j < N; j++) {
+                uint32_t v = (uint32_t)d[i] * (uint32_t)t[j];
+                int k = i + j;
+                if (k < N) ex[k] += v; else ex[k - N] -= v;
+            }
+        }
+        for (int j = 0; j < N; j++) {
+            double r = std::nearbyint(out[j]);
+            maxerr = std::fmax(maxerr, std::fabs(out[j] - r));
+            uint32_t got = (uint32_t)(int64_t)r;
+            if (got != ex[j]) mismatches++;
+        }
+    }
+    // round trip of a single polynomial
+    double rt = 0;
+    for (int j = 0; j < N; j++) pt[j] = (int32_t)(uint32_t)rng();
+    E.forward(pt.data(), X);
+    E.inverse(X, out.data());
+    for (int j = 0; j < N; j++) rt = std::fmax(rt, std::fabs(out[j] - pt[j]));
+    printf("maxerr %.6g mismatches %ld roundtrip %.6g\n", maxerr, mismatches, rt);
+    return mismatches ? 1 : 0;
+}
+
+int main(int argc, char** argv) {
+    const int N = argc > 1 ? atoi(argv[1]) : 1024;
+    const int trials = argc > 2 ? atoi(argv[2]) : 4;
+    const unsigned seed = argc > 3 ? atoi(argv[3]) : 1;
+    const int dbits = argc > 4 ? atoi(argv[4]) : 7;
+    if (N == 1024) return run<512>(trials, seed, dbits);
+    if (N == 2048) return run<1024>(trials, seed, dbits);
+    fprintf(stderr, "N must be 1024 or 2048\n");
+    return 2;
+}

@@ -1,0 +1,208 @@
+"""GPU parity: the HIP path (through the C-ABI in libtfhe_amd.so) against the
+CPU oracle (oracle/tfhe_oracle.cpp) and closed forms, on the same seeded keys.
+
+Bars:
+  keyswitch                 bit-exact (pure int32)
+  one external product      bit-exact vs the exact integer negacyclic product
+                            (FP64 transform with round-to-nearest; the
+                            transform's rounding error is < 0.04 units, so the
+                            rounded result equals the exact one)
+  blind rotation / gates    bit-exact ciphertexts vs the oracle on the same keys
+                            and inputs; decrypted bits equal the plaintext gate
+  noiseless key             closed form X^{sum bara_i s_i - barb} * testvector
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+import oracle_py as O
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def p128(gpu):
+    T = gpu
+    P = T.default_params(110)
+    sk = T.SecretKey(P, seed=0)
+    arrs = sk.export()
+    keys = O.Keys(O.p128(), **arrs)
+    sk.upload()
+    yield T, P, sk, keys
+    sk.close()
+
+
+def rand_i32(rng, shape):
+    return rng.integers(-2 ** 31, 2 ** 31, size=shape, dtype=np.int64).astype(np.int32)
+
+
+@pytest.mark.parametrize("count", [1, 3, 40, 600])
+def test_keyswitch_bitexact(p128, count):
+    T, P, sk, keys = p128
+    rng = np.random.default_rng(100 + count)
+    ext = rand_i32(rng, (count, P.N + 1))
+    ext[0, :] = 0  # all-zero digits row: the output must be (0, b) = 0
+    got = T.keyswitch(sk.cloud, P, ext)
+    idx = list(range(count)) if count <= 40 else list(rng.choice(count, 40, replace=False))
+    for g in idx:
+        assert np.array_equal(got[g], O.keyswitch(keys, ext[g])), f"gate {g}"
+
+
+def _row_ptr(cloud, i):
+    base = cloud.contents.bkFFT.contents.bkFFT
+    addr = ctypes.addressof(base.contents) + i * ctypes.sizeof(base._type_)
+    return ctypes.cast(addr, type(base))
+
+
+def _tlwe_params(P):
+    return P.ptr.contents.tgsw_params.contents.tlwe_params
+
+
+@pytest.mark.parametrize("row,nrows", [(0, 1), (17, 1), (629, 1), (100, 4)])
+def test_blind_rotate_exact(p128, row, nrows):
+    """tfhe_blindRotate_FFT on a random accumulator vs the exact product."""
+    T, P, sk, keys = p128
+    L = T.lib()
+    rng = np.random.default_rng(row * 7 + nrows)
+    acc0 = rand_i32(rng, (2, P.N))
+    bara = rng.integers(1, 2 * P.N, size=nrows).astype(np.int32)
+    if nrows > 1:
+        bara[1] = P.N  # rotation by exactly N (pure negation)
+    acc = L.new_TLweSample(_tlwe_params(P))
+    try:
+        for c in range(2):
+            ctypes.memmove(acc.contents.a[c].coefsT, acc0[c].ctypes.data, 4 * P.N)
+        L.tfhe_blindRotate_FFT(acc, _row_ptr(sk.cloud, row), bara.ctypes.data_as(T.i32p), nrows,
+                               P.ptr.contents.tgsw_params)
+        got = np.zeros((2, P.N), np.int32)
+        for c in range(2):
+            ctypes.memmove(got[c].ctypes.data, acc.contents.a[c].coefsT, 4 * P.N)
+    finally:
+        L.delete_TLweSample(acc)
+    ref = acc0.copy()
+    OP = O.p128()
+    OP.n = nrows
+    bk_rows = np.ascontiguousarray(keys.bk[row:row + nrows])
+    O.lib().orc_blind_rotate(ctypes.byref(OP), O.ptr(bk_rows), O.ptr(bara), O.ptr(ref))
+    assert np.array_equal(got, ref), f"max diff {np.abs(got.astype(np.int64) - ref).max()}"
+
+
+def test_noiseless_key_closed_form(p128):
+    """With BK_i = s_i * H (no mask, no noise) the rotation is exact:
+    extracted sample = (0, +-MU) with sign from X^{sum bara_i s_i - barb}."""
+    T, P, sk, keys = p128
+    n, N, l = P.n, P.N, P.l
+    bk = np.zeros((n, P.kpl, 2, N), np.int32)
+    for r in range(P.kpl):
+        blk, p = divmod(r, l)
+        bk[:, r, blk, 0] = keys.lwe_key * (1 << (32 - (p + 1) * P.Bgbit))
+    ck = T.cloud_from_arrays(P, bk, keys.ksk)
+    try:
+        rng = np.random.default_rng(5)
+        count = 64
+        a = rand_i32(rng, (count, n + 1))
+        b = rand_i32(rng, (count, n + 1))
+        ops = rng.integers(0, 10, size=count).astype(np.int32)
+        ext = T.run_gates_woks(ck, P, ops, a, b)
+        mu = 1 << 29
+        for g in range(count):
+            lin = np.zeros(n + 1, np.int32)
+            O.lib().orc_gate_linear(int(ops[g]), 0, O.ptr(a[g]), O.ptr(b[g]), O.ptr(a[g]), n, O.ptr(lin))
+            bara = [O.lib().orc_modswitch_from_torus32(int(v), 2 * N) for v in lin]
+            e = (sum(int(x) * int(s) for x, s in zip(bara[:n], keys.lwe_key)) - bara[n]) % (2 * N)
+            idx = (-e) % (2 * N)
+            expect_b = mu if idx < N else -mu
+            assert not ext[g, :N].any(), f"gate {g}: mask not zero"
+            assert ext[g, N] == expect_b, f"gate {g}"
+    finally:
+        T.lib().delete_gate_bootstrapping_cloud_keyset(ck)
+
+
+def _encrypt_inputs(sk, rng, count):
+    bits = rng.integers(0, 2, size=(3, count)).astype(np.int32)
+    return bits, sk.encrypt(bits[0]), sk.encrypt(bits[1]), sk.encrypt(bits[2])
+
+
+def test_gates_bitexact_vs_oracle(p128):
+    """Every gate type (MUX included): ciphertexts equal the exact oracle's."""
+    T, P, sk, keys = p128
+    rng = np.random.default_rng(7)
+    ops = np.array(list(range(11)) + [10, 0, 3, 5], np.int32)
+    bits, a, b, c = _encrypt_inputs(sk, rng, len(ops))
+    got = T.run_gates(sk.cloud, ops, a, b, c)
+    ref = O.gates(keys, ops, a, b, c, threads=16)
+    exact = (got == ref).all(axis=1)
+    dph = sk.phases(got).astype(np.int64) - sk.phases(ref).astype(np.int64)
+    assert exact.all(), f"bit-exact {exact.mean():.3f}, max phase diff {np.abs(dph).max()} units"
+    dec = sk.decrypt(got)
+    for g, op in enumerate(ops):
+        assert dec[g] == O.TRUTH[O.GATES[op]](bits[0][g], bits[1][g], bits[2][g])
+
+
+def test_gates_truth_tables_4096(p128):
+    T, P, sk, keys = p128
+    rng = np.random.default_rng(11)
+    count = 4096
+    ops = rng.integers(0, 6, size=count).astype(np.int32)
+    bits, a, b, c = _encrypt_inputs(sk, rng, count)
+    got = T.run_gates(sk.cloud, ops, a, b)
+    dec = sk.decrypt(got)
+    exp = np.array([O.TRUTH[O.GATES[o]](x, y, 0) for o, x, y in zip(ops, bits[0], bits[1])])
+    assert (dec == exp).all(), f"{(dec != exp).sum()} decryption errors"
+    # a sample of them bit-exact against the oracle
+    sel = rng.choice(count, 8, replace=False)
+    ref = O.gates(keys, ops[sel], a[sel], b[sel], threads=16)
+    assert np.array_equal(got[sel], ref)
+
+
+def test_c_abi_single_gates_match_batch(p128):
+    """bootsNAND/.../bootsMUX (one gate per call, host LweSample) equal the
+    batched device path on the same inputs, including result aliasing an input."""
+    T, P, sk, keys = p128
+    L = T.lib()
+    rng = np.random.default_rng(3)
+    ops = np.arange(11, dtype=np.int32)
+    bits, a, b, c = _encrypt_inputs(sk, rng, len(ops))
+    ref = T.run_gates(sk.cloud, ops, a, b, c)
+    H = T.HostCiphertexts(P, 4)
+    try:
+        for g, op in enumerate(ops):
+            H.set(np.stack([a[g], b[g], c[g], np.zeros_like(a[g])]))
+            out, x, y, z = H.item(3), H.item(0), H.item(1), H.item(2)
+            if op == T.MUX:
+                L.bootsMUX(out, x, y, z, sk.cloud)
+            else:
+                getattr(L, "boots" + T.GATES[op])(out, x, y, sk.cloud)
+            assert np.array_equal(H.get()[3], ref[g]), T.GATES[op]
+        # aliasing: result is the first input
+        H.set(np.stack([a[0], b[0], c[0], np.zeros_like(a[0])]))
+        L.bootsNAND(H.item(0), H.item(0), H.item(1), sk.cloud)
+        assert np.array_equal(H.get()[0], ref[0])
+    finally:
+        H.close()
+
+
+def test_bootsgate_batch_dedup_and_alias(p128):
+    """bootsGate_batch with shared inputs and outputs aliasing inputs."""
+    T, P, sk, keys = p128
+    L = T.lib()
+    rng = np.random.default_rng(9)
+    bits, a, b, c = _encrypt_inputs(sk, rng, 6)
+    H = T.HostCiphertexts(P, 6)
+    try:
+        H.set(a)
+        ops = np.array([0, 3, 10, 1, 2, 5], np.int32)
+        # gate g reads items (g, (g+1)%6[, (g+2)%6]) and writes item g: every output aliases an input
+        i0 = [H.item(g) for g in range(6)]
+        i1 = [H.item((g + 1) % 6) for g in range(6)]
+        i2 = [H.item((g + 2) % 6) for g in range(6)]
+        arr = lambda xs: (T.LSP * 6)(*xs)
+        before = H.get()
+        rc = L.bootsGate_batch(6, ops.ctypes.data_as(T.i32p), arr(i0), arr(i0), arr(i1), arr(i2), sk.cloud)
+        assert rc == 0, L.tfhe_amd_last_error()
+        got = H.get()
+        ref = T.run_gates(sk.cloud, ops, before, before[[1, 2, 3, 4, 5, 0]], before[[2, 3, 4, 5, 0, 1]])
+        assert np.array_equal(got, ref)
+    finally:
+        H.close()

@@ -1,0 +1,399 @@
+"""tfhe_amd: MI355X gate-bootstrapping engine behind the tfhe/tfhe C gate API.
+
+This module is a thin ctypes mirror of the C-ABI in include/tfhe_amd.h
+(libtfhe_amd.so). Every bootstrap and keyswitch runs in the library's HIP
+kernels; there is no CPU fallback here: `lib()` raises if the shared library
+was not built, and gate calls raise if the library reports an error.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libtfhe_amd.so")
+
+GATES = ["NAND", "AND", "OR", "XOR", "XNOR", "NOR", "ANDNY", "ANDYN", "ORNY", "ORYN", "MUX"]
+GATE_CODE = {g: i for i, g in enumerate(GATES)}
+MUX = GATE_CODE["MUX"]
+
+i32 = ctypes.c_int32
+i32p = ctypes.POINTER(ctypes.c_int32)
+vp = ctypes.c_void_p
+
+
+class LweParams(ctypes.Structure):
+    _fields_ = [("n", i32), ("alpha_min", ctypes.c_double), ("alpha_max", ctypes.c_double)]
+
+
+class TLweParams(ctypes.Structure):
+    _fields_ = [("N", i32), ("k", i32), ("alpha_min", ctypes.c_double), ("alpha_max", ctypes.c_double),
+                ("extracted_lweparams", LweParams)]
+
+
+class TGswParams(ctypes.Structure):
+    _fields_ = [("l", i32), ("Bgbit", i32), ("Bg", i32), ("halfBg", i32), ("maskMod", ctypes.c_uint32),
+                ("tlwe_params", ctypes.POINTER(TLweParams)), ("kpl", i32), ("h", i32p), ("offset", ctypes.c_uint32)]
+
+
+class ParameterSet(ctypes.Structure):
+    _fields_ = [("ks_t", i32), ("ks_basebit", i32), ("in_out_params", ctypes.POINTER(LweParams)),
+                ("tgsw_params", ctypes.POINTER(TGswParams))]
+
+
+class LweSample(ctypes.Structure):
+    _fields_ = [("a", i32p), ("b", i32), ("current_variance", ctypes.c_double)]
+
+
+class TorusPolynomial(ctypes.Structure):
+    _fields_ = [("N", i32), ("coefsT", i32p)]
+
+
+class TLweSample(ctypes.Structure):
+    _fields_ = [("a", ctypes.POINTER(TorusPolynomial)), ("b", ctypes.POINTER(TorusPolynomial)),
+                ("current_variance", ctypes.c_double), ("k", i32)]
+
+
+class TGswSampleFFT(ctypes.Structure):
+    _fields_ = [("all_samples", vp), ("sample", vp), ("k", i32), ("l", i32), ("owner", vp), ("row", i32)]
+
+
+class LweKeySwitchKey(ctypes.Structure):
+    _fields_ = [("n", i32), ("t", i32), ("basebit", i32), ("base", i32), ("out_params", vp),
+                ("ks0_raw", vp), ("ks1_raw", vp), ("ks", vp)]
+
+
+class LweBootstrappingKeyFFT(ctypes.Structure):
+    _fields_ = [("in_out_params", vp), ("bk_params", ctypes.POINTER(TGswParams)), ("accum_params", vp),
+                ("extract_params", vp), ("bkFFT", ctypes.POINTER(TGswSampleFFT)),
+                ("ks", ctypes.POINTER(LweKeySwitchKey)), ("dkey", vp), ("bk_coef", vp)]
+
+
+class CloudKeySet(ctypes.Structure):
+    _fields_ = [("params", ctypes.POINTER(ParameterSet)), ("bk", vp),
+                ("bkFFT", ctypes.POINTER(LweBootstrappingKeyFFT))]
+
+
+class SecretKeySet(ctypes.Structure):
+    _fields_ = [("params", ctypes.POINTER(ParameterSet)), ("lwe_key", vp), ("tgsw_key", vp), ("cloud", CloudKeySet)]
+
+
+PP = ctypes.POINTER(ParameterSet)
+SKP = ctypes.POINTER(SecretKeySet)
+CKP = ctypes.POINTER(CloudKeySet)
+LSP = ctypes.POINTER(LweSample)
+
+# name: (restype, argtypes)
+_SIGS = {
+    "new_default_gate_bootstrapping_parameters": (PP, [i32]),
+    "new_gate_bootstrapping_parameters": (PP, [i32, i32, i32, i32, i32, i32, i32, ctypes.c_double,
+                                                ctypes.c_double, ctypes.c_double]),
+    "delete_gate_bootstrapping_parameters": (None, [PP]),
+    "new_random_gate_bootstrapping_secret_keyset": (SKP, [PP]),
+    "delete_gate_bootstrapping_secret_keyset": (None, [SKP]),
+    "delete_gate_bootstrapping_cloud_keyset": (None, [CKP]),
+    "new_gate_bootstrapping_cloud_keyset_from_arrays": (CKP, [PP, i32p, i32p]),
+    "new_gate_bootstrapping_ciphertext": (LSP, [PP]),
+    "new_gate_bootstrapping_ciphertext_array": (LSP, [i32, PP]),
+    "delete_gate_bootstrapping_ciphertext": (None, [LSP]),
+    "delete_gate_bootstrapping_ciphertext_array": (None, [i32, LSP]),
+    "bootsSymEncrypt": (None, [LSP, i32, SKP]),
+    "bootsSymDecrypt": (i32, [LSP, SKP]),
+    "bootsCONSTANT": (None, [LSP, i32, CKP]),
+    "bootsNOT": (None, [LSP, LSP, CKP]),
+    "bootsCOPY": (None, [LSP, LSP, CKP]),
+    "bootsMUX": (None, [LSP, LSP, LSP, LSP, CKP]),
+    "bootsGate_batch": (i32, [i32, i32p, ctypes.POINTER(LSP), ctypes.POINTER(LSP), ctypes.POINTER(LSP),
+                              ctypes.POINTER(LSP), CKP]),
+    "tfhe_random_generator_setSeed": (None, [ctypes.POINTER(ctypes.c_uint32), i32]),
+    "tfhe_blindRotate_FFT": (None, [ctypes.POINTER(TLweSample), ctypes.POINTER(TGswSampleFFT), i32p, i32,
+                                    ctypes.POINTER(TGswParams)]),
+    "tfhe_blindRotateAndExtract_FFT": (None, [LSP, ctypes.POINTER(TorusPolynomial), ctypes.POINTER(TGswSampleFFT),
+                                              i32, i32p, i32, ctypes.POINTER(TGswParams)]),
+    "tfhe_bootstrap_woKS_FFT": (None, [LSP, ctypes.POINTER(LweBootstrappingKeyFFT), i32, LSP]),
+    "tfhe_bootstrap_FFT": (None, [LSP, ctypes.POINTER(LweBootstrappingKeyFFT), i32, LSP]),
+    "lweKeySwitch": (None, [LSP, ctypes.POINTER(LweKeySwitchKey), LSP]),
+    "lwePhase": (i32, [LSP, vp]),
+    "new_LweSample": (LSP, [ctypes.POINTER(LweParams)]),
+    "delete_LweSample": (None, [LSP]),
+    "new_TLweSample": (ctypes.POINTER(TLweSample), [ctypes.POINTER(TLweParams)]),
+    "delete_TLweSample": (None, [ctypes.POINTER(TLweSample)]),
+    "new_TorusPolynomial": (ctypes.POINTER(TorusPolynomial), [i32]),
+    "delete_TorusPolynomial": (None, [ctypes.POINTER(TorusPolynomial)]),
+    "modSwitchFromTorus32": (i32, [i32, i32]),
+    "modSwitchToTorus32": (i32, [i32, i32]),
+    "dtot32": (i32, [ctypes.c_double]),
+    "tfhe_amd_encrypt_bits": (i32, [SKP, i32, i32p, i32p]),
+    "tfhe_amd_phases": (i32, [SKP, i32, i32p, i32p]),
+    "tfhe_amd_export_keys": (i32, [SKP, i32p, i32p, i32p, i32p]),
+    "tfhe_amd_cloud_of": (CKP, [SKP]),
+    "tfhe_amd_cloud_key_upload": (i32, [CKP]),
+    "tfhe_amd_cloud_key_release": (None, [CKP]),
+    "tfhe_amd_cloud_key_device_bytes": (ctypes.c_size_t, [CKP]),
+    "tfhe_amd_gates_device": (i32, [CKP, i32, vp, vp, vp, vp, vp, vp, vp]),
+    "tfhe_amd_gates_device_woks": (i32, [CKP, i32, vp, vp, vp, vp, vp, vp, vp]),
+    "tfhe_amd_keyswitch_device": (i32, [CKP, i32, vp, vp, vp]),
+    "tfhe_amd_set_launch_config": (None, [i32, i32, i32]),
+    "tfhe_amd_get_launch_config": (i32, [i32p, i32p, i32p]),
+    "tfhe_amd_profile_enable": (None, [i32]),
+    "tfhe_amd_profile_read": (i32, [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double), i32p]),
+    "tfhe_amd_malloc": (vp, [ctypes.c_size_t]),
+    "tfhe_amd_free": (i32, [vp]),
+    "tfhe_amd_memcpy_h2d": (i32, [vp, vp, ctypes.c_size_t]),
+    "tfhe_amd_memcpy_d2h": (i32, [vp, vp, ctypes.c_size_t]),
+    "tfhe_amd_synchronize": (i32, []),
+    "tfhe_amd_stream": (vp, []),
+    "tfhe_amd_device_count": (i32, []),
+    "tfhe_amd_last_error": (ctypes.c_char_p, []),
+}
+for _g in GATES[:-1]:
+    _SIGS["boots" + _g] = (None, [LSP, LSP, LSP, CKP])
+
+_lib = None
+
+
+class TfheAmdError(RuntimeError):
+    pass
+
+
+def lib():
+    """Loads libtfhe_amd.so (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise TfheAmdError(f"{LIB_PATH} not built: run python -m tfhe_amd.build")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc, what="tfhe_amd call"):
+    if rc != 0:
+        raise TfheAmdError(f"{what} failed: {lib().tfhe_amd_last_error().decode()}")
+
+
+def _p(a):
+    assert a.dtype == np.int32 and a.flags.c_contiguous
+    return a.ctypes.data_as(i32p)
+
+
+def set_seed(seed):
+    lib().tfhe_random_generator_setSeed((ctypes.c_uint32 * 1)(seed), 1)
+
+
+class Params:
+    def __init__(self, ptr):
+        self.ptr = ptr
+        p = ptr.contents
+        g = p.tgsw_params.contents
+        t = g.tlwe_params.contents
+        self.n = p.in_out_params.contents.n
+        self.N, self.k, self.l, self.Bgbit = t.N, t.k, g.l, g.Bgbit
+        self.ks_t, self.ks_basebit = p.ks_t, p.ks_basebit
+        self.kpl = g.kpl
+
+
+def default_params(minimum_lambda=110):
+    return Params(lib().new_default_gate_bootstrapping_parameters(minimum_lambda))
+
+
+def big_params():
+    """N = 2048 set: n = 750, l = 2, Bgbit = 10, stdevs 2^-17 / 2^-44, keyswitch t = 8, basebit = 2."""
+    return Params(lib().new_gate_bootstrapping_parameters(750, 2048, 1, 2, 10, 8, 2, 2.0 ** -17, 2.0 ** -44,
+                                                          0.012467))
+
+
+class SecretKey:
+    """Wraps a TFheGateBootstrappingSecretKeySet (host keys; device key uploaded on first use)."""
+
+    def __init__(self, params, seed=None):
+        if seed is not None:
+            set_seed(seed)
+        self.params = params
+        self.ptr = lib().new_random_gate_bootstrapping_secret_keyset(params.ptr)
+        self.cloud = lib().tfhe_amd_cloud_of(self.ptr)
+
+    def encrypt(self, bits):
+        bits = np.ascontiguousarray(bits, np.int32)
+        out = np.zeros((len(bits), self.params.n + 1), np.int32)
+        check(lib().tfhe_amd_encrypt_bits(self.ptr, len(bits), _p(bits), _p(out)), "encrypt")
+        return out
+
+    def phases(self, cts):
+        cts = np.ascontiguousarray(cts, np.int32)
+        ph = np.zeros(len(cts), np.int32)
+        check(lib().tfhe_amd_phases(self.ptr, len(cts), _p(cts), _p(ph)), "phases")
+        return ph
+
+    def decrypt(self, cts):
+        return (self.phases(cts) > 0).astype(np.int32)
+
+    def export(self):
+        P = self.params
+        arrs = dict(lwe_key=np.zeros(P.n, np.int32), tlwe_key=np.zeros(P.k * P.N, np.int32),
+                    bk=np.zeros((P.n, P.kpl, P.k + 1, P.N), np.int32),
+                    ksk=np.zeros((P.k * P.N, P.ks_t, 1 << P.ks_basebit, P.n + 1), np.int32))
+        check(lib().tfhe_amd_export_keys(self.ptr, _p(arrs["lwe_key"]), _p(arrs["tlwe_key"]), _p(arrs["bk"]),
+                                         _p(arrs["ksk"])), "export_keys")
+        return arrs
+
+    def upload(self):
+        check(lib().tfhe_amd_cloud_key_upload(self.cloud), "cloud key upload")
+
+    def close(self):
+        if self.ptr:
+            lib().delete_gate_bootstrapping_secret_keyset(self.ptr)
+            self.ptr = None
+
+
+def cloud_from_arrays(params, bk, ksk):
+    bk = np.ascontiguousarray(bk, np.int32)
+    ksk = np.ascontiguousarray(ksk, np.int32)
+    ck = lib().new_gate_bootstrapping_cloud_keyset_from_arrays(params.ptr, _p(bk), _p(ksk))
+    if not ck:
+        raise TfheAmdError(lib().tfhe_amd_last_error().decode())
+    return ck
+
+
+class DeviceArray:
+    """int32 device buffer allocated through the library."""
+
+    def __init__(self, shape):
+        self.shape = tuple(shape) if hasattr(shape, "__len__") else (shape,)
+        self.nbytes = int(np.prod(self.shape)) * 4
+        self.ptr = lib().tfhe_amd_malloc(max(self.nbytes, 4))
+        if not self.ptr:
+            raise TfheAmdError(lib().tfhe_amd_last_error().decode())
+
+    @classmethod
+    def from_numpy(cls, a):
+        a = np.ascontiguousarray(a, np.int32)
+        d = cls(a.shape)
+        check(lib().tfhe_amd_memcpy_h2d(d.ptr, a.ctypes.data_as(vp), d.nbytes), "h2d")
+        return d
+
+    def numpy(self):
+        a = np.zeros(self.shape, np.int32)
+        check(lib().tfhe_amd_memcpy_d2h(a.ctypes.data_as(vp), self.ptr, self.nbytes), "d2h")
+        return a
+
+    def free(self):
+        if self.ptr:
+            lib().tfhe_amd_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def gates_device(cloud, ops, in0, in1, in2, out, pool, stream=None):
+    """Asynchronous device gates over a device pool (all DeviceArray)."""
+    count = ops.shape[0]
+    check(lib().tfhe_amd_gates_device(cloud, count, ops.ptr, in0.ptr, in1.ptr, in2.ptr, out.ptr, pool.ptr, stream),
+          "gates_device")
+
+
+def profile_enable(on=True):
+    lib().tfhe_amd_profile_enable(1 if on else 0)
+
+
+def profile_read():
+    """(blind-rotation ms, keyswitch ms, batches) summed since the last read."""
+    br, ks, n = ctypes.c_double(), ctypes.c_double(), ctypes.c_int32()
+    check(lib().tfhe_amd_profile_read(ctypes.byref(br), ctypes.byref(ks), ctypes.byref(n)), "profile_read")
+    return br.value, ks.value, n.value
+
+
+def synchronize():
+    check(lib().tfhe_amd_synchronize(), "synchronize")
+
+
+def run_gates(cloud, ops, a, b, c=None):
+    """Evaluate gates on host ciphertext arrays [count][n+1] through the device
+    API (one pool upload, one launch sequence, one download)."""
+    ops = np.ascontiguousarray(ops, np.int32)
+    count = len(ops)
+    w = a.shape[1]
+    c = a if c is None else c
+    pool = np.concatenate([a, b, c, np.zeros((count, w), np.int32)]).astype(np.int32)
+    idx = np.arange(count, dtype=np.int32)
+    d_pool = DeviceArray.from_numpy(pool)
+    d_ops = DeviceArray.from_numpy(ops)
+    d_i0 = DeviceArray.from_numpy(idx)
+    d_i1 = DeviceArray.from_numpy(idx + count)
+    d_i2 = DeviceArray.from_numpy(idx + 2 * count)
+    d_o = DeviceArray.from_numpy(idx + 3 * count)
+    gates_device(cloud, d_ops, d_i0, d_i1, d_i2, d_o, d_pool)
+    synchronize()
+    out = d_pool.numpy()[3 * count:]
+    for d in (d_pool, d_ops, d_i0, d_i1, d_i2, d_o):
+        d.free()
+    return out
+
+
+def run_gates_woks(cloud, params, ops, a, b, c=None):
+    """Bootstrap without keyswitch: extracted samples [count][k*N+1]."""
+    ops = np.ascontiguousarray(ops, np.int32)
+    count = len(ops)
+    c = a if c is None else c
+    pool = np.concatenate([a, b, c]).astype(np.int32)
+    idx = np.arange(count, dtype=np.int32)
+    d_pool = DeviceArray.from_numpy(pool)
+    d_ops = DeviceArray.from_numpy(ops)
+    d_i0 = DeviceArray.from_numpy(idx)
+    d_i1 = DeviceArray.from_numpy(idx + count)
+    d_i2 = DeviceArray.from_numpy(idx + 2 * count)
+    d_ext = DeviceArray((count, params.k * params.N + 1))
+    check(lib().tfhe_amd_gates_device_woks(cloud, count, d_ops.ptr, d_i0.ptr, d_i1.ptr, d_i2.ptr, d_pool.ptr,
+                                           d_ext.ptr, None), "gates_device_woks")
+    synchronize()
+    return d_ext.numpy()
+
+
+def keyswitch(cloud, params, ext):
+    ext = np.ascontiguousarray(ext, np.int32)
+    count = ext.shape[0]
+    d_in = DeviceArray.from_numpy(ext)
+    d_out = DeviceArray((count, params.n + 1))
+    check(lib().tfhe_amd_keyswitch_device(cloud, count, d_in.ptr, d_out.ptr, None), "keyswitch_device")
+    synchronize()
+    return d_out.numpy()
+
+
+class HostCiphertexts:
+    """An LweSample array allocated by the library, with numpy views."""
+
+    def __init__(self, params, count):
+        self.params, self.count = params, count
+        self.ptr = lib().new_gate_bootstrapping_ciphertext_array(count, params.ptr)
+
+    def item(self, i):
+        return ctypes.pointer(self.ptr[i])
+
+    def set(self, arr):
+        n = self.params.n
+        for i in range(self.count):
+            s = self.ptr[i]
+            ctypes.memmove(s.a, arr[i].ctypes.data, 4 * n)
+            s.b = int(arr[i][n])
+
+    def get(self):
+        n = self.params.n
+        out = np.zeros((self.count, n + 1), np.int32)
+        for i in range(self.count):
+            s = self.ptr[i]
+            ctypes.memmove(out[i].ctypes.data, s.a, 4 * n)
+            out[i][n] = s.b
+        return out
+
+    def close(self):
+        if self.ptr:
+            lib().delete_gate_bootstrapping_ciphertext_array(self.count, self.ptr)
+            self.ptr = None

@@ -1,0 +1,49 @@
+"""Builds tfhe_amd/libtfhe_amd.so (HIP for gfx950 + host C++), in-tree.
+
+    python -m tfhe_amd.build            # or tfhe_amd.build.build()
+"""
+import os
+import shutil
+import subprocess
+import sys
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+CSRC = os.path.join(PKG, "csrc")
+OUT = os.path.join(PKG, "libtfhe_amd.so")
+BUILD = os.path.join(PKG, "_build")
+
+ARCH = os.environ.get("TFHE_AMD_ARCH", "gfx950")
+HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
+CXX = os.environ.get("CXX_HOST", shutil.which("g++") or "g++")
+
+# Packed FP32 instructions are disabled for every HIP translation unit.
+HIP_FLAGS = [
+    "-O3", "-std=c++17", "-fPIC", "-fvisibility=hidden", f"--offload-arch={ARCH}",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall", "-Wno-unused-function",
+]
+HOST_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-fvisibility=hidden", "-Wall", "-Wextra", "-Wno-unused-parameter"]
+
+
+def _run(cmd):
+    print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+
+
+def build(verbose_resources=False):
+    os.makedirs(BUILD, exist_ok=True)
+    eng_o = os.path.join(BUILD, "engine.o")
+    host_o = os.path.join(BUILD, "host.o")
+    extra = ["-Rpass-analysis=kernel-resource-usage"] if verbose_resources else []
+    _run([HIPCC] + HIP_FLAGS + extra + ["-c", os.path.join(CSRC, "engine.hip"), "-o", eng_o])
+    _run([CXX] + HOST_FLAGS + ["-c", os.path.join(CSRC, "host.cpp"), "-o", host_o])
+    tmp = OUT + ".tmp"
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, eng_o, host_o,
+          "-Wl,-Bsymbolic", "-Wl,--no-undefined", "-lpthread"])
+    os.replace(tmp, OUT)
+    return OUT
+
+
+if __name__ == "__main__":
+    build(verbose_resources="-v" in sys.argv)

@@ -1,0 +1,42 @@
+// Internal interface between the host library (host.cpp) and the GPU engine
+// (engine.hip). Not part of the public C-ABI.
+#pragma once
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "../../include/tfhe_amd.h"
+
+namespace tfhe_amd {
+
+// Gate codes beyond the public TFHE_AMD_* ones, used by the subsystem entry points.
+constexpr int OP_IDENTITY = 11;  // bootstrap the input sample as is (tfhe_bootstrap_FFT)
+
+struct HostParams {
+    int32_t n, N, k, l, Bgbit, ks_t, ks_basebit;
+};
+
+HostParams host_params(const TFheGateBootstrappingParameterSet* p);
+HostParams host_params_fft(const LweBootstrappingKeyFFT* bf);
+
+// Records an error message (thread-local) and returns -1.
+int set_error(const char* fmt, ...);
+[[noreturn]] void die(const char* what);
+
+// Device key of a transform-domain key; uploads on first use. Returns nullptr on error.
+tfhe_amd_device_key* engine_key(const LweBootstrappingKeyFFT* bf);
+
+// Gates on host ciphertexts (stages through pinned memory). in2 may be null.
+int engine_gates_host(const LweBootstrappingKeyFFT* bf, int32_t count, const int32_t* ops,
+                      LweSample* const* out, const LweSample* const* in0, const LweSample* const* in1,
+                      const LweSample* const* in2, int32_t mu, bool keyswitch);
+
+// Subsystem entry points on host buffers.
+// acc: (k+1)*N int32 in/out, bara: n entries, rows [row0, row0+n) of the key.
+int engine_blind_rotate_host(tfhe_amd_device_key* dk, int32_t row0, int32_t n, const int32_t* bara, int32_t* acc,
+                             int32_t* ext_out);
+int engine_keyswitch_host(tfhe_amd_device_key* dk, const int32_t* in, int32_t* out);
+
+void engine_forget_host_key(const LweBootstrappingKeyFFT* bkfft);
+
+}  // namespace tfhe_amd

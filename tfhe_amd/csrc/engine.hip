@@ -1,0 +1,941 @@
+// GPU engine for TFHE gate bootstrapping on MI355X (gfx950).
+//
+// Kernels:
+//   bk_convert_kernel  coefficient-domain bootstrapping key -> FP64 transform
+//                      domain (replaces init_LweBootstrappingKeyFFT,
+//                      libtfhe/lwe-bootstrapping-functions-fft.cpp:23-52)
+//   br_kernel          one bootstrap per 64-lane wavefront: gate linear form +
+//                      modulus switch, test-vector rotation, n external
+//                      products with the accumulator in LDS, sample extraction
+//                      (tfhe_bootstrap_woKS_FFT / tfhe_blindRotateAndExtract_FFT /
+//                      tfhe_blindRotate_FFT, lwe-bootstrapping-functions-fft.cpp:67-200)
+//   ks_prep_kernel     sums the two halves of MUX gates and transposes the
+//                      extracted masks for the keyswitch
+//   ks_kernel          exact int32 keyswitch, lanes over output coefficients,
+//                      a tile of gates per block so each key row read from
+//                      HBM/L2 serves the whole tile (lweKeySwitch,
+//                      lwe-keyswitch-functions.cpp:114-129,228-238)
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <unordered_map>
+#include <vector>
+
+#include "engine.h"
+#include "negacyclic_fft.h"
+
+using tfhe_amd::c64;
+
+struct tfhe_amd_device_key {
+    int device;
+    tfhe_amd::HostParams P;
+    int M, kpl, npad;
+    c64* bkf;      // [n][kpl][k+1][M]     FP64 transform of every TGSW row polynomial
+    int32_t* ksk;  // [k*N][t][base-1][npad]  non-trivial keyswitch rows, coefficient b at column n
+    c64* tw;       // TwiddleTable<M>::COUNT
+    size_t bytes;
+};
+
+namespace tfhe_amd {
+
+#define HIP_TRY(call)                                                                          \
+    do {                                                                                       \
+        hipError_t e_ = (call);                                                                \
+        if (e_ != hipSuccess) return set_error("%s: %s", #call, hipGetErrorString(e_));       \
+    } while (0)
+
+// ---------------------------------------------------------------- kernels --
+
+struct BrArgs {
+    const int32_t* op;
+    const int32_t* in0;
+    const int32_t* in1;
+    const int32_t* in2;
+    const int32_t* pool;
+    int32_t count;  // gates; the grid has 2*count blocks, the second half serves MUX gates
+    int32_t mu;
+    const int32_t* bara_in;  // explicit mode (one job): n modulus-switched coefficients
+    const int32_t* acc_in;   // explicit mode: initial accumulator [(k+1)][N]
+    int32_t* acc_out;        // explicit mode: final accumulator
+    int32_t* ext;            // extracted samples [2*count][kN+1]
+    const c64* bkf;          // key rows, already offset to the first row used
+    const c64* tw;
+    int32_t n;
+    int32_t Bgbit;
+    uint32_t offset;  // gadget offset, libtfhe/tgsw.cpp:21-27
+};
+
+// round to nearest, result mod 2^32 (valid for |v| < 2^51)
+__device__ __forceinline__ int32_t round_mod32(double v) {
+    const double t = v + 6755399441055744.0;  // 1.5 * 2^52
+    return (int32_t)(uint32_t)(unsigned long long)__double_as_longlong(t);
+}
+
+// Linear form applied to the gate inputs before the bootstrap
+// (libtfhe/boot-gates.cpp:34-319; MUX halves :342-353).
+__device__ __forceinline__ void gate_coeffs(int op, int half, int32_t& k0, int32_t& ca, int32_t& cb, int32_t& cc) {
+    const int32_t e8 = 1 << 29, e4 = 1 << 30;
+    cc = 0;
+    switch (op) {
+        case TFHE_AMD_NAND: k0 = e8; ca = -1; cb = -1; break;
+        case TFHE_AMD_AND: k0 = -e8; ca = 1; cb = 1; break;
+        case TFHE_AMD_OR: k0 = e8; ca = 1; cb = 1; break;
+        case TFHE_AMD_XOR: k0 = e4; ca = 2; cb = 2; break;
+        case TFHE_AMD_XNOR: k0 = -e4; ca = -2; cb = -2; break;
+        case TFHE_AMD_NOR: k0 = -e8; ca = -1; cb = -1; break;
+        case TFHE_AMD_ANDNY: k0 = -e8; ca = -1; cb = 1; break;
+        case TFHE_AMD_ANDYN: k0 = -e8; ca = 1; cb = -1; break;
+        case TFHE_AMD_ORNY: k0 = e8; ca = -1; cb = 1; break;
+        case TFHE_AMD_ORYN: k0 = e8; ca = 1; cb = -1; break;
+        case TFHE_AMD_MUX:
+            k0 = -e8;
+            if (half == 0) { ca = 1; cb = 1; } else { ca = -1; cb = 0; cc = 1; }
+            break;
+        default: k0 = 0; ca = 1; cb = 0; break;  // OP_IDENTITY
+    }
+}
+
+template <int M, int L>
+__global__ __launch_bounds__(64, 2) void br_kernel(BrArgs A) {
+    using F = Fft<M>;
+    constexpr int R = F::R, N = 2 * M, KPL = 2 * L;
+    constexpr int LOG2N2 = (M == 512) ? 11 : 12;  // log2(2N)
+    __shared__ int32_t s_acc[2 * N];
+    __shared__ c64 s_fft[F::LDS_C64];
+    __shared__ int16_t s_bara[1024];
+    __shared__ int32_t s_barb;
+
+    const int lane = threadIdx.x;
+    const int job = blockIdx.x;
+    const int n = A.n;
+
+    if (A.bara_in == nullptr) {
+        int gate = job, half = 0;
+        if (job >= A.count) { gate = job - A.count; half = 1; }
+        const int op = A.op[gate];
+        if (half && op != TFHE_AMD_MUX) return;
+        int32_t k0, ca, cb, cc;
+        gate_coeffs(op, half, k0, ca, cb, cc);
+        const int32_t* a = A.pool + (size_t)A.in0[gate] * (n + 1);
+        const int32_t* b = (op == OP_IDENTITY) ? a : A.pool + (size_t)A.in1[gate] * (n + 1);
+        const int32_t* c = (op == TFHE_AMD_MUX) ? A.pool + (size_t)A.in2[gate] * (n + 1) : a;
+        for (int i = lane; i <= n; i += 64) {
+            uint32_t v = (uint32_t)ca * (uint32_t)a[i] + (uint32_t)cb * (uint32_t)b[i] + (uint32_t)cc * (uint32_t)c[i];
+            if (i == n) v += (uint32_t)k0;
+            // modSwitchFromTorus32(v, 2N), libtfhe/numeric-functions.cpp:59-65
+            const uint32_t bar = (v + (1u << (31 - LOG2N2))) >> (32 - LOG2N2);
+            if (i < n) s_bara[i] = (int16_t)bar;
+            else s_barb = (int32_t)bar;
+        }
+        __syncthreads();
+        const int barb = s_barb;
+        // ACC = (0, X^{2N-barb} * [mu...mu]), lwe-bootstrapping-functions-fft.cpp:146-148
+        for (int j = lane; j < N; j += 64) {
+            s_acc[j] = 0;
+            s_acc[N + j] = (((j + barb) & (2 * N - 1)) < N) ? A.mu : -A.mu;
+        }
+    } else {
+        for (int i = lane; i < n; i += 64) s_bara[i] = (int16_t)A.bara_in[i];
+        for (int j = lane; j < 2 * N; j += 64) s_acc[j] = A.acc_in[j];
+    }
+    __syncthreads();
+
+    c64 tw1[R], twist[R], itwist[R];
+#pragma unroll
+    for (int k = 0; k < R; k++) tw1[k] = A.tw[lane * (R + 8) + k];
+    const c64 tw2 = A.tw[lane * (R + 8) + R + 1];
+#pragma unroll
+    for (int a = 0; a < R; a++) {
+        twist[a] = A.tw[64 * (R + 8) + a];
+        itwist[a] = A.tw[64 * (R + 8) + R + a];
+    }
+    const uint32_t offset = A.offset;
+    const int Bgbit = A.Bgbit;
+    const uint32_t dmask = (1u << Bgbit) - 1u;
+    const int32_t dhalf = 1 << (Bgbit - 1);
+
+    for (int i = 0; i < n; i++) {
+        const int abar = s_bara[i];
+        if (abar == 0) continue;  // lwe-bootstrapping-functions-fft.cpp:101
+        c64 r0[R], r1[R];
+#pragma unroll
+        for (int s = 0; s < R; s++) r0[s] = r1[s] = c64{0.0, 0.0};
+        const c64* bkrow = A.bkf + (size_t)i * KPL * 2 * M + lane;
+#pragma unroll 1
+        for (int c = 0; c < 2; c++) {
+            const int32_t* acc = s_acc + c * N;
+            // T = (X^abar - 1) * ACC_c at this lane's 2R coefficients
+            uint32_t tv[2 * R];
+#pragma unroll
+            for (int h = 0; h < 2; h++)
+#pragma unroll
+                for (int s = 0; s < R; s++) {
+                    const int j = 64 * s + lane + h * M;
+                    const int idx = (j - abar) & (2 * N - 1);
+                    const int32_t rot = idx < N ? acc[idx] : -acc[idx - N];
+                    tv[h * R + s] = (uint32_t)rot - (uint32_t)acc[j] + offset;
+                }
+#pragma unroll 1
+            for (int p = 0; p < L; p++) {
+                // digit p of the truncating gadget decomposition, tgsw-functions.cpp:337-348
+                const int shift = 32 - (p + 1) * Bgbit;
+                c64 x[R];
+#pragma unroll
+                for (int s = 0; s < R; s++) {
+                    const int32_t d0 = (int32_t)((tv[s] >> shift) & dmask) - dhalf;
+                    const int32_t d1 = (int32_t)((tv[R + s] >> shift) & dmask) - dhalf;
+                    x[s] = c64{(double)d0, (double)d1};
+                }
+                F::fwd1(x, lane, s_fft, twist, tw1);
+                __syncthreads();
+                F::fwd2(x, lane, s_fft, tw2);
+                __syncthreads();
+                F::fwd3(x, lane, s_fft);
+                __syncthreads();
+                F::fwd4(x, lane, s_fft);
+                const c64* bk = bkrow + (size_t)(c * L + p) * 2 * M;
+#pragma unroll
+                for (int s = 0; s < R; s++) {
+                    cmac(r0[s], x[s], bk[s * 64]);
+                    cmac(r1[s], x[s], bk[M + s * 64]);
+                }
+                __syncthreads();
+            }
+        }
+        // inverse transforms and ACC += result (libtfhe/lwe-bootstrapping-functions-fft.cpp:67-76)
+#pragma unroll
+        for (int cc = 0; cc < 2; cc++) {
+            c64* x = cc ? r1 : r0;
+            F::inv1(x, lane, s_fft);
+            __syncthreads();
+            F::inv2(x, lane, s_fft, tw2);
+            __syncthreads();
+            F::inv3(x, lane, s_fft);
+            __syncthreads();
+            F::inv4(x, lane, s_fft, tw1, itwist);
+            int32_t* acc = s_acc + cc * N;
+#pragma unroll
+            for (int s = 0; s < R; s++) {
+                const int j = 64 * s + lane;
+                acc[j] = (int32_t)((uint32_t)acc[j] + (uint32_t)round_mod32(x[s].re));
+                acc[j + M] = (int32_t)((uint32_t)acc[j + M] + (uint32_t)round_mod32(x[s].im));
+            }
+            __syncthreads();
+        }
+    }
+
+    if (A.acc_out) {
+        for (int j = lane; j < 2 * N; j += 64) A.acc_out[j] = s_acc[j];
+    }
+    if (A.ext) {
+        // tLweExtractLweSampleIndex(.., 0, ..), libtfhe/lwe.cpp:41-56
+        int32_t* e = A.ext + (size_t)job * (N + 1);
+        for (int m = lane; m < N; m += 64) e[m] = (m == 0) ? s_acc[0] : (int32_t)(0u - (uint32_t)s_acc[N - m]);
+        if (lane == 0) e[N] = s_acc[N];
+    }
+}
+
+template <int M>
+__global__ __launch_bounds__(64) void bk_convert_kernel(const int32_t* __restrict__ bk, c64* __restrict__ bkf,
+                                                        const c64* __restrict__ tw) {
+    using F = Fft<M>;
+    constexpr int R = F::R;
+    __shared__ c64 s_fft[F::LDS_C64];
+    const int lane = threadIdx.x;
+    const size_t poly = blockIdx.x;
+    const int32_t* p = bk + poly * 2 * M;
+    c64 x[R], tw1[R], twist[R];
+#pragma unroll
+    for (int k = 0; k < R; k++) tw1[k] = tw[lane * (R + 8) + k];
+    const c64 tw2 = tw[lane * (R + 8) + R + 1];
+#pragma unroll
+    for (int a = 0; a < R; a++) twist[a] = tw[64 * (R + 8) + a];
+#pragma unroll
+    for (int s = 0; s < R; s++) x[s] = c64{(double)p[64 * s + lane], (double)p[64 * s + lane + M]};
+    F::fwd1(x, lane, s_fft, twist, tw1);
+    __syncthreads();
+    F::fwd2(x, lane, s_fft, tw2);
+    __syncthreads();
+    F::fwd3(x, lane, s_fft);
+    __syncthreads();
+    F::fwd4(x, lane, s_fft);
+#pragma unroll
+    for (int s = 0; s < R; s++) bkf[poly * M + s * 64 + lane] = x[s];
+}
+
+// ext: [2*count][nin+1]; op may be null (no MUX). Writes abar[i][cpad] = a_i + prec_offset
+// and binit[g] = b; if out_init is set, also writes the keyswitch start row
+// (0,...,0,b) of every gate (needed when the keyswitch accumulates atomically).
+__global__ __launch_bounds__(256) void ks_prep_kernel(const int32_t* __restrict__ ext, const int32_t* __restrict__ op,
+                                                      int count, int nin, int32_t mu, uint32_t prec_offset,
+                                                      uint32_t* __restrict__ abar, int cpad,
+                                                      int32_t* __restrict__ binit, int32_t* out, const int32_t* out_idx,
+                                                      int n) {
+    const int g = blockIdx.x;
+    const bool mux = op && op[g] == TFHE_AMD_MUX;
+    const int32_t* e0 = ext + (size_t)g * (nin + 1);
+    const int32_t* e1 = ext + (size_t)(count + g) * (nin + 1);
+    for (int i = threadIdx.x; i <= nin; i += blockDim.x) {
+        uint32_t v = (uint32_t)e0[i];
+        // libtfhe/boot-gates.cpp:358-361: (0, 1/8) + u1 + u2
+        if (mux) v += (uint32_t)e1[i] + (i == nin ? (uint32_t)mu : 0u);
+        if (i < nin) abar[(size_t)i * cpad + g] = v + prec_offset;
+        else binit[g] = (int32_t)v;
+    }
+    if (out) {
+        int32_t* o = out + (size_t)(out_idx ? out_idx[g] : g) * (n + 1);
+        __syncthreads();
+        for (int c = threadIdx.x; c <= n; c += blockDim.x) o[c] = (c == n) ? binit[g] : 0;
+    }
+}
+
+__global__ __launch_bounds__(256) void sum_halves_kernel(const int32_t* ext, const int32_t* op, int count, int nin,
+                                                         int32_t mu, int32_t* out) {
+    const int g = blockIdx.x;
+    const bool mux = op && op[g] == TFHE_AMD_MUX;
+    const int32_t* e0 = ext + (size_t)g * (nin + 1);
+    const int32_t* e1 = ext + (size_t)(count + g) * (nin + 1);
+    int32_t* o = out + (size_t)g * (nin + 1);
+    for (int i = threadIdx.x; i <= nin; i += blockDim.x) {
+        uint32_t v = (uint32_t)e0[i];
+        if (mux) v += (uint32_t)e1[i] + (i == nin ? (uint32_t)mu : 0u);
+        o[i] = (int32_t)v;
+    }
+}
+
+template <int G, bool ATOMIC>
+__global__ __launch_bounds__(64) void ks_kernel(const int32_t* __restrict__ ksk, const uint32_t* __restrict__ abar,
+                                                int cpad, const int32_t* __restrict__ binit, int count, int nin, int t,
+                                                int n, int npad, int i_per_split, int32_t* out,
+                                                const int32_t* __restrict__ out_idx) {
+    const int lane = threadIdx.x;
+    const int col = blockIdx.x * 64 + lane;
+    const int g0 = blockIdx.y * G;
+    const int i0 = blockIdx.z * i_per_split;
+    const int i1 = min(nin, i0 + i_per_split);
+    int32_t acc[G];
+#pragma unroll
+    for (int gg = 0; gg < G; gg++) acc[gg] = 0;
+    for (int i = i0; i < i1; i++) {
+        uint32_t ab[G];
+#pragma unroll
+        for (int gg = 0; gg < G; gg++) ab[gg] = abar[(size_t)i * cpad + g0 + gg];
+        for (int j = 0; j < t; j++) {
+            const int32_t* row = ksk + ((size_t)(i * t + j) * 3) * npad + col;
+            const int32_t v1 = row[0], v2 = row[npad], v3 = row[2 * npad];
+            const int shift = 30 - 2 * j;
+#pragma unroll
+            for (int gg = 0; gg < G; gg++) {
+                const uint32_t d = (ab[gg] >> shift) & 3u;
+                const int32_t v = (d & 2u) ? ((d & 1u) ? v3 : v2) : ((d & 1u) ? v1 : 0);
+                acc[gg] = (int32_t)((uint32_t)acc[gg] - (uint32_t)v);
+            }
+        }
+    }
+    if (col > n) return;
+#pragma unroll
+    for (int gg = 0; gg < G; gg++) {
+        const int g = g0 + gg;
+        if (g >= count) break;
+        int32_t* dst = out + (size_t)(out_idx ? out_idx[g] : g) * (n + 1) + col;
+        if (ATOMIC) atomicAdd(dst, acc[gg]);
+        else *dst = (int32_t)((col == n ? (uint32_t)binit[g] : 0u) + (uint32_t)acc[gg]);
+    }
+}
+
+// ---------------------------------------------------------------- runtime --
+
+namespace {
+
+struct Scratch {
+    void* d = nullptr;
+    size_t cap = 0;
+};
+
+struct Ctx {
+    std::mutex mu;
+    bool ready = false;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    Scratch scratch;
+    void* h_pinned = nullptr;
+    size_t h_cap = 0;
+    int br_mode = 0, br_bpc = 0, ks_g = 0;
+    // optional per-launch timing with HIP events on the launch stream
+    bool prof = false;
+    struct Ev { hipEvent_t e[3]; };
+    std::vector<Ev> ev;
+};
+
+Ctx& ctx() {
+    static Ctx c;
+    return c;
+}
+
+int ensure_ready(Ctx& C) {
+    if (C.ready) return 0;
+    int ndev = 0;
+    HIP_TRY(hipGetDeviceCount(&ndev));
+    if (ndev < 1) return set_error("no GPU device visible");
+    HIP_TRY(hipGetDevice(&C.device));
+    HIP_TRY(hipStreamCreateWithFlags(&C.stream, hipStreamNonBlocking));
+    C.ready = true;
+    return 0;
+}
+
+int ensure_scratch(Ctx& C, size_t bytes) {
+    if (C.scratch.cap >= bytes) return 0;
+    if (C.scratch.d) {
+        // work queued on any stream may still use the old buffer
+        HIP_TRY(hipDeviceSynchronize());
+        HIP_TRY(hipFree(C.scratch.d));
+        C.scratch.d = nullptr;
+        C.scratch.cap = 0;
+    }
+    size_t cap = bytes + bytes / 2 + (1 << 20);
+    HIP_TRY(hipMalloc(&C.scratch.d, cap));
+    C.scratch.cap = cap;
+    return 0;
+}
+
+int ensure_pinned(Ctx& C, size_t bytes) {
+    if (C.h_cap >= bytes) return 0;
+    if (C.h_pinned) {
+        HIP_TRY(hipStreamSynchronize(C.stream));
+        HIP_TRY(hipHostFree(C.h_pinned));
+        C.h_pinned = nullptr;
+        C.h_cap = 0;
+    }
+    size_t cap = bytes + bytes / 2 + (1 << 20);
+    HIP_TRY(hipHostMalloc(&C.h_pinned, cap, hipHostMallocDefault));
+    C.h_cap = cap;
+    return 0;
+}
+
+inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+uint32_t gadget_offset(const HostParams& P) {
+    uint32_t o = 0;
+    for (int i = 0; i < P.l; i++) o += 1u << (32 - (i + 1) * P.Bgbit);
+    return o * (1u << (P.Bgbit - 1));
+}
+
+int check_supported(const HostParams& P) {
+    if (P.k != 1) return set_error("only k = 1 is supported (got k = %d)", P.k);
+    if (P.N != 1024 && P.N != 2048) return set_error("only N = 1024 or 2048 is supported (got %d)", P.N);
+    if (!((P.N == 1024 && P.l == 3) || (P.N == 2048 && P.l == 2) || (P.N == 1024 && P.l == 2) || (P.N == 2048 && P.l == 3)))
+        return set_error("unsupported gadget length l = %d", P.l);
+    if (P.n > 1024 || P.n < 1) return set_error("LWE dimension n = %d out of range [1, 1024]", P.n);
+    if (P.ks_basebit != 2) return set_error("keyswitch basebit must be 2 (got %d)", P.ks_basebit);
+    if (P.ks_t * P.ks_basebit > 31) return set_error("keyswitch t*basebit must be < 32");
+    if (P.Bgbit * P.l > 32 || P.Bgbit < 1) return set_error("bad gadget base");
+    return 0;
+}
+
+template <int M>
+void launch_br_t(const BrArgs& A, int blocks, int l, hipStream_t s) {
+    if (l == 3) hipLaunchKernelGGL((br_kernel<M, 3>), dim3(blocks), dim3(64), 0, s, A);
+    else hipLaunchKernelGGL((br_kernel<M, 2>), dim3(blocks), dim3(64), 0, s, A);
+}
+
+void launch_br(const tfhe_amd_device_key* dk, const BrArgs& A, int blocks, hipStream_t s) {
+    if (dk->M == 512) launch_br_t<512>(A, blocks, dk->P.l, s);
+    else launch_br_t<1024>(A, blocks, dk->P.l, s);
+}
+
+BrArgs base_args(const tfhe_amd_device_key* dk) {
+    BrArgs A;
+    memset(&A, 0, sizeof A);
+    A.bkf = dk->bkf;
+    A.tw = dk->tw;
+    A.n = dk->P.n;
+    A.Bgbit = dk->P.Bgbit;
+    A.offset = gadget_offset(dk->P);
+    A.mu = 1 << 29;
+    return A;
+}
+
+template <int G, bool AT>
+void launch_ks_g(const tfhe_amd_device_key* dk, const uint32_t* abar, int cpad, const int32_t* binit, int count,
+                 int splits, int32_t* out, const int32_t* out_idx, hipStream_t s) {
+    const int nin = dk->P.k * dk->P.N;
+    const int ips = (nin + splits - 1) / splits;
+    dim3 grid((dk->P.n + 1 + 63) / 64, (count + G - 1) / G, splits);
+    hipLaunchKernelGGL((ks_kernel<G, AT>), grid, dim3(64), 0, s, dk->ksk, abar, cpad, binit, count, nin, dk->P.ks_t,
+                       dk->P.n, dk->npad, ips, out, out_idx);
+}
+
+// Keyswitch of ext rows (sum of halves for MUX) into out rows. Uses the
+// region of scratch starting at `scr`.
+int run_keyswitch(Ctx& C, const tfhe_amd_device_key* dk, const int32_t* ext, const int32_t* d_op, int count,
+                  int32_t* out, const int32_t* out_idx, char* scr, hipStream_t s) {
+    const HostParams& P = dk->P;
+    const int nin = P.k * P.N;
+    int G = C.ks_g;
+    if (G <= 0) G = count >= 512 ? 32 : (count >= 32 ? 8 : 1);
+    const int cpad = ((count + 31) / 32) * 32 + 32;
+    uint32_t* abar = (uint32_t*)scr;
+    int32_t* binit = (int32_t*)(scr + align256(sizeof(uint32_t) * (size_t)nin * cpad));
+    // enough blocks to fill the device: split the input range when there are few gate tiles
+    const int col_tiles = (P.n + 1 + 63) / 64;
+    const int tiles = col_tiles * ((count + G - 1) / G);
+    int splits = 1;
+    while (tiles * splits < 1024 && splits < 64) splits *= 2;
+    const bool atomic = splits > 1;
+    const uint32_t prec_offset = 1u << (32 - (1 + P.ks_basebit * P.ks_t));
+    hipLaunchKernelGGL(ks_prep_kernel, dim3(count), dim3(256), 0, s, ext, d_op, count, nin, 1 << 29, prec_offset,
+                       abar, cpad, binit, atomic ? out : nullptr, out_idx, P.n);
+    if (G == 32) { if (atomic) launch_ks_g<32, true>(dk, abar, cpad, binit, count, splits, out, out_idx, s); else launch_ks_g<32, false>(dk, abar, cpad, binit, count, splits, out, out_idx, s); }
+    else if (G == 8) { if (atomic) launch_ks_g<8, true>(dk, abar, cpad, binit, count, splits, out, out_idx, s); else launch_ks_g<8, false>(dk, abar, cpad, binit, count, splits, out, out_idx, s); }
+    else { if (atomic) launch_ks_g<1, true>(dk, abar, cpad, binit, count, splits, out, out_idx, s); else launch_ks_g<1, false>(dk, abar, cpad, binit, count, splits, out, out_idx, s); }
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+size_t ks_scratch_bytes(const HostParams& P, int count) {
+    const int cpad = ((count + 31) / 32) * 32 + 32;
+    return align256(sizeof(uint32_t) * (size_t)P.k * P.N * cpad) + align256(sizeof(int32_t) * (size_t)cpad);
+}
+
+size_t ext_bytes(const HostParams& P, int count) {
+    return align256(sizeof(int32_t) * 2 * (size_t)count * (P.k * P.N + 1));
+}
+
+// Core: gates on device-resident data. `scr` must hold ext_bytes(count) +
+// ks_scratch_bytes(count). d_ext_out (optional) receives the summed extracted
+// samples instead of running the keyswitch.
+int gates_device_locked(Ctx& C, tfhe_amd_device_key* dk, int count, const int32_t* d_op, const int32_t* d_in0,
+                        const int32_t* d_in1, const int32_t* d_in2, const int32_t* d_out, int32_t* d_pool,
+                        int32_t* d_ext_out, int32_t mu, char* scr, hipStream_t s) {
+    if (count <= 0) return count == 0 ? 0 : set_error("negative gate count");
+    const HostParams& P = dk->P;
+    const size_t eb = ext_bytes(P, count);
+    int32_t* ext = (int32_t*)scr;
+    BrArgs A = base_args(dk);
+    A.op = d_op;
+    A.in0 = d_in0;
+    A.in1 = d_in1;
+    A.in2 = d_in2;
+    A.pool = d_pool;
+    A.count = count;
+    A.mu = mu;
+    A.ext = ext;
+    Ctx::Ev ev{};
+    if (C.prof) {
+        for (auto& e : ev.e) HIP_TRY(hipEventCreate(&e));
+        HIP_TRY(hipEventRecord(ev.e[0], s));
+    }
+    launch_br(dk, A, 2 * count, s);
+    HIP_TRY(hipGetLastError());
+    if (C.prof) HIP_TRY(hipEventRecord(ev.e[1], s));
+    int rc = 0;
+    if (d_ext_out) {
+        hipLaunchKernelGGL(sum_halves_kernel, dim3(count), dim3(256), 0, s, ext, d_op, count, P.k * P.N, mu, d_ext_out);
+        HIP_TRY(hipGetLastError());
+    } else {
+        rc = run_keyswitch(C, dk, ext, d_op, count, d_pool, d_out, scr + eb, s);
+    }
+    if (C.prof) {
+        HIP_TRY(hipEventRecord(ev.e[2], s));
+        C.ev.push_back(ev);
+    }
+    return rc;
+}
+
+}  // namespace
+
+// ----------------------------------------------------------- key upload --
+
+static int upload_key(Ctx& C, const LweBootstrappingKeyFFT* bf, tfhe_amd_device_key** out) {
+    const HostParams P = host_params_fft(bf);
+    if (int rc = check_supported(P)) return rc;
+    const LweBootstrappingKey* bk = bf->bk_coef;
+    if (!bk || !bk->bk || !bk->ks) return set_error("cloud key has no coefficient-domain bootstrapping key");
+    const int N = P.N, k = P.k, kpl = (k + 1) * P.l, n = P.n, M = N / 2;
+    const int base = 1 << P.ks_basebit, t = P.ks_t, nin = k * N;
+    const int npad = ((n + 1 + 63) / 64) * 64;
+    auto* dk = new tfhe_amd_device_key();
+    dk->device = C.device;
+    dk->P = P;
+    dk->M = M;
+    dk->kpl = kpl;
+    dk->npad = npad;
+    const size_t npoly = (size_t)n * kpl * (k + 1);
+    const size_t bkf_bytes = npoly * M * sizeof(c64);
+    const size_t ksk_rows = (size_t)nin * t * (base - 1);
+    const size_t ksk_bytes = ksk_rows * npad * sizeof(int32_t);
+    const int twc = (M == 512) ? TwiddleTable<512>::COUNT : TwiddleTable<1024>::COUNT;
+    dk->bytes = bkf_bytes + ksk_bytes + twc * sizeof(c64);
+    auto fail = [&](int rc) {
+        (void)hipFree(dk->bkf);
+        (void)hipFree(dk->ksk);
+        (void)hipFree(dk->tw);
+        delete dk;
+        return rc;
+    };
+    if (hipMalloc(&dk->bkf, bkf_bytes) != hipSuccess) return fail(set_error("hipMalloc(bootstrapping key, %zu bytes) failed", bkf_bytes));
+    if (hipMalloc(&dk->ksk, ksk_bytes) != hipSuccess) return fail(set_error("hipMalloc(keyswitch key, %zu bytes) failed", ksk_bytes));
+    if (hipMalloc(&dk->tw, twc * sizeof(c64)) != hipSuccess) return fail(set_error("hipMalloc(twiddles) failed"));
+    // twiddles
+    std::vector<c64> tw(twc);
+    if (M == 512) build_twiddles<512>(tw.data());
+    else build_twiddles<1024>(tw.data());
+    // coefficient-domain BK, [n][kpl][k+1][N], staged through one host buffer
+    std::vector<int32_t> coef(npoly * N);
+    for (int i = 0; i < n; i++)
+        for (int r = 0; r < kpl; r++)
+            for (int c = 0; c <= k; c++)
+                memcpy(&coef[(((size_t)i * kpl + r) * (k + 1) + c) * N], bk->bk[i].all_sample[r].a[c].coefsT,
+                       sizeof(int32_t) * N);
+    // keyswitch key rows h >= 1, [nin][t][base-1][npad]
+    std::vector<int32_t> kh(ksk_rows * npad, 0);
+    for (int i = 0; i < nin; i++)
+        for (int j = 0; j < t; j++)
+            for (int h = 1; h < base; h++) {
+                const LweSample& s = bk->ks->ks0_raw[((size_t)i * t + j) * base + h];
+                int32_t* dst = &kh[(((size_t)i * t + j) * (base - 1) + (h - 1)) * npad];
+                memcpy(dst, s.a, sizeof(int32_t) * n);
+                dst[n] = s.b;
+            }
+    int32_t* d_coef = nullptr;
+    if (hipMalloc(&d_coef, coef.size() * sizeof(int32_t)) != hipSuccess) return fail(set_error("hipMalloc(staging) failed"));
+    hipError_t e = hipMemcpyAsync(d_coef, coef.data(), coef.size() * sizeof(int32_t), hipMemcpyHostToDevice, C.stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(dk->tw, tw.data(), twc * sizeof(c64), hipMemcpyHostToDevice, C.stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(dk->ksk, kh.data(), ksk_bytes, hipMemcpyHostToDevice, C.stream);
+    if (e == hipSuccess) {
+        if (M == 512) hipLaunchKernelGGL(bk_convert_kernel<512>, dim3((unsigned)npoly), dim3(64), 0, C.stream, d_coef, dk->bkf, dk->tw);
+        else hipLaunchKernelGGL(bk_convert_kernel<1024>, dim3((unsigned)npoly), dim3(64), 0, C.stream, d_coef, dk->bkf, dk->tw);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(C.stream);
+    (void)hipFree(d_coef);
+    if (e != hipSuccess) return fail(set_error("key upload failed: %s", hipGetErrorString(e)));
+    *out = dk;
+    return 0;
+}
+
+tfhe_amd_device_key* engine_key(const LweBootstrappingKeyFFT* bf0) {
+    if (!bf0) {
+        set_error("null bootstrapping key");
+        return nullptr;
+    }
+    Ctx& C = ctx();
+    std::lock_guard<std::mutex> g(C.mu);
+    if (ensure_ready(C)) return nullptr;
+    LweBootstrappingKeyFFT* bf = const_cast<LweBootstrappingKeyFFT*>(bf0);
+    if (bf->dkey) return bf->dkey;
+    tfhe_amd_device_key* dk = nullptr;
+    if (upload_key(C, bf, &dk)) return nullptr;
+    bf->dkey = dk;
+    return dk;
+}
+
+static void free_key(Ctx& C, tfhe_amd_device_key* dk) {
+    if (!dk) return;
+    if (C.stream) hipStreamSynchronize(C.stream);
+    (void)hipFree(dk->bkf);
+    (void)hipFree(dk->ksk);
+    (void)hipFree(dk->tw);
+    delete dk;
+}
+
+void engine_forget_host_key(const LweBootstrappingKeyFFT* bkfft) {
+    if (!bkfft || !bkfft->dkey) return;
+    Ctx& C = ctx();
+    std::lock_guard<std::mutex> g(C.mu);
+    LweBootstrappingKeyFFT* bf = const_cast<LweBootstrappingKeyFFT*>(bkfft);
+    free_key(C, bf->dkey);
+    bf->dkey = nullptr;
+}
+
+// ------------------------------------------------------ host-buffer paths --
+
+int engine_gates_host(const LweBootstrappingKeyFFT* bf, int32_t count, const int32_t* ops,
+                      LweSample* const* out, const LweSample* const* in0, const LweSample* const* in1,
+                      const LweSample* const* in2, int32_t mu, bool keyswitch) {
+    if (count == 0) return 0;
+    if (count < 0 || !ops || !out || !in0) return set_error("bad arguments to gate batch");
+    tfhe_amd_device_key* dk = engine_key(bf);
+    if (!dk) return -1;
+    Ctx& C = ctx();
+    std::lock_guard<std::mutex> g(C.mu);
+    const HostParams& P = dk->P;
+    const int w = P.n + 1, nx = P.k * P.N + 1;
+    for (int i = 0; i < count; i++) {
+        const int op = ops[i];
+        if (op < 0 || op > OP_IDENTITY) return set_error("gate %d: unknown gate code %d", i, op);
+        if (!out[i] || !in0[i] || (op != OP_IDENTITY && !(in1 && in1[i])) || (op == TFHE_AMD_MUX && !(in2 && in2[i])))
+            return set_error("gate %d: missing ciphertext", i);
+    }
+    // deduplicate input ciphertexts into pool slots
+    std::unordered_map<const LweSample*, int32_t> slot;
+    std::vector<const LweSample*> uniq;
+    auto slot_of = [&](const LweSample* p) {
+        auto it = slot.find(p);
+        if (it != slot.end()) return it->second;
+        int32_t s = (int32_t)uniq.size();
+        slot.emplace(p, s);
+        uniq.push_back(p);
+        return s;
+    };
+    std::vector<int32_t> idx(5 * (size_t)count);
+    for (int i = 0; i < count; i++) {
+        idx[i] = ops[i];
+        idx[count + i] = slot_of(in0[i]);
+        idx[2 * count + i] = ops[i] == OP_IDENTITY ? idx[count + i] : slot_of(in1[i]);
+        idx[3 * count + i] = ops[i] == TFHE_AMD_MUX ? slot_of(in2[i]) : idx[count + i];
+    }
+    const int nin_slots = (int)uniq.size();
+    for (int i = 0; i < count; i++) idx[4 * count + i] = nin_slots + i;
+    const size_t idx_bytes = align256(idx.size() * sizeof(int32_t));
+    const size_t pool_in_bytes = (size_t)nin_slots * w * sizeof(int32_t);
+    const size_t out_row = keyswitch ? (size_t)w : (size_t)nx;
+    const size_t out_bytes = (size_t)count * out_row * sizeof(int32_t);
+    const size_t pool_bytes = align256(pool_in_bytes + (keyswitch ? out_bytes : 0));
+    const size_t extout_bytes = keyswitch ? 0 : align256(out_bytes);
+    if (int rc = ensure_pinned(C, idx_bytes + pool_bytes + extout_bytes)) return rc;
+    char* h = (char*)C.h_pinned;
+    memcpy(h, idx.data(), idx.size() * sizeof(int32_t));
+    int32_t* hp = (int32_t*)(h + idx_bytes);
+    for (int s = 0; s < nin_slots; s++) {
+        memcpy(hp + (size_t)s * w, uniq[s]->a, sizeof(int32_t) * P.n);
+        hp[(size_t)s * w + P.n] = uniq[s]->b;
+    }
+    // device layout: [idx][pool][ext out] then the engine's own scratch
+    const size_t dev_bytes = idx_bytes + pool_bytes + extout_bytes;
+    const size_t work = ext_bytes(P, count) + ks_scratch_bytes(P, count);
+    if (int rc = ensure_scratch(C, dev_bytes + work)) return rc;
+    // the engine's working buffers follow the staged region
+    char* d = (char*)C.scratch.d;
+    HIP_TRY(hipMemcpyAsync(d, h, idx_bytes + pool_in_bytes, hipMemcpyHostToDevice, C.stream));
+    const int32_t* d_idx = (const int32_t*)d;
+    int32_t* d_pool = (int32_t*)(d + idx_bytes);
+    int32_t* d_extout = keyswitch ? nullptr : (int32_t*)(d + idx_bytes + pool_bytes);
+    int rc = gates_device_locked(C, dk, count, d_idx, d_idx + count, d_idx + 2 * count, d_idx + 3 * count,
+                                 d_idx + 4 * count, d_pool, d_extout, mu, d + dev_bytes, C.stream);
+    if (rc) return rc;
+    const void* src = keyswitch ? (const void*)(d_pool + (size_t)nin_slots * w) : (const void*)d_extout;
+    HIP_TRY(hipMemcpyAsync(h + idx_bytes, src, out_bytes, hipMemcpyDeviceToHost, C.stream));
+    HIP_TRY(hipStreamSynchronize(C.stream));
+    const int32_t* ho = (const int32_t*)(h + idx_bytes);
+    for (int i = 0; i < count; i++) {
+        const int32_t* row = ho + (size_t)i * out_row;
+        LweSample* o = out[i];
+        memcpy(o->a, row, sizeof(int32_t) * (out_row - 1));
+        o->b = row[out_row - 1];
+    }
+    return 0;
+}
+
+int engine_blind_rotate_host(tfhe_amd_device_key* dk, int32_t row0, int32_t n, const int32_t* bara, int32_t* acc,
+                             int32_t* ext_out) {
+    Ctx& C = ctx();
+    std::lock_guard<std::mutex> g(C.mu);
+    const HostParams& P = dk->P;
+    if (n < 0 || n > 1024 || row0 < 0 || row0 + n > P.n) return set_error("blind rotate: bad key row range");
+    const int N = P.N, nx = P.k * N + 1;
+    const size_t acc_bytes = align256(sizeof(int32_t) * 2 * N);
+    const size_t bara_bytes = align256(sizeof(int32_t) * 1024);
+    const size_t ext_b = align256(sizeof(int32_t) * nx);
+    const size_t total = 2 * acc_bytes + bara_bytes + ext_b;
+    if (int rc = ensure_scratch(C, total)) return rc;
+    if (int rc = ensure_pinned(C, total)) return rc;
+    char* h = (char*)C.h_pinned;
+    memcpy(h, acc, sizeof(int32_t) * 2 * N);
+    if (n) memcpy(h + acc_bytes, bara, sizeof(int32_t) * n);
+    char* d = (char*)C.scratch.d;
+    HIP_TRY(hipMemcpyAsync(d, h, acc_bytes + bara_bytes, hipMemcpyHostToDevice, C.stream));
+    BrArgs A = base_args(dk);
+    A.n = n;
+    A.bkf = dk->bkf + (size_t)row0 * dk->kpl * (P.k + 1) * dk->M;
+    A.acc_in = (const int32_t*)d;
+    A.bara_in = (const int32_t*)(d + acc_bytes);
+    A.acc_out = (int32_t*)(d + acc_bytes + bara_bytes);
+    A.ext = ext_out ? (int32_t*)(d + 2 * acc_bytes + bara_bytes) : nullptr;
+    launch_br(dk, A, 1, C.stream);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(h, d + acc_bytes + bara_bytes, acc_bytes + ext_b, hipMemcpyDeviceToHost, C.stream));
+    HIP_TRY(hipStreamSynchronize(C.stream));
+    memcpy(acc, h, sizeof(int32_t) * 2 * N);
+    if (ext_out) memcpy(ext_out, h + acc_bytes, sizeof(int32_t) * nx);
+    return 0;
+}
+
+int engine_keyswitch_host(tfhe_amd_device_key* dk, const int32_t* in, int32_t* out) {
+    Ctx& C = ctx();
+    std::lock_guard<std::mutex> g(C.mu);
+    const HostParams& P = dk->P;
+    const int nx = P.k * P.N + 1, w = P.n + 1;
+    const size_t in_b = align256(sizeof(int32_t) * 2 * nx), out_b = align256(sizeof(int32_t) * w);
+    const size_t work = ks_scratch_bytes(P, 1);
+    if (int rc = ensure_scratch(C, in_b + out_b + work)) return rc;
+    if (int rc = ensure_pinned(C, in_b + out_b)) return rc;
+    char* h = (char*)C.h_pinned;
+    memcpy(h, in, sizeof(int32_t) * nx);
+    char* d = (char*)C.scratch.d;
+    HIP_TRY(hipMemcpyAsync(d, h, sizeof(int32_t) * nx, hipMemcpyHostToDevice, C.stream));
+    if (int rc = run_keyswitch(C, dk, (const int32_t*)d, nullptr, 1, (int32_t*)(d + in_b), nullptr, d + in_b + out_b, C.stream))
+        return rc;
+    HIP_TRY(hipMemcpyAsync(h, d + in_b, sizeof(int32_t) * w, hipMemcpyDeviceToHost, C.stream));
+    HIP_TRY(hipStreamSynchronize(C.stream));
+    memcpy(out, h, sizeof(int32_t) * w);
+    return 0;
+}
+
+}  // namespace tfhe_amd
+
+// ------------------------------------------------------------- C-ABI --------
+using namespace tfhe_amd;
+
+TFHE_AMD_EXPORT int32_t tfhe_amd_cloud_key_upload(const TFheGateBootstrappingCloudKeySet* ck) {
+    if (!ck) return set_error("null cloud key");
+    return engine_key(ck->bkFFT) ? 0 : -1;
+}
+
+TFHE_AMD_EXPORT void tfhe_amd_cloud_key_release(const TFheGateBootstrappingCloudKeySet* ck) {
+    if (ck) engine_forget_host_key(ck->bkFFT);
+}
+
+TFHE_AMD_EXPORT size_t tfhe_amd_cloud_key_device_bytes(const TFheGateBootstrappingCloudKeySet* ck) {
+    return (ck && ck->bkFFT && ck->bkFFT->dkey) ? ck->bkFFT->dkey->bytes : 0;
+}
+
+TFHE_AMD_EXPORT int32_t tfhe_amd_gates_device(const TFheGateBootstrappingCloudKeySet* ck, int32_t count,
+                                              const int32_t* d_op, const int32_t* d_in0, const int32_t* d_in1,
+                                              const int32_t* d_in2, const int32_t* d_out, int32_t* d_pool,
+                                              void* stream) {
+    if (!ck) return set_error("null cloud key");
+    tfhe_amd_device_key* dk = engine_key(ck->bkFFT);
+    if (!dk) return -1;
+    Ctx& C = ctx();
+    std::lock_guard<std::mutex> g(C.mu);
+    if (count < 0) return set_error("negative gate count");
+    if (int rc = ensure_scratch(C, ext_bytes(dk->P, count) + ks_scratch_bytes(dk->P, count))) return rc;
+    return gates_device_locked(C, dk, count, d_op, d_in0, d_in1, d_in2, d_out, d_pool, nullptr, 1 << 29,
+                               (char*)C.scratch.d, stream ? (hipStream_t)stream : C.stream);
+}
+
+TFHE_AMD_EXPORT int32_t tfhe_amd_gates_device_woks(const TFheGateBootstrappingCloudKeySet* ck, int32_t count,
+                                                   const int32_t* d_op, const int32_t* d_in0, const int32_t* d_in1,
+                                                   const int32_t* d_in2, const int32_t* d_pool, int32_t* d_ext,
+                                                   void* stream) {
+    if (!ck) return set_error("null cloud key");
+    tfhe_amd_device_key* dk = engine_key(ck->bkFFT);
+    if (!dk) return -1;
+    Ctx& C = ctx();
+    std::lock_guard<std::mutex> g(C.mu);
+    if (count < 0) return set_error("negative gate count");
+    if (int rc = ensure_scratch(C, ext_bytes(dk->P, count) + ks_scratch_bytes(dk->P, count))) return rc;
+    return gates_device_locked(C, dk, count, d_op, d_in0, d_in1, d_in2, nullptr, const_cast<int32_t*>(d_pool), d_ext, 1 << 29,
+                               (char*)C.scratch.d, stream ? (hipStream_t)stream : C.stream);
+}
+
+TFHE_AMD_EXPORT int32_t tfhe_amd_keyswitch_device(const TFheGateBootstrappingCloudKeySet* ck, int32_t count,
+                                                  const int32_t* d_in, int32_t* d_out, void* stream) {
+    if (!ck) return set_error("null cloud key");
+    tfhe_amd_device_key* dk = engine_key(ck->bkFFT);
+    if (!dk) return -1;
+    if (count <= 0) return count == 0 ? 0 : set_error("negative count");
+    Ctx& C = ctx();
+    std::lock_guard<std::mutex> g(C.mu);
+    // run_keyswitch reads ext rows [2*count][..]; with op == null only the first count are used
+    if (int rc = ensure_scratch(C, ks_scratch_bytes(dk->P, count))) return rc;
+    return run_keyswitch(C, dk, d_in, nullptr, count, d_out, nullptr, (char*)C.scratch.d,
+                         stream ? (hipStream_t)stream : C.stream);
+}
+
+TFHE_AMD_EXPORT void tfhe_amd_set_launch_config(int32_t br_mode, int32_t br_blocks_per_cu, int32_t ks_gates_per_block) {
+    Ctx& C = ctx();
+    std::lock_guard<std::mutex> g(C.mu);
+    C.br_mode = br_mode;
+    C.br_bpc = br_blocks_per_cu;
+    C.ks_g = (ks_gates_per_block == 1 || ks_gates_per_block == 8 || ks_gates_per_block == 32) ? ks_gates_per_block : 0;
+}
+
+TFHE_AMD_EXPORT int32_t tfhe_amd_get_launch_config(int32_t* br_mode, int32_t* br_blocks_per_cu, int32_t* ks_gates_per_block) {
+    Ctx& C = ctx();
+    std::lock_guard<std::mutex> g(C.mu);
+    if (br_mode) *br_mode = C.br_mode;
+    if (br_blocks_per_cu) *br_blocks_per_cu = C.br_bpc;
+    if (ks_gates_per_block) *ks_gates_per_block = C.ks_g;
+    return 0;
+}
+
+TFHE_AMD_EXPORT void tfhe_amd_profile_enable(int32_t on) {
+    Ctx& C = ctx();
+    std::lock_guard<std::mutex> g(C.mu);
+    C.prof = on != 0;
+}
+
+TFHE_AMD_EXPORT int32_t tfhe_amd_profile_read(double* br_ms, double* ks_ms, int32_t* launches) {
+    Ctx& C = ctx();
+    std::lock_guard<std::mutex> g(C.mu);
+    double br = 0, ks = 0;
+    int rc = 0;
+    for (auto& e : C.ev) {
+        float a = 0, b = 0;
+        if (hipEventSynchronize(e.e[2]) != hipSuccess || hipEventElapsedTime(&a, e.e[0], e.e[1]) != hipSuccess ||
+            hipEventElapsedTime(&b, e.e[1], e.e[2]) != hipSuccess)
+            rc = set_error("event timing failed");
+        br += a;
+        ks += b;
+        for (auto x : e.e) (void)hipEventDestroy(x);
+    }
+    if (br_ms) *br_ms = br;
+    if (ks_ms) *ks_ms = ks;
+    if (launches) *launches = (int32_t)C.ev.size();
+    C.ev.clear();
+    return rc;
+}
+
+TFHE_AMD_EXPORT void* tfhe_amd_malloc(size_t bytes) {
+    void* p = nullptr;
+    if (hipMalloc(&p, bytes) != hipSuccess) {
+        set_error("hipMalloc(%zu) failed", bytes);
+        return nullptr;
+    }
+    return p;
+}
+
+TFHE_AMD_EXPORT int32_t tfhe_amd_free(void* ptr) {
+    HIP_TRY(hipFree(ptr));
+    return 0;
+}
+
+TFHE_AMD_EXPORT int32_t tfhe_amd_memcpy_h2d(void* dst, const void* src, size_t bytes) {
+    HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return 0;
+}
+
+TFHE_AMD_EXPORT int32_t tfhe_amd_memcpy_d2h(void* dst, const void* src, size_t bytes) {
+    HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+TFHE_AMD_EXPORT int32_t tfhe_amd_synchronize(void) {
+    Ctx& C = ctx();
+    {
+        std::lock_guard<std::mutex> g(C.mu);
+        if (int rc = ensure_ready(C)) return rc;
+    }
+    HIP_TRY(hipStreamSynchronize(C.stream));
+    HIP_TRY(hipDeviceSynchronize());
+    return 0;
+}
+
+TFHE_AMD_EXPORT void* tfhe_amd_stream(void) {
+    Ctx& C = ctx();
+    std::lock_guard<std::mutex> g(C.mu);
+    if (ensure_ready(C)) return nullptr;
+    return (void*)C.stream;
+}
+
+TFHE_AMD_EXPORT int32_t tfhe_amd_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}

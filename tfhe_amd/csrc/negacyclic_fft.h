@@ -1,0 +1,278 @@
+// Negacyclic FP64 transform for polynomials mod X^N+1, laid out for one
+// 64-lane wavefront.
+//
+// A real polynomial p of degree < N = 2M is folded to M complex points
+//   z_j = (p_j + i p_{j+M}) * psi^j,   psi = exp(i*pi/N),   j < M
+// and Z = DFT_M(z) (positive exponent) evaluates p at the M roots of X^N+1 of
+// the form exp(i*pi*(4k+1)/N); the other M roots are their conjugates. The
+// negacyclic product is then pointwise, as in the reference's
+// IntPolynomial_ifft / TorusPolynomial_fft pair
+// (fft_processors/spqlios/fft_processor_spqlios.cpp:38-115), whose element
+// order, scaling and truncation are NOT reproduced: the order here is private
+// to this library (the bootstrapping key is transformed by the same routine),
+// the inverse is exactly scaled and results are rounded to nearest.
+//
+// Wavefront layout, M = 64 R (R = 8 for N = 1024, 16 for N = 2048):
+//   L0  lane b, slot a      : z_{64a+b}                        (coefficient order)
+//   stage 1  (in registers) : R-point DFT over a, times psi^b * w_M^{b k1}
+//   T1  (LDS transpose)     : lane (b0 + 8c), slot (g, b1)  with b = b0 + 8 b1, k1 = cG+g
+//   stage 2a (in registers) : 8-point DFT over b1, times w_64^{b0 k2a}
+//   T2  (LDS transpose)     : lane (k2a + 8c), slot (g, b0)
+//   stage 2b (in registers) : 8-point DFT over b0
+// Register (lane, g*8+j) then holds frequency k1 + R*(k2a + 8*k2b) with
+// k2b = bitrev3(j). Forward stages use decimation in frequency (natural in,
+// bit-reversed out); the inverse runs the same steps backwards with
+// decimation-in-time butterflies, so no explicit bit reversal is ever done.
+//
+// The functions below are split at every point where lanes exchange data
+// through LDS, so the same code runs on the GPU (one wavefront, in order) and
+// in a lane-by-lane CPU emulation used by the tests.
+#pragma once
+
+#include <stdint.h>
+
+#include <cmath>
+
+#if defined(__HIPCC__)
+#define TA_HD __host__ __device__ __forceinline__
+#else
+#define TA_HD inline
+#endif
+
+namespace tfhe_amd {
+
+struct __attribute__((aligned(16))) c64 {
+    double re, im;
+};
+
+TA_HD c64 cadd(c64 a, c64 b) { return {a.re + b.re, a.im + b.im}; }
+TA_HD c64 csub(c64 a, c64 b) { return {a.re - b.re, a.im - b.im}; }
+TA_HD c64 cmul(c64 a, c64 b) { return {fma(a.re, b.re, -(a.im * b.im)), fma(a.re, b.im, a.im * b.re)}; }
+// a * conj(b)
+TA_HD c64 cmulc(c64 a, c64 b) { return {fma(a.re, b.re, a.im * b.im), fma(a.im, b.re, -(a.re * b.im))}; }
+// acc += a * b
+TA_HD void cmac(c64& acc, c64 a, c64 b) {
+    acc.re = fma(a.re, b.re, acc.re);
+    acc.re = fma(-a.im, b.im, acc.re);
+    acc.im = fma(a.re, b.im, acc.im);
+    acc.im = fma(a.im, b.re, acc.im);
+}
+
+constexpr double kC8 = 0.92387953251128675613;  // cos(pi/8)
+constexpr double kS8 = 0.38268343236508977173;  // sin(pi/8)
+constexpr double kH = 0.70710678118654752440;   // cos(pi/4)
+
+// v * exp(S * 2*pi*i * m / L) for L in {8, 16}; m is a compile-time constant
+// after unrolling, so the switch folds away.
+template <int L, int S>
+TA_HD c64 twc(c64 v, int m) {
+    const int q = ((m * (16 / L)) % 16 + 16) % 16;
+    double c, s;
+    switch (q) {
+        case 0: return v;
+        case 4: return S > 0 ? c64{-v.im, v.re} : c64{v.im, -v.re};
+        case 8: return c64{-v.re, -v.im};
+        case 12: return S > 0 ? c64{v.im, -v.re} : c64{-v.im, v.re};
+        case 1: c = kC8; s = kS8; break;
+        case 2: c = kH; s = kH; break;
+        case 3: c = kS8; s = kC8; break;
+        case 5: c = -kS8; s = kC8; break;
+        case 6: c = -kH; s = kH; break;
+        case 7: c = -kC8; s = kS8; break;
+        case 9: c = -kC8; s = -kS8; break;
+        case 10: c = -kH; s = -kH; break;
+        case 11: c = -kS8; s = -kC8; break;
+        case 13: c = kS8; s = -kC8; break;
+        case 14: c = kH; s = -kH; break;
+        default: c = kC8; s = -kS8; break;  // 15
+    }
+    return cmul(v, c64{c, S > 0 ? s : -s});
+}
+
+// In-register DFT, natural order in, bit-reversed order out:
+// x[j] <- sum_n x[n] exp(S 2 pi i n bitrev(j) / L)
+template <int L, int S>
+TA_HD void dif(c64* x) {
+#pragma unroll
+    for (int span = L / 2; span >= 1; span >>= 1) {
+#pragma unroll
+        for (int start = 0; start < L; start += 2 * span) {
+#pragma unroll
+            for (int j = 0; j < span; j++) {
+                c64 a = x[start + j], b = x[start + j + span];
+                x[start + j] = cadd(a, b);
+                x[start + j + span] = twc<L, S>(csub(a, b), j * (L / (2 * span)));
+            }
+        }
+    }
+}
+
+// In-register DFT, bit-reversed order in, natural order out.
+template <int L, int S>
+TA_HD void dit(c64* x) {
+#pragma unroll
+    for (int span = 1; span < L; span <<= 1) {
+#pragma unroll
+        for (int start = 0; start < L; start += 2 * span) {
+#pragma unroll
+            for (int j = 0; j < span; j++) {
+                c64 a = x[start + j], b = twc<L, S>(x[start + j + span], j * (L / (2 * span)));
+                x[start + j] = cadd(a, b);
+                x[start + j + span] = csub(a, b);
+            }
+        }
+    }
+}
+
+TA_HD constexpr int bitrev(int v, int bits) {
+    int r = 0;
+    for (int i = 0; i < bits; i++) r |= ((v >> i) & 1) << (bits - 1 - i);
+    return r;
+}
+
+template <int M>
+struct Fft {
+    static constexpr int R = M / 64;  // slots per lane
+    static constexpr int G = R / 8;   // 8-point groups per lane in stage 2
+    static constexpr int LOGR = (R == 8) ? 3 : 4;
+    static constexpr int ROW = 72;    // LDS row stride in c64 (64 + 8 pad)
+    static constexpr int LDS_C64 = R * ROW;
+    static_assert(R == 8 || R == 16, "M must be 512 or 1024");
+
+    TA_HD static int a1(int k1, int b) { return k1 * ROW + b; }
+    TA_HD static int a2(int k1, int b0, int k2a) { return k1 * ROW + k2a * 9 + b0; }
+
+    // ---- forward ----
+    // x[a] = p_{64a+b} + i p_{64a+b+M} (untwisted); twist[a] = psi^{64a}; tw1[k1] = psi^b w_M^{b k1}
+    TA_HD static void fwd1(c64* x, int b, c64* lds, const c64* twist, const c64* tw1) {
+#pragma unroll
+        for (int a = 1; a < R; a++) x[a] = cmul(x[a], twist[a]);
+        dif<R, 1>(x);
+#pragma unroll
+        for (int j = 0; j < R; j++) {
+            const int k1 = bitrev(j, LOGR);
+            lds[a1(k1, b)] = cmul(x[j], tw1[k1]);
+        }
+    }
+    // Powers w^0..w^7 of w = w_64^{b0}, b0 = lane & 7 (the stage-2a twiddles),
+    // rebuilt from one base value each time instead of occupying 28 registers.
+    TA_HD static void powers8(c64 w, c64* t) {
+        t[0] = c64{1.0, 0.0};
+        t[1] = w;
+        t[2] = cmul(w, w);
+        t[3] = cmul(t[2], w);
+        t[4] = cmul(t[2], t[2]);
+        t[5] = cmul(t[4], w);
+        t[6] = cmul(t[4], t[2]);
+        t[7] = cmul(t[4], t[3]);
+    }
+    TA_HD static void fwd2(c64* x, int lane, const c64* lds, c64 w) {
+        const int b0 = lane & 7, c = lane >> 3;
+        c64 t[8];
+        powers8(w, t);
+#pragma unroll
+        for (int g = 0; g < G; g++) {
+#pragma unroll
+            for (int b1 = 0; b1 < 8; b1++) x[g * 8 + b1] = lds[a1(c * G + g, b0 + 8 * b1)];
+            dif<8, 1>(x + g * 8);
+#pragma unroll
+            for (int j = 1; j < 8; j++) x[g * 8 + j] = cmul(x[g * 8 + j], t[bitrev(j, 3)]);
+        }
+    }
+    TA_HD static void fwd3(const c64* x, int lane, c64* lds) {
+        const int b0 = lane & 7, c = lane >> 3;
+#pragma unroll
+        for (int g = 0; g < G; g++)
+#pragma unroll
+            for (int j = 0; j < 8; j++) lds[a2(c * G + g, b0, bitrev(j, 3))] = x[g * 8 + j];
+    }
+    TA_HD static void fwd4(c64* x, int lane, const c64* lds) {
+        const int k2a = lane & 7, c = lane >> 3;
+#pragma unroll
+        for (int g = 0; g < G; g++) {
+#pragma unroll
+            for (int b0 = 0; b0 < 8; b0++) x[g * 8 + b0] = lds[a2(c * G + g, b0, k2a)];
+            dif<8, 1>(x + g * 8);
+        }
+    }
+
+    // ---- inverse (exact inverse of the forward steps, times M) ----
+    TA_HD static void inv1(c64* x, int lane, c64* lds) {
+        const int k2a = lane & 7, c = lane >> 3;
+#pragma unroll
+        for (int g = 0; g < G; g++) {
+            dit<8, -1>(x + g * 8);
+#pragma unroll
+            for (int b0 = 0; b0 < 8; b0++) lds[a2(c * G + g, b0, k2a)] = x[g * 8 + b0];
+        }
+    }
+    TA_HD static void inv2(c64* x, int lane, const c64* lds, c64 w) {
+        const int b0 = lane & 7, c = lane >> 3;
+        c64 t[8];
+        powers8(w, t);
+#pragma unroll
+        for (int g = 0; g < G; g++) {
+            x[g * 8] = lds[a2(c * G + g, b0, 0)];
+#pragma unroll
+            for (int j = 1; j < 8; j++) {
+                const int k2a = bitrev(j, 3);
+                x[g * 8 + j] = cmulc(lds[a2(c * G + g, b0, k2a)], t[k2a]);
+            }
+            dit<8, -1>(x + g * 8);
+        }
+    }
+    TA_HD static void inv3(const c64* x, int lane, c64* lds) {
+        const int b0 = lane & 7, c = lane >> 3;
+#pragma unroll
+        for (int g = 0; g < G; g++)
+#pragma unroll
+            for (int b1 = 0; b1 < 8; b1++) lds[a1(c * G + g, b0 + 8 * b1)] = x[g * 8 + b1];
+    }
+    // itwist[a] = conj(psi^{64a}) / M; result x[a] = p_{64a+b} + i p_{64a+b+M}
+    TA_HD static void inv4(c64* x, int b, const c64* lds, const c64* tw1, const c64* itwist) {
+#pragma unroll
+        for (int j = 0; j < R; j++) {
+            const int k1 = bitrev(j, LOGR);
+            x[j] = cmulc(lds[a1(k1, b)], tw1[k1]);
+        }
+        dit<R, -1>(x);
+#pragma unroll
+        for (int a = 0; a < R; a++) x[a] = cmul(x[a], itwist[a]);
+    }
+};
+
+// Per-lane twiddle tables, built on the host in long double and rounded once.
+// Layout: for each lane, R entries of tw1 then 8 entries of tw2; then the R
+// uniform twist entries and R inverse-twist entries.
+template <int M>
+struct TwiddleTable {
+    static constexpr int R = M / 64;
+    static constexpr int PER_LANE = R + 8;
+    static constexpr int COUNT = 64 * PER_LANE + 2 * R;
+};
+
+template <int M>
+inline void build_twiddles(c64* out) {
+    constexpr int R = M / 64;
+    const long double pi = 3.141592653589793238462643383279502884L;
+    auto ex = [&](long double ang) { return c64{(double)cosl(ang), (double)sinl(ang)}; };
+    for (int b = 0; b < 64; b++) {
+        c64* t = out + b * (R + 8);
+        for (int k1 = 0; k1 < R; k1++) {
+            // psi^b * w_M^{b k1} = exp(i pi b (1 + 4 k1) / (2M))
+            const long e = ((long)b * (1 + 4 * k1)) % (4 * M);
+            t[k1] = ex(pi * (long double)e / (2.0L * M));
+        }
+        const int b0 = b & 7;
+        for (int k2a = 0; k2a < 8; k2a++) t[R + k2a] = ex(2.0L * pi * (long double)((b0 * k2a) % 64) / 64.0L);
+    }
+    c64* tw = out + 64 * (R + 8);
+    for (int a = 0; a < R; a++) {
+        // psi^{64a} = exp(i pi 64 a / (2M))
+        c64 v = ex(pi * 64.0L * a / (2.0L * M));
+        tw[a] = v;
+        tw[R + a] = c64{v.re / M, -v.im / M};
+    }
+}
+
+}  // namespace tfhe_amd
