@@ -36,6 +36,10 @@ def parse():
     ap.add_argument("--per-gate", type=int, default=64, help="one-gate-per-launch samples")
     ap.add_argument("--cpu-gates", type=int, default=500, help="reference CPU sample size (0 = skip)")
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--compare-modes", action="store_true",
+                    help="also time the one-wavefront-per-bootstrap kernel on the same batch")
+    ap.add_argument("--copy-calib-mb", type=int, default=0,
+                    help="also run one plain copy kernel of this many MiB (HBM counter calibration)")
     return ap.parse_args()
 
 
@@ -76,6 +80,14 @@ def main():
 
     if T.lib().tfhe_amd_device_count() < 1:
         raise SystemExit("no GPU visible to libtfhe_amd.so")
+    if args.copy_calib_mb:
+        nb = args.copy_calib_mb << 20
+        src, dst = T.DeviceArray(nb // 4), T.DeviceArray(nb // 4)
+        T.check(T.lib().tfhe_amd_copy_device(dst.ptr, src.ptr, nb, None), "copy")
+        T.synchronize()
+        print(f"[bench] copy kernel: {nb} bytes read, {nb} bytes written", file=sys.stderr, flush=True)
+        src.free()
+        dst.free()
     P = T.default_params(110)
     sk = T.SecretKey(P, seed=args.seed)
     sk.upload()
@@ -120,6 +132,22 @@ def main():
     print(f"[bench] batched {batched:.1f} gates/s, {ms_per_step:.3f} ms/step, errors {errors}", file=sys.stderr,
           flush=True)
 
+    mode_cmp = None
+    if args.compare_modes:
+        mode_cmp = {}
+        for mode in (1, 2):
+            T.lib().tfhe_amd_set_launch_config(mode, 0, 0)
+            step()
+            T.synchronize()
+            t1 = time.perf_counter()
+            for _ in range(args.steps):
+                step()
+            T.synchronize()
+            mode_cmp[{1: "one_wave_per_bootstrap", 2: "four_gate_lockstep"}[mode]] = round(
+                B * args.steps / (time.perf_counter() - t1), 2)
+        T.lib().tfhe_amd_set_launch_config(0, 0, 0)
+        print(f"[bench] modes {mode_cmp}", file=sys.stderr, flush=True)
+
     # one gate per launch (same kernels, count = 1, synchronised each time)
     lat = []
     one = T.DeviceArray.from_numpy(np.zeros(1, np.int32))
@@ -163,6 +191,7 @@ def main():
         "batched_over_per_gate": round(batched / per_gate, 2),
         "batch_latency_ms": {"p50": round(ms_per_step, 4), "p99": round(ms_per_step, 4)},
         "decrypt_errors": errors,
+        "kernel_variants_gates_per_s": mode_cmp,
         "kernel_ms": {"blind_rotate_avg": round(br_ms / max(launches, 1), 4),
                       "keyswitch_avg": round(ks_ms / max(launches, 1), 4)},
         "roofline": {"bound": "mfma", "achieved": round(achieved_tflops, 3) if achieved_tflops else None,

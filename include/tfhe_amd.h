@@ -254,6 +254,9 @@ TFHE_AMD_EXPORT void tfhe_amd_profile_enable(int32_t on);
 TFHE_AMD_EXPORT int32_t tfhe_amd_profile_read(double* br_ms, double* ks_ms, int32_t* launches);
 
 /* Device memory helpers for callers without a HIP runtime of their own. */
+/* Asynchronous device-to-device copy with a plain copy kernel (16-byte
+ * aligned sizes and pointers), e.g. to move ciphertexts between pools. */
+TFHE_AMD_EXPORT int32_t tfhe_amd_copy_device(void* dst, const void* src, size_t bytes, void* stream);
 TFHE_AMD_EXPORT void* tfhe_amd_malloc(size_t bytes);
 TFHE_AMD_EXPORT int32_t tfhe_amd_free(void* ptr);
 TFHE_AMD_EXPORT int32_t tfhe_amd_memcpy_h2d(void* dst, const void* src, size_t bytes);

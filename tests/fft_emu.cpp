@@ -21,15 +21,13 @@ struct Emu {
     Emu() : tw(TwiddleTable<M>::COUNT) { build_twiddles<M>(tw.data()); }
     const c64* tw1(int lane) const { return tw.data() + lane * (R + 8); }
     const c64* tw2(int lane) const { return tw.data() + lane * (R + 8) + R; }
-    const c64* twist() const { return tw.data() + 64 * (R + 8); }
-    const c64* itwist() const { return tw.data() + 64 * (R + 8) + R; }
 
     // poly (length 2M, as doubles) -> frequency registers x[64][R]
     void forward(const double* p, c64 (*x)[R]) {
         std::vector<c64> lds(F::LDS_C64);
         for (int b = 0; b < 64; b++)
             for (int a = 0; a < R; a++) x[b][a] = c64{p[64 * a + b], p[64 * a + b + M]};
-        for (int l = 0; l < 64; l++) F::fwd1(x[l], l, lds.data(), twist(), tw1(l));
+        for (int l = 0; l < 64; l++) F::fwd1(x[l], l, lds.data(), tw1(l));
         for (int l = 0; l < 64; l++) F::fwd2(x[l], l, lds.data(), tw2(l)[1]);
         for (int l = 0; l < 64; l++) F::fwd3(x[l], l, lds.data());
         for (int l = 0; l < 64; l++) F::fwd4(x[l], l, lds.data());
@@ -39,7 +37,7 @@ struct Emu {
         for (int l = 0; l < 64; l++) F::inv1(x[l], l, lds.data());
         for (int l = 0; l < 64; l++) F::inv2(x[l], l, lds.data(), tw2(l)[1]);
         for (int l = 0; l < 64; l++) F::inv3(x[l], l, lds.data());
-        for (int l = 0; l < 64; l++) F::inv4(x[l], l, lds.data(), tw1(l), itwist());
+        for (int l = 0; l < 64; l++) F::inv4(x[l], l, lds.data(), tw1(l));
         for (int b = 0; b < 64; b++)
             for (int a = 0; a < R; a++) {
                 p[64 * a + b] = x[b][a].re;

@@ -137,6 +137,7 @@ _SIGS = {
     "tfhe_amd_get_launch_config": (i32, [i32p, i32p, i32p]),
     "tfhe_amd_profile_enable": (None, [i32]),
     "tfhe_amd_profile_read": (i32, [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double), i32p]),
+    "tfhe_amd_copy_device": (i32, [vp, vp, ctypes.c_size_t, vp]),
     "tfhe_amd_malloc": (vp, [ctypes.c_size_t]),
     "tfhe_amd_free": (i32, [vp]),
     "tfhe_amd_memcpy_h2d": (i32, [vp, vp, ctypes.c_size_t]),

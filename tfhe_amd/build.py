@@ -22,6 +22,8 @@ HIP_FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "-fvisibility=hidden", f"--offload-arch={ARCH}",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function",
+    # keep per-thread arrays in registers (never in LDS, which the kernels budget explicitly)
+    "-mllvm", "-disable-promote-alloca-to-lds",
 ]
 HOST_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-fvisibility=hidden", "-Wall", "-Wextra", "-Wno-unused-parameter"]
 

@@ -11,10 +11,11 @@
 //                      tfhe_blindRotate_FFT, lwe-bootstrapping-functions-fft.cpp:67-200)
 //   ks_prep_kernel     sums the two halves of MUX gates and transposes the
 //                      extracted masks for the keyswitch
-//   ks_kernel          exact int32 keyswitch, lanes over output coefficients,
-//                      a tile of gates per block so each key row read from
-//                      HBM/L2 serves the whole tile (lweKeySwitch,
+//   ks_kernel          exact int32 keyswitch, lanes over gates: key rows staged
+//                      in LDS per block so each row read from HBM/L2 serves
+//                      up to 256 gates (lweKeySwitch,
 //                      lwe-keyswitch-functions.cpp:114-129,228-238)
+//   ks_reduce_kernel   adds the partial sums of the input-coefficient splits
 #include <hip/hip_runtime.h>
 
 #include <cstdarg>
@@ -56,7 +57,9 @@ struct BrArgs {
     const int32_t* in1;
     const int32_t* in2;
     const int32_t* pool;
-    int32_t count;  // gates; the grid has 2*count blocks, the second half serves MUX gates
+    int32_t count;              // gates; jobs [0, count) are their first bootstraps
+    const int32_t* mux_gate;    // gate of job count + k (second bootstrap of the k-th MUX gate)
+    const int32_t* n_jobs;      // count + number of MUX gates (device)
     int32_t mu;
     const int32_t* bara_in;  // explicit mode (one job): n modulus-switched coefficients
     const int32_t* acc_in;   // explicit mode: initial accumulator [(k+1)][N]
@@ -68,6 +71,15 @@ struct BrArgs {
     int32_t Bgbit;
     uint32_t offset;  // gadget offset, libtfhe/tgsw.cpp:21-27
 };
+
+// Orders LDS accesses between the lanes of one wavefront. The blind-rotation
+// block is a single wavefront, whose LDS instructions execute in program
+// order, so only the compiler needs fencing; a workgroup barrier here would
+// also force every outstanding global load (the key prefetch) to complete.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
 
 // round to nearest, result mod 2^32 (valid for |v| < 2^51)
 __device__ __forceinline__ int32_t round_mod32(double v) {
@@ -114,10 +126,10 @@ __global__ __launch_bounds__(64, 2) void br_kernel(BrArgs A) {
     const int n = A.n;
 
     if (A.bara_in == nullptr) {
+        if (job >= *A.n_jobs) return;
         int gate = job, half = 0;
-        if (job >= A.count) { gate = job - A.count; half = 1; }
+        if (job >= A.count) { gate = A.mux_gate[job - A.count]; half = 1; }
         const int op = A.op[gate];
-        if (half && op != TFHE_AMD_MUX) return;
         int32_t k0, ca, cb, cc;
         gate_coeffs(op, half, k0, ca, cb, cc);
         const int32_t* a = A.pool + (size_t)A.in0[gate] * (n + 1);
@@ -144,19 +156,26 @@ __global__ __launch_bounds__(64, 2) void br_kernel(BrArgs A) {
     }
     __syncthreads();
 
-    c64 tw1[R], twist[R], itwist[R];
+    c64 tw1[R];
 #pragma unroll
     for (int k = 0; k < R; k++) tw1[k] = A.tw[lane * (R + 8) + k];
     const c64 tw2 = A.tw[lane * (R + 8) + R + 1];
-#pragma unroll
-    for (int a = 0; a < R; a++) {
-        twist[a] = A.tw[64 * (R + 8) + a];
-        itwist[a] = A.tw[64 * (R + 8) + R + a];
-    }
     const uint32_t offset = A.offset;
     const int Bgbit = A.Bgbit;
     const uint32_t dmask = (1u << Bgbit) - 1u;
     const int32_t dhalf = 1 << (Bgbit - 1);
+
+    // Key rows of the current digit polynomial, loaded one digit ahead of use
+    // so the loads are in flight during the transform.
+    c64 bk0[R], bk1[R];
+    auto load_bk = [&](int row, int r) {
+        const c64* q = A.bkf + ((size_t)row * KPL + r) * 2 * M + lane;
+#pragma unroll
+        for (int s = 0; s < R; s++) {
+            bk0[s] = q[s * 64];
+            bk1[s] = q[M + s * 64];
+        }
+    };
 
     for (int i = 0; i < n; i++) {
         const int abar = s_bara[i];
@@ -164,46 +183,46 @@ __global__ __launch_bounds__(64, 2) void br_kernel(BrArgs A) {
         c64 r0[R], r1[R];
 #pragma unroll
         for (int s = 0; s < R; s++) r0[s] = r1[s] = c64{0.0, 0.0};
-        const c64* bkrow = A.bkf + (size_t)i * KPL * 2 * M + lane;
+        uint32_t tv[2 * R];
 #pragma unroll 1
-        for (int c = 0; c < 2; c++) {
-            const int32_t* acc = s_acc + c * N;
-            // T = (X^abar - 1) * ACC_c at this lane's 2R coefficients
-            uint32_t tv[2 * R];
+        for (int r = 0; r < KPL; r++) {
+            const int c = r >= L ? 1 : 0, p = r - c * L;
+            if (p == 0) {
+                // T = (X^abar - 1) * ACC_c (+ gadget offset) at this lane's 2R coefficients
+                const int32_t* acc = s_acc + c * N;
 #pragma unroll
-            for (int h = 0; h < 2; h++)
+                for (int h = 0; h < 2; h++)
 #pragma unroll
-                for (int s = 0; s < R; s++) {
-                    const int j = 64 * s + lane + h * M;
-                    const int idx = (j - abar) & (2 * N - 1);
-                    const int32_t rot = idx < N ? acc[idx] : -acc[idx - N];
-                    tv[h * R + s] = (uint32_t)rot - (uint32_t)acc[j] + offset;
-                }
-#pragma unroll 1
-            for (int p = 0; p < L; p++) {
-                // digit p of the truncating gadget decomposition, tgsw-functions.cpp:337-348
-                const int shift = 32 - (p + 1) * Bgbit;
-                c64 x[R];
+                    for (int s = 0; s < R; s++) {
+                        const int j = 64 * s + lane + h * M;
+                        const int idx = (j - abar) & (2 * N - 1);
+                        const uint32_t v = (uint32_t)acc[idx & (N - 1)];
+                        const uint32_t rot = (idx & N) ? 0u - v : v;
+                        tv[h * R + s] = rot - (uint32_t)acc[j] + offset;
+                    }
+            }
+            // digit p of the truncating gadget decomposition, tgsw-functions.cpp:337-348
+            const int shift = 32 - (p + 1) * Bgbit;
+            c64 x[R];
 #pragma unroll
-                for (int s = 0; s < R; s++) {
-                    const int32_t d0 = (int32_t)((tv[s] >> shift) & dmask) - dhalf;
-                    const int32_t d1 = (int32_t)((tv[R + s] >> shift) & dmask) - dhalf;
-                    x[s] = c64{(double)d0, (double)d1};
-                }
-                F::fwd1(x, lane, s_fft, twist, tw1);
-                __syncthreads();
-                F::fwd2(x, lane, s_fft, tw2);
-                __syncthreads();
-                F::fwd3(x, lane, s_fft);
-                __syncthreads();
-                F::fwd4(x, lane, s_fft);
-                const c64* bk = bkrow + (size_t)(c * L + p) * 2 * M;
+            for (int s = 0; s < R; s++) {
+                const int32_t d0 = (int32_t)((tv[s] >> shift) & dmask) - dhalf;
+                const int32_t d1 = (int32_t)((tv[R + s] >> shift) & dmask) - dhalf;
+                x[s] = c64{(double)d0, (double)d1};
+            }
+            F::fwd1(x, lane, s_fft, tw1);
+            wave_sync();
+            F::fwd2(x, lane, s_fft, tw2);
+            wave_sync();
+            F::fwd3(x, lane, s_fft);
+            wave_sync();
+            F::fwd4(x, lane, s_fft);
+            wave_sync();
+            load_bk(i, r);
 #pragma unroll
-                for (int s = 0; s < R; s++) {
-                    cmac(r0[s], x[s], bk[s * 64]);
-                    cmac(r1[s], x[s], bk[M + s * 64]);
-                }
-                __syncthreads();
+            for (int s = 0; s < R; s++) {
+                cmac(r0[s], x[s], bk0[s]);
+                cmac(r1[s], x[s], bk1[s]);
             }
         }
         // inverse transforms and ACC += result (libtfhe/lwe-bootstrapping-functions-fft.cpp:67-76)
@@ -211,12 +230,12 @@ __global__ __launch_bounds__(64, 2) void br_kernel(BrArgs A) {
         for (int cc = 0; cc < 2; cc++) {
             c64* x = cc ? r1 : r0;
             F::inv1(x, lane, s_fft);
-            __syncthreads();
+            wave_sync();
             F::inv2(x, lane, s_fft, tw2);
-            __syncthreads();
+            wave_sync();
             F::inv3(x, lane, s_fft);
-            __syncthreads();
-            F::inv4(x, lane, s_fft, tw1, itwist);
+            wave_sync();
+            F::inv4(x, lane, s_fft, tw1);
             int32_t* acc = s_acc + cc * N;
 #pragma unroll
             for (int s = 0; s < R; s++) {
@@ -224,7 +243,7 @@ __global__ __launch_bounds__(64, 2) void br_kernel(BrArgs A) {
                 acc[j] = (int32_t)((uint32_t)acc[j] + (uint32_t)round_mod32(x[s].re));
                 acc[j + M] = (int32_t)((uint32_t)acc[j + M] + (uint32_t)round_mod32(x[s].im));
             }
-            __syncthreads();
+            wave_sync();
         }
     }
 
@@ -239,6 +258,221 @@ __global__ __launch_bounds__(64, 2) void br_kernel(BrArgs A) {
     }
 }
 
+// Job list for the lockstep kernel: jobs [0, count) are the first bootstrap
+// of every gate; jobs [count, count + n_mux) the second bootstrap of the MUX
+// gates, in gate order. second[g] = job index of gate g's second bootstrap or -1.
+// One block; count is split into contiguous chunks, one per thread.
+__global__ __launch_bounds__(1024) void mux_scan_kernel(const int32_t* __restrict__ op, int count,
+                                                        int32_t* __restrict__ mux_gate, int32_t* __restrict__ second,
+                                                        int32_t* __restrict__ n_jobs) {
+    __shared__ int32_t s_cnt[1024];
+    const int t = threadIdx.x;
+    const int per = (count + 1023) / 1024;
+    const int g0 = min(count, t * per), g1 = min(count, g0 + per);
+    int c = 0;
+    for (int g = g0; g < g1; g++) c += (op[g] == TFHE_AMD_MUX);
+    s_cnt[t] = c;
+    __syncthreads();
+    // inclusive scan (Hillis-Steele)
+    for (int d = 1; d < 1024; d <<= 1) {
+        const int v = t >= d ? s_cnt[t - d] : 0;
+        __syncthreads();
+        s_cnt[t] += v;
+        __syncthreads();
+    }
+    int rank = s_cnt[t] - c;
+    for (int g = g0; g < g1; g++) {
+        if (op[g] == TFHE_AMD_MUX) {
+            mux_gate[rank] = g;
+            second[g] = count + rank;
+            rank++;
+        } else {
+            second[g] = -1;
+        }
+    }
+    if (t == 1023) *n_jobs = count + s_cnt[1023];
+}
+
+// Blind rotation with four gates per workgroup in lockstep (M = 512, i.e.
+// N = 1024). Each wavefront owns one bootstrap; for every digit polynomial of
+// every key row the workgroup loads the 16 KiB of transformed key it needs
+// once into LDS (prefetched one digit ahead into registers) and all four
+// wavefronts read it from there. The accumulator lives in registers in the
+// transform's coefficient layout; the negacyclic rotation is staged through
+// the wavefront's own transform buffer.
+template <int L>
+__global__ __launch_bounds__(256, 2) void br4_kernel(BrArgs A, const int32_t* __restrict__ mux_gate,
+                                                    const int32_t* __restrict__ n_jobs) {
+    constexpr int M = 512;
+    using F = Fft<M>;
+    constexpr int R = F::R, N = 2 * M, KPL = 2 * L;
+    constexpr int LOG2N2 = 11;
+    __shared__ c64 s_fft[4][F::LDS_C64];
+    __shared__ c64 s_bk[2 * M];
+    __shared__ c64 s_tw1[R * 64];
+    __shared__ int16_t s_bara[4][1024];
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int w = tid >> 6;
+    const int njobs = *n_jobs;
+    const int job = blockIdx.x * 4 + w;
+    if (blockIdx.x * 4 >= njobs) return;  // whole block idle (uniform)
+    const bool active = job < njobs;
+    const int n = A.n;
+    int16_t* bara = s_bara[w];
+    c64* fb = s_fft[w];
+    int32_t* fbi = (int32_t*)fb;
+
+    // accumulator: acc[c][h*R + s] = ACC_c[64 s + lane + h M]
+    uint32_t acc[2][2 * R];
+    if (active) {
+        const int gate = job < A.count ? job : mux_gate[job - A.count];
+        const int half = job < A.count ? 0 : 1;
+        const int op = A.op[gate];
+        int32_t k0, ca, cb, cc;
+        gate_coeffs(op, half, k0, ca, cb, cc);
+        const int32_t* a = A.pool + (size_t)A.in0[gate] * (n + 1);
+        const int32_t* b = (op == OP_IDENTITY) ? a : A.pool + (size_t)A.in1[gate] * (n + 1);
+        const int32_t* c = (op == TFHE_AMD_MUX) ? A.pool + (size_t)A.in2[gate] * (n + 1) : a;
+        uint32_t barb = 0;
+        for (int i = lane; i <= n; i += 64) {
+            uint32_t v = (uint32_t)ca * (uint32_t)a[i] + (uint32_t)cb * (uint32_t)b[i] + (uint32_t)cc * (uint32_t)c[i];
+            if (i == n) v += (uint32_t)k0;
+            // modSwitchFromTorus32(v, 2N), libtfhe/numeric-functions.cpp:59-65
+            const uint32_t bar = (v + (1u << (31 - LOG2N2))) >> (32 - LOG2N2);
+            if (i < n) bara[i] = (int16_t)bar;
+            else barb = bar;
+        }
+        // the lane that handled i == n holds barb; broadcast it
+        barb = __shfl(barb, n & 63);
+        // ACC = (0, X^{2N-barb} * [mu...mu]), lwe-bootstrapping-functions-fft.cpp:146-148
+#pragma unroll
+        for (int h = 0; h < 2; h++)
+#pragma unroll
+            for (int s = 0; s < R; s++) {
+                const int j = 64 * s + lane + h * M;
+                acc[0][h * R + s] = 0;
+                acc[1][h * R + s] = (((j + barb) & (2 * N - 1)) < N) ? (uint32_t)A.mu : 0u - (uint32_t)A.mu;
+            }
+    } else {
+        for (int i = lane; i < n; i += 64) bara[i] = 0;
+#pragma unroll
+        for (int q = 0; q < 2 * R; q++) acc[0][q] = acc[1][q] = 0;
+    }
+
+    // per-lane stage-1 twiddles psi^b w_M^{b k1}, shared by the four wavefronts: s_tw1[k1][lane]
+    for (int q = tid; q < R * 64; q += 256) s_tw1[q] = A.tw[(q & 63) * (R + 8) + (q >> 6)];
+    const c64* tw1 = s_tw1 + lane;  // tw1[k1 * 64]
+    const c64 tw2 = A.tw[lane * (R + 8) + R + 1];
+    const uint32_t offset = A.offset;
+    const int Bgbit = A.Bgbit;
+    const uint32_t dmask = (1u << Bgbit) - 1u;
+    const int32_t dhalf = 1 << (Bgbit - 1);
+
+    __syncthreads();
+
+    for (int i = 0; i < n; i++) {
+        const int abar = bara[i];
+        c64 r0[R], r1[R];
+#pragma unroll
+        for (int s = 0; s < R; s++) r0[s] = r1[s] = c64{0.0, 0.0};
+        uint32_t tv[2 * R];
+#pragma unroll 1
+        for (int r = 0; r < KPL; r++) {
+            const int c = r >= L ? 1 : 0, p = r - c * L;
+            __syncthreads();  // previous digit's MAC is done with s_bk
+            {
+                // key digit polynomial (i, r): 2*M contiguous c64, copied global -> LDS
+                // without registers; complete (vmcnt) at the barrier before the MAC
+                const c64* q = A.bkf + ((size_t)i * KPL + r) * 2 * M + tid;
+                c64* d = s_bk + (tid & ~63);
+#pragma unroll
+                for (int u = 0; u < 4; u++)
+                    __builtin_amdgcn_global_load_lds((const void*)(q + u * 256),
+                                                     (__attribute__((address_space(3))) void*)(d + u * 256), 16, 0, 0);
+            }
+            c64 x[R];
+            if (active) {
+                if (p == 0) {
+                    // T = (X^abar - 1) * ACC_c + gadget offset, through the wave's LDS buffer
+#pragma unroll
+                    for (int h = 0; h < 2; h++)
+#pragma unroll
+                        for (int s = 0; s < R; s++) fbi[64 * s + lane + h * M] = (int32_t)(c ? acc[1][h * R + s] : acc[0][h * R + s]);
+                    wave_sync();
+#pragma unroll
+                    for (int h = 0; h < 2; h++)
+#pragma unroll
+                        for (int s = 0; s < R; s++) {
+                            const int j = 64 * s + lane + h * M;
+                            const int idx = (j - abar) & (2 * N - 1);
+                            const uint32_t v = (uint32_t)fbi[idx & (N - 1)];
+                            const uint32_t rot = (idx & N) ? 0u - v : v;
+                            tv[h * R + s] = rot - (c ? acc[1][h * R + s] : acc[0][h * R + s]) + offset;
+                        }
+                    wave_sync();
+                }
+                // digit p of the truncating gadget decomposition, tgsw-functions.cpp:337-348
+                const int shift = 32 - (p + 1) * Bgbit;
+#pragma unroll
+                for (int s = 0; s < R; s++) {
+                    const int32_t d0 = (int32_t)((tv[s] >> shift) & dmask) - dhalf;
+                    const int32_t d1 = (int32_t)((tv[R + s] >> shift) & dmask) - dhalf;
+                    x[s] = c64{(double)d0, (double)d1};
+                }
+                F::template fwd1<64>(x, lane, fb, tw1);
+                wave_sync();
+                F::fwd2(x, lane, fb, tw2);
+                wave_sync();
+                F::fwd3(x, lane, fb);
+                wave_sync();
+                F::fwd4(x, lane, fb);
+            }
+            __syncthreads();  // s_bk holds digit (i, r) for every wavefront
+            if (active) {
+#pragma unroll
+                for (int s = 0; s < R; s++) {
+                    cmac(r0[s], x[s], s_bk[s * 64 + lane]);
+                    cmac(r1[s], x[s], s_bk[M + s * 64 + lane]);
+                }
+            }
+        }
+        if (active) {
+            // inverse transforms and ACC += result (lwe-bootstrapping-functions-fft.cpp:67-76)
+#pragma unroll
+            for (int cc = 0; cc < 2; cc++) {
+                c64* x = cc ? r1 : r0;
+                wave_sync();
+                F::inv1(x, lane, fb);
+                wave_sync();
+                F::inv2(x, lane, fb, tw2);
+                wave_sync();
+                F::inv3(x, lane, fb);
+                wave_sync();
+                F::template inv4<64>(x, lane, fb, tw1);
+#pragma unroll
+                for (int s = 0; s < R; s++) {
+                    acc[cc][s] += (uint32_t)round_mod32(x[s].re);
+                    acc[cc][R + s] += (uint32_t)round_mod32(x[s].im);
+                }
+            }
+        }
+    }
+    if (!active) return;
+    // tLweExtractLweSampleIndex(.., 0, ..), libtfhe/lwe.cpp:41-56: a'_0 = a_0, a'_m = -a_{N-m}
+    int32_t* e = A.ext + (size_t)job * (N + 1);
+#pragma unroll
+    for (int h = 0; h < 2; h++)
+#pragma unroll
+        for (int s = 0; s < R; s++) {
+            const int j = 64 * s + lane + h * M;
+            const uint32_t v = acc[0][h * R + s];
+            e[j == 0 ? 0 : N - j] = (int32_t)(j == 0 ? v : 0u - v);
+        }
+    if (lane == 0) e[N] = (int32_t)acc[1][0];
+}
+
 template <int M>
 __global__ __launch_bounds__(64) void bk_convert_kernel(const int32_t* __restrict__ bk, c64* __restrict__ bkf,
                                                         const c64* __restrict__ tw) {
@@ -248,15 +482,13 @@ __global__ __launch_bounds__(64) void bk_convert_kernel(const int32_t* __restric
     const int lane = threadIdx.x;
     const size_t poly = blockIdx.x;
     const int32_t* p = bk + poly * 2 * M;
-    c64 x[R], tw1[R], twist[R];
+    c64 x[R], tw1[R];
 #pragma unroll
     for (int k = 0; k < R; k++) tw1[k] = tw[lane * (R + 8) + k];
     const c64 tw2 = tw[lane * (R + 8) + R + 1];
 #pragma unroll
-    for (int a = 0; a < R; a++) twist[a] = tw[64 * (R + 8) + a];
-#pragma unroll
     for (int s = 0; s < R; s++) x[s] = c64{(double)p[64 * s + lane], (double)p[64 * s + lane + M]};
-    F::fwd1(x, lane, s_fft, twist, tw1);
+    F::fwd1(x, lane, s_fft, tw1);
     __syncthreads();
     F::fwd2(x, lane, s_fft, tw2);
     __syncthreads();
@@ -268,17 +500,16 @@ __global__ __launch_bounds__(64) void bk_convert_kernel(const int32_t* __restric
 }
 
 // ext: [2*count][nin+1]; op may be null (no MUX). Writes abar[i][cpad] = a_i + prec_offset
-// and binit[g] = b; if out_init is set, also writes the keyswitch start row
-// (0,...,0,b) of every gate (needed when the keyswitch accumulates atomically).
-__global__ __launch_bounds__(256) void ks_prep_kernel(const int32_t* __restrict__ ext, const int32_t* __restrict__ op,
-                                                      int count, int nin, int32_t mu, uint32_t prec_offset,
-                                                      uint32_t* __restrict__ abar, int cpad,
-                                                      int32_t* __restrict__ binit, int32_t* out, const int32_t* out_idx,
-                                                      int n) {
+// (masks transposed so the keyswitch reads one word per gate per i) and binit[g] = b.
+__global__ __launch_bounds__(256) void ks_prep_kernel(const int32_t* __restrict__ ext,
+                                                      const int32_t* __restrict__ second, int nin, int32_t mu,
+                                                      uint32_t prec_offset, uint32_t* __restrict__ abar, int cpad,
+                                                      int32_t* __restrict__ binit) {
     const int g = blockIdx.x;
-    const bool mux = op && op[g] == TFHE_AMD_MUX;
+    const int j2 = second ? second[g] : -1;
+    const bool mux = j2 >= 0;
     const int32_t* e0 = ext + (size_t)g * (nin + 1);
-    const int32_t* e1 = ext + (size_t)(count + g) * (nin + 1);
+    const int32_t* e1 = ext + (size_t)(mux ? j2 : g) * (nin + 1);
     for (int i = threadIdx.x; i <= nin; i += blockDim.x) {
         uint32_t v = (uint32_t)e0[i];
         // libtfhe/boot-gates.cpp:358-361: (0, 1/8) + u1 + u2
@@ -286,19 +517,15 @@ __global__ __launch_bounds__(256) void ks_prep_kernel(const int32_t* __restrict_
         if (i < nin) abar[(size_t)i * cpad + g] = v + prec_offset;
         else binit[g] = (int32_t)v;
     }
-    if (out) {
-        int32_t* o = out + (size_t)(out_idx ? out_idx[g] : g) * (n + 1);
-        __syncthreads();
-        for (int c = threadIdx.x; c <= n; c += blockDim.x) o[c] = (c == n) ? binit[g] : 0;
-    }
 }
 
-__global__ __launch_bounds__(256) void sum_halves_kernel(const int32_t* ext, const int32_t* op, int count, int nin,
+__global__ __launch_bounds__(256) void sum_halves_kernel(const int32_t* ext, const int32_t* second, int nin,
                                                          int32_t mu, int32_t* out) {
     const int g = blockIdx.x;
-    const bool mux = op && op[g] == TFHE_AMD_MUX;
+    const int j2 = second[g];
+    const bool mux = j2 >= 0;
     const int32_t* e0 = ext + (size_t)g * (nin + 1);
-    const int32_t* e1 = ext + (size_t)(count + g) * (nin + 1);
+    const int32_t* e1 = ext + (size_t)(mux ? j2 : g) * (nin + 1);
     int32_t* o = out + (size_t)g * (nin + 1);
     for (int i = threadIdx.x; i <= nin; i += blockDim.x) {
         uint32_t v = (uint32_t)e0[i];
@@ -307,43 +534,71 @@ __global__ __launch_bounds__(256) void sum_halves_kernel(const int32_t* ext, con
     }
 }
 
-template <int G, bool ATOMIC>
-__global__ __launch_bounds__(64) void ks_kernel(const int32_t* __restrict__ ksk, const uint32_t* __restrict__ abar,
-                                                int cpad, const int32_t* __restrict__ binit, int count, int nin, int t,
-                                                int n, int npad, int i_per_split, int32_t* out,
-                                                const int32_t* __restrict__ out_idx) {
-    const int lane = threadIdx.x;
-    const int col = blockIdx.x * 64 + lane;
-    const int g0 = blockIdx.y * G;
+// Plain device-to-device copy (ciphertext pools); 16-byte vector accesses.
+__global__ __launch_bounds__(256) void copy_kernel(const int4* __restrict__ src, int4* __restrict__ dst, size_t n16) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+// Keyswitch with lanes over gates: a block covers WAVES*64 gates, one chunk of
+// 64 output coefficients and a range of input coefficients i. For every
+// (i, j) the block stages the three non-trivial key rows of its chunk in LDS
+// (next to a zero row for digit 0); each lane then subtracts row[digit] of its
+// own gate with broadcast 16-byte reads, so the key is read once per block
+// and each gate-coefficient costs one integer subtraction. Partial sums go to
+// part[split][gate][npad]; ks_reduce_kernel adds the splits.
+template <int WAVES>
+__global__ __launch_bounds__(WAVES * 64) void ks_kernel(const int32_t* __restrict__ ksk,
+                                                        const uint32_t* __restrict__ abar, int cpad, int nin, int t,
+                                                        int npad, int i_per_split, int32_t* __restrict__ part) {
+    __shared__ int4 tab[2][4][16];
+    const int tid = threadIdx.x;
+    const int g = blockIdx.y * (WAVES * 64) + tid;  // < cpad by construction
+    const int chunk = blockIdx.x;
     const int i0 = blockIdx.z * i_per_split;
     const int i1 = min(nin, i0 + i_per_split);
-    int32_t acc[G];
+    if (tid < 32) tab[tid >> 4][0][tid & 15] = make_int4(0, 0, 0, 0);
+    int4 acc[16];
 #pragma unroll
-    for (int gg = 0; gg < G; gg++) acc[gg] = 0;
+    for (int q = 0; q < 16; q++) acc[q] = make_int4(0, 0, 0, 0);
+    int step = 0;
     for (int i = i0; i < i1; i++) {
-        uint32_t ab[G];
+        const uint32_t a = abar[(size_t)i * cpad + g];
+        for (int j = 0; j < t; j++, step++) {
+            const int buf = step & 1;
+            if (tid < 48) {
+                const int h = tid >> 4, q = tid & 15;
+                tab[buf][1 + h][q] =
+                    *(const int4*)(ksk + ((size_t)(i * t + j) * 3 + h) * npad + chunk * 64 + q * 4);
+            }
+            __syncthreads();
+            const int d = (a >> (30 - 2 * j)) & 3;
+            const int4* row = tab[buf][d];
 #pragma unroll
-        for (int gg = 0; gg < G; gg++) ab[gg] = abar[(size_t)i * cpad + g0 + gg];
-        for (int j = 0; j < t; j++) {
-            const int32_t* row = ksk + ((size_t)(i * t + j) * 3) * npad + col;
-            const int32_t v1 = row[0], v2 = row[npad], v3 = row[2 * npad];
-            const int shift = 30 - 2 * j;
-#pragma unroll
-            for (int gg = 0; gg < G; gg++) {
-                const uint32_t d = (ab[gg] >> shift) & 3u;
-                const int32_t v = (d & 2u) ? ((d & 1u) ? v3 : v2) : ((d & 1u) ? v1 : 0);
-                acc[gg] = (int32_t)((uint32_t)acc[gg] - (uint32_t)v);
+            for (int q = 0; q < 16; q++) {
+                const int4 v = row[q];
+                acc[q].x = (int32_t)((uint32_t)acc[q].x - (uint32_t)v.x);
+                acc[q].y = (int32_t)((uint32_t)acc[q].y - (uint32_t)v.y);
+                acc[q].z = (int32_t)((uint32_t)acc[q].z - (uint32_t)v.z);
+                acc[q].w = (int32_t)((uint32_t)acc[q].w - (uint32_t)v.w);
             }
         }
     }
-    if (col > n) return;
+    int4* dst = (int4*)(part + ((size_t)blockIdx.z * cpad + g) * npad + chunk * 64);
 #pragma unroll
-    for (int gg = 0; gg < G; gg++) {
-        const int g = g0 + gg;
-        if (g >= count) break;
-        int32_t* dst = out + (size_t)(out_idx ? out_idx[g] : g) * (n + 1) + col;
-        if (ATOMIC) atomicAdd(dst, acc[gg]);
-        else *dst = (int32_t)((col == n ? (uint32_t)binit[g] : 0u) + (uint32_t)acc[gg]);
+    for (int q = 0; q < 16; q++) dst[q] = acc[q];
+}
+
+// out[gate] = (0, ..., 0, b) + sum over splits of the partial keyswitch sums
+__global__ __launch_bounds__(256) void ks_reduce_kernel(const int32_t* __restrict__ part, int splits, int cpad,
+                                                        int npad, const int32_t* __restrict__ binit, int n,
+                                                        int32_t* __restrict__ out, const int32_t* __restrict__ out_idx) {
+    const int g = blockIdx.x;
+    int32_t* o = out + (size_t)(out_idx ? out_idx[g] : g) * (n + 1);
+    for (int c = threadIdx.x; c <= n; c += blockDim.x) {
+        uint32_t v = (c == n) ? (uint32_t)binit[g] : 0u;
+        for (int sp = 0; sp < splits; sp++) v += (uint32_t)part[((size_t)sp * cpad + g) * npad + c];
+        o[c] = (int32_t)v;
     }
 }
 
@@ -364,7 +619,7 @@ struct Ctx {
     Scratch scratch;
     void* h_pinned = nullptr;
     size_t h_cap = 0;
-    int br_mode = 0, br_bpc = 0, ks_g = 0;
+    int br_mode = 0, br_bpc = 0;
     // optional per-launch timing with HIP events on the launch stream
     bool prof = false;
     struct Ev { hipEvent_t e[3]; };
@@ -459,50 +714,64 @@ BrArgs base_args(const tfhe_amd_device_key* dk) {
     return A;
 }
 
-template <int G, bool AT>
-void launch_ks_g(const tfhe_amd_device_key* dk, const uint32_t* abar, int cpad, const int32_t* binit, int count,
-                 int splits, int32_t* out, const int32_t* out_idx, hipStream_t s) {
-    const int nin = dk->P.k * dk->P.N;
-    const int ips = (nin + splits - 1) / splits;
-    dim3 grid((dk->P.n + 1 + 63) / 64, (count + G - 1) / G, splits);
-    hipLaunchKernelGGL((ks_kernel<G, AT>), grid, dim3(64), 0, s, dk->ksk, abar, cpad, binit, count, nin, dk->P.ks_t,
-                       dk->P.n, dk->npad, ips, out, out_idx);
+struct KsPlan {
+    int waves, tiles, chunks, splits, cpad;
+};
+
+KsPlan ks_plan(const HostParams& P, int count) {
+    KsPlan k;
+    k.waves = count >= 256 ? 4 : 1;
+    const int nt = 64 * k.waves;
+    k.tiles = (count + nt - 1) / nt;
+    k.cpad = k.tiles * nt;
+    k.chunks = (P.n + 1 + 63) / 64;
+    const int blocks = k.tiles * k.chunks;
+    k.splits = 1;
+    // enough wavefronts to fill the device (about 8 per CU on 256 CUs)
+    while (blocks * k.splits * k.waves < 2048 && k.splits < 64) k.splits *= 2;
+    return k;
 }
 
-// Keyswitch of ext rows (sum of halves for MUX) into out rows. Uses the
-// region of scratch starting at `scr`.
-int run_keyswitch(Ctx& C, const tfhe_amd_device_key* dk, const int32_t* ext, const int32_t* d_op, int count,
-                  int32_t* out, const int32_t* out_idx, char* scr, hipStream_t s) {
+size_t ks_scratch_bytes(const HostParams& P, int count) {
+    const KsPlan k = ks_plan(P, count);
+    const int npad = ((P.n + 1 + 63) / 64) * 64;
+    return align256(sizeof(uint32_t) * (size_t)P.k * P.N * k.cpad) + align256(sizeof(int32_t) * (size_t)k.cpad) +
+           align256(sizeof(int32_t) * (size_t)k.splits * k.cpad * npad);
+}
+
+// Keyswitch of ext rows (sum of halves for MUX) into out rows, using the
+// scratch region starting at `scr` (ks_scratch_bytes(count) bytes).
+int run_keyswitch(const tfhe_amd_device_key* dk, const int32_t* ext, const int32_t* second, int count, int32_t* out,
+                  const int32_t* out_idx, char* scr, hipStream_t s) {
     const HostParams& P = dk->P;
     const int nin = P.k * P.N;
-    int G = C.ks_g;
-    if (G <= 0) G = count >= 512 ? 32 : (count >= 32 ? 8 : 1);
-    const int cpad = ((count + 31) / 32) * 32 + 32;
+    const KsPlan k = ks_plan(P, count);
     uint32_t* abar = (uint32_t*)scr;
-    int32_t* binit = (int32_t*)(scr + align256(sizeof(uint32_t) * (size_t)nin * cpad));
-    // enough blocks to fill the device: split the input range when there are few gate tiles
-    const int col_tiles = (P.n + 1 + 63) / 64;
-    const int tiles = col_tiles * ((count + G - 1) / G);
-    int splits = 1;
-    while (tiles * splits < 1024 && splits < 64) splits *= 2;
-    const bool atomic = splits > 1;
+    char* p = scr + align256(sizeof(uint32_t) * (size_t)nin * k.cpad);
+    int32_t* binit = (int32_t*)p;
+    int32_t* part = (int32_t*)(p + align256(sizeof(int32_t) * (size_t)k.cpad));
     const uint32_t prec_offset = 1u << (32 - (1 + P.ks_basebit * P.ks_t));
-    hipLaunchKernelGGL(ks_prep_kernel, dim3(count), dim3(256), 0, s, ext, d_op, count, nin, 1 << 29, prec_offset,
-                       abar, cpad, binit, atomic ? out : nullptr, out_idx, P.n);
-    if (G == 32) { if (atomic) launch_ks_g<32, true>(dk, abar, cpad, binit, count, splits, out, out_idx, s); else launch_ks_g<32, false>(dk, abar, cpad, binit, count, splits, out, out_idx, s); }
-    else if (G == 8) { if (atomic) launch_ks_g<8, true>(dk, abar, cpad, binit, count, splits, out, out_idx, s); else launch_ks_g<8, false>(dk, abar, cpad, binit, count, splits, out, out_idx, s); }
-    else { if (atomic) launch_ks_g<1, true>(dk, abar, cpad, binit, count, splits, out, out_idx, s); else launch_ks_g<1, false>(dk, abar, cpad, binit, count, splits, out, out_idx, s); }
+    hipLaunchKernelGGL(ks_prep_kernel, dim3(count), dim3(256), 0, s, ext, second, nin, 1 << 29, prec_offset, abar,
+                       k.cpad, binit);
+    const int ips = (nin + k.splits - 1) / k.splits;
+    dim3 grid(k.chunks, k.tiles, k.splits);
+    if (k.waves == 4)
+        hipLaunchKernelGGL(ks_kernel<4>, grid, dim3(256), 0, s, dk->ksk, abar, k.cpad, nin, P.ks_t, dk->npad, ips, part);
+    else
+        hipLaunchKernelGGL(ks_kernel<1>, grid, dim3(64), 0, s, dk->ksk, abar, k.cpad, nin, P.ks_t, dk->npad, ips, part);
+    hipLaunchKernelGGL(ks_reduce_kernel, dim3(count), dim3(256), 0, s, part, k.splits, k.cpad, dk->npad, binit, P.n,
+                       out, out_idx);
     HIP_TRY(hipGetLastError());
     return 0;
 }
 
-size_t ks_scratch_bytes(const HostParams& P, int count) {
-    const int cpad = ((count + 31) / 32) * 32 + 32;
-    return align256(sizeof(uint32_t) * (size_t)P.k * P.N * cpad) + align256(sizeof(int32_t) * (size_t)cpad);
-}
-
-size_t ext_bytes(const HostParams& P, int count) {
+// Per-batch bootstrap workspace: extracted samples of every job, then the
+// MUX job list (mux_gate[count], second[count], n_jobs).
+size_t ext_rows_bytes(const HostParams& P, int count) {
     return align256(sizeof(int32_t) * 2 * (size_t)count * (P.k * P.N + 1));
+}
+size_t ext_bytes(const HostParams& P, int count) {
+    return ext_rows_bytes(P, count) + 2 * align256(sizeof(int32_t) * (size_t)count) + 256;
 }
 
 // Core: gates on device-resident data. `scr` must hold ext_bytes(count) +
@@ -515,7 +784,14 @@ int gates_device_locked(Ctx& C, tfhe_amd_device_key* dk, int count, const int32_
     const HostParams& P = dk->P;
     const size_t eb = ext_bytes(P, count);
     int32_t* ext = (int32_t*)scr;
+    char* jp = scr + ext_rows_bytes(P, count);
+    int32_t* mux_gate = (int32_t*)jp;
+    int32_t* second = (int32_t*)(jp + align256(sizeof(int32_t) * (size_t)count));
+    int32_t* n_jobs = (int32_t*)(jp + 2 * align256(sizeof(int32_t) * (size_t)count));
+    hipLaunchKernelGGL(mux_scan_kernel, dim3(1), dim3(1024), 0, s, d_op, count, mux_gate, second, n_jobs);
     BrArgs A = base_args(dk);
+    A.mux_gate = mux_gate;
+    A.n_jobs = n_jobs;
     A.op = d_op;
     A.in0 = d_in0;
     A.in1 = d_in1;
@@ -529,15 +805,22 @@ int gates_device_locked(Ctx& C, tfhe_amd_device_key* dk, int count, const int32_
         for (auto& e : ev.e) HIP_TRY(hipEventCreate(&e));
         HIP_TRY(hipEventRecord(ev.e[0], s));
     }
-    launch_br(dk, A, 2 * count, s);
+    const bool lockstep = dk->M == 512 && C.br_mode != 1;
+    if (lockstep) {
+        const int blocks = (2 * count + 3) / 4;
+        if (dk->P.l == 3) hipLaunchKernelGGL(br4_kernel<3>, dim3(blocks), dim3(256), 0, s, A, mux_gate, n_jobs);
+        else hipLaunchKernelGGL(br4_kernel<2>, dim3(blocks), dim3(256), 0, s, A, mux_gate, n_jobs);
+    } else {
+        launch_br(dk, A, 2 * count, s);
+    }
     HIP_TRY(hipGetLastError());
     if (C.prof) HIP_TRY(hipEventRecord(ev.e[1], s));
     int rc = 0;
     if (d_ext_out) {
-        hipLaunchKernelGGL(sum_halves_kernel, dim3(count), dim3(256), 0, s, ext, d_op, count, P.k * P.N, mu, d_ext_out);
+        hipLaunchKernelGGL(sum_halves_kernel, dim3(count), dim3(256), 0, s, ext, second, P.k * P.N, mu, d_ext_out);
         HIP_TRY(hipGetLastError());
     } else {
-        rc = run_keyswitch(C, dk, ext, d_op, count, d_pool, d_out, scr + eb, s);
+        rc = run_keyswitch(dk, ext, second, count, d_pool, d_out, scr + eb, s);
     }
     if (C.prof) {
         HIP_TRY(hipEventRecord(ev.e[2], s));
@@ -778,7 +1061,7 @@ int engine_keyswitch_host(tfhe_amd_device_key* dk, const int32_t* in, int32_t* o
     memcpy(h, in, sizeof(int32_t) * nx);
     char* d = (char*)C.scratch.d;
     HIP_TRY(hipMemcpyAsync(d, h, sizeof(int32_t) * nx, hipMemcpyHostToDevice, C.stream));
-    if (int rc = run_keyswitch(C, dk, (const int32_t*)d, nullptr, 1, (int32_t*)(d + in_b), nullptr, d + in_b + out_b, C.stream))
+    if (int rc = run_keyswitch(dk, (const int32_t*)d, nullptr, 1, (int32_t*)(d + in_b), nullptr, d + in_b + out_b, C.stream))
         return rc;
     HIP_TRY(hipMemcpyAsync(h, d + in_b, sizeof(int32_t) * w, hipMemcpyDeviceToHost, C.stream));
     HIP_TRY(hipStreamSynchronize(C.stream));
@@ -844,7 +1127,7 @@ TFHE_AMD_EXPORT int32_t tfhe_amd_keyswitch_device(const TFheGateBootstrappingClo
     std::lock_guard<std::mutex> g(C.mu);
     // run_keyswitch reads ext rows [2*count][..]; with op == null only the first count are used
     if (int rc = ensure_scratch(C, ks_scratch_bytes(dk->P, count))) return rc;
-    return run_keyswitch(C, dk, d_in, nullptr, count, d_out, nullptr, (char*)C.scratch.d,
+    return run_keyswitch(dk, d_in, nullptr, count, d_out, nullptr, (char*)C.scratch.d,
                          stream ? (hipStream_t)stream : C.stream);
 }
 
@@ -853,7 +1136,7 @@ TFHE_AMD_EXPORT void tfhe_amd_set_launch_config(int32_t br_mode, int32_t br_bloc
     std::lock_guard<std::mutex> g(C.mu);
     C.br_mode = br_mode;
     C.br_bpc = br_blocks_per_cu;
-    C.ks_g = (ks_gates_per_block == 1 || ks_gates_per_block == 8 || ks_gates_per_block == 32) ? ks_gates_per_block : 0;
+    (void)ks_gates_per_block;
 }
 
 TFHE_AMD_EXPORT int32_t tfhe_amd_get_launch_config(int32_t* br_mode, int32_t* br_blocks_per_cu, int32_t* ks_gates_per_block) {
@@ -861,7 +1144,7 @@ TFHE_AMD_EXPORT int32_t tfhe_amd_get_launch_config(int32_t* br_mode, int32_t* br
     std::lock_guard<std::mutex> g(C.mu);
     if (br_mode) *br_mode = C.br_mode;
     if (br_blocks_per_cu) *br_blocks_per_cu = C.br_bpc;
-    if (ks_gates_per_block) *ks_gates_per_block = C.ks_g;
+    if (ks_gates_per_block) *ks_gates_per_block = 0;
     return 0;
 }
 
@@ -890,6 +1173,19 @@ TFHE_AMD_EXPORT int32_t tfhe_amd_profile_read(double* br_ms, double* ks_ms, int3
     if (launches) *launches = (int32_t)C.ev.size();
     C.ev.clear();
     return rc;
+}
+
+TFHE_AMD_EXPORT int32_t tfhe_amd_copy_device(void* dst, const void* src, size_t bytes, void* stream) {
+    if (bytes % 16 || ((uintptr_t)dst | (uintptr_t)src) % 16) return set_error("copy_device: 16-byte multiples only");
+    Ctx& C = ctx();
+    {
+        std::lock_guard<std::mutex> g(C.mu);
+        if (int rc = ensure_ready(C)) return rc;
+    }
+    hipStream_t s = stream ? (hipStream_t)stream : C.stream;
+    hipLaunchKernelGGL(copy_kernel, dim3(2048), dim3(256), 0, s, (const int4*)src, (int4*)dst, bytes / 16);
+    HIP_TRY(hipGetLastError());
+    return 0;
 }
 
 TFHE_AMD_EXPORT void* tfhe_amd_malloc(size_t bytes) {

@@ -58,6 +58,26 @@ TA_HD void cmac(c64& acc, c64 a, c64 b) {
     acc.im = fma(a.im, b.re, acc.im);
 }
 
+// cos(pi k / 32), k = 0..16 (sin(pi k/32) = cos(pi (16-k)/32))
+constexpr double kCos32[17] = {
+    1,
+    0.995184726672196886231,
+    0.980785280403230449119,
+    0.956940335732208864931,
+    0.923879532511286756101,
+    0.881921264348355029715,
+    0.831469612302545237081,
+    0.773010453362736960797,
+    0.707106781186547524382,
+    0.634393284163645498203,
+    0.555570233019602224757,
+    0.471396736825997648572,
+    0.382683432365089771696,
+    0.290284677254462367618,
+    0.19509032201612826787,
+    0.0980171403295606019347,
+    -2.50827880633416601173e-20};
+
 constexpr double kC8 = 0.92387953251128675613;  // cos(pi/8)
 constexpr double kS8 = 0.38268343236508977173;  // sin(pi/8)
 constexpr double kH = 0.70710678118654752440;   // cos(pi/4)
@@ -142,16 +162,22 @@ struct Fft {
     TA_HD static int a1(int k1, int b) { return k1 * ROW + b; }
     TA_HD static int a2(int k1, int b0, int k2a) { return k1 * ROW + k2a * 9 + b0; }
 
+    // psi^{64a} = exp(i pi 32 a / M) = cos/sin(pi k / 32) with k = 16 a / R: a compile-time constant
+    TA_HD static constexpr c64 twist(int a) {
+        return c64{kCos32[a * (16 / R)], kCos32[16 - a * (16 / R)]};
+    }
     // ---- forward ----
-    // x[a] = p_{64a+b} + i p_{64a+b+M} (untwisted); twist[a] = psi^{64a}; tw1[k1] = psi^b w_M^{b k1}
-    TA_HD static void fwd1(c64* x, int b, c64* lds, const c64* twist, const c64* tw1) {
+    // x[a] = p_{64a+b} + i p_{64a+b+M} (untwisted); tw1[k1] = psi^b w_M^{b k1}
+    // tw1[k1 * TS] = psi^b w_M^{b k1} (TS = 1: per-lane array; TS = 64: shared table column)
+    template <int TS = 1>
+    TA_HD static void fwd1(c64* x, int b, c64* lds, const c64* tw1) {
 #pragma unroll
-        for (int a = 1; a < R; a++) x[a] = cmul(x[a], twist[a]);
+        for (int a = 1; a < R; a++) x[a] = cmul(x[a], twist(a));
         dif<R, 1>(x);
 #pragma unroll
         for (int j = 0; j < R; j++) {
             const int k1 = bitrev(j, LOGR);
-            lds[a1(k1, b)] = cmul(x[j], tw1[k1]);
+            lds[a1(k1, b)] = cmul(x[j], tw1[k1 * TS]);
         }
     }
     // Powers w^0..w^7 of w = w_64^{b0}, b0 = lane & 7 (the stage-2a twiddles),
@@ -228,16 +254,20 @@ struct Fft {
 #pragma unroll
             for (int b1 = 0; b1 < 8; b1++) lds[a1(c * G + g, b0 + 8 * b1)] = x[g * 8 + b1];
     }
-    // itwist[a] = conj(psi^{64a}) / M; result x[a] = p_{64a+b} + i p_{64a+b+M}
-    TA_HD static void inv4(c64* x, int b, const c64* lds, const c64* tw1, const c64* itwist) {
+    // result x[a] = p_{64a+b} + i p_{64a+b+M} (scaled by 1/M and untwisted by conj(psi^{64a}))
+    template <int TS = 1>
+    TA_HD static void inv4(c64* x, int b, const c64* lds, const c64* tw1) {
 #pragma unroll
         for (int j = 0; j < R; j++) {
             const int k1 = bitrev(j, LOGR);
-            x[j] = cmulc(lds[a1(k1, b)], tw1[k1]);
+            x[j] = cmulc(lds[a1(k1, b)], tw1[k1 * TS]);
         }
         dit<R, -1>(x);
 #pragma unroll
-        for (int a = 0; a < R; a++) x[a] = cmul(x[a], itwist[a]);
+        for (int a = 0; a < R; a++) {
+            const c64 t = twist(a);
+            x[a] = cmul(x[a], c64{t.re / M, -t.im / M});
+        }
     }
 };
 
