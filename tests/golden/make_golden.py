@@ -1,0 +1,72 @@
+#!/usr/bin/env python3
+"""Writes the golden vectors in tests/golden/ from the reference build.
+
+Needs oracle/_ref/ref_harness (make -C oracle ref, from the reference's own
+sources); run in a container where those sources exist. The vectors are data
+only: hashes of the reference's seed-0 keys, a few reference keyswitch
+input/output pairs, and the reference's output phases for gates on
+encryptions that the oracle replays from the same seed.
+
+    python tests/golden/make_golden.py
+"""
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(HERE))
+import oracle_py as O  # noqa: E402
+
+
+def run(*args):
+    subprocess.run([O.REF_HARNESS] + [str(a) for a in args], check=True)
+
+
+def main():
+    if not os.path.exists(O.REF_HARNESS):
+        subprocess.run(["make", "-s", "-j8", "-C", O.ORACLE_DIR, "ref"], check=True)
+    P = O.p128()
+    with tempfile.TemporaryDirectory() as tmp:
+        # keys (seed 0)
+        run("keys", 0, tmp)
+        R = O.load_ref_keys(P, tmp)
+        K = O.keygen(P, 0)
+        bdiff = K.bk[:, :, 1, :].astype(np.int64) - R.bk[:, :, 1, :]
+        np.savez_compressed(
+            os.path.join(HERE, "ref_p128_seed0_keys.npz"),
+            lwe_key_hash=np.uint64(O.fnv1a(R.lwe_key)),
+            tlwe_key_hash=np.uint64(O.fnv1a(R.tlwe_key)),
+            ksk_hash=np.uint64(O.fnv1a(R.ksk)),
+            bk_mask_hash=np.uint64(O.fnv1a(np.ascontiguousarray(R.bk[:, :, 0, :]))),
+            bk_b_row0=np.ascontiguousarray(R.bk[0, 0, 1, :]),
+            bk_b_diff_max=np.int64(np.abs(bdiff).max()),
+            bk_b_diff_frac=np.float64((bdiff != 0).mean()),
+        )
+        # keyswitch pairs
+        path = os.path.join(tmp, "ks.bin")
+        run("ks", 0, 4, path)
+        raw = np.fromfile(path, np.int32)
+        cnt, nx, n = raw[:3]
+        body = raw[3:].reshape(cnt, nx + 1 + n + 1)
+        np.savez_compressed(os.path.join(HERE, "ref_p128_seed0_keyswitch.npz"),
+                            inputs=body[:, :nx + 1].copy(), outputs=body[:, nx + 1:].copy())
+        # gates: 1000 NAND, 256 uniform over the six two-input gates, 64 MUX
+        for name, count, ops in (("nand1000", 1000, "a"), ("rand6_256", 256, "rand6"), ("mux64", 64, "k")):
+            path = os.path.join(tmp, name + ".bin")
+            run("gates", 0, count, ops, path)
+            G = O.read_ref_gates(path)
+            nin = 3 if ops == "k" else 2
+            inputs = np.ascontiguousarray(G["cts"][:, :nin, :])
+            np.savez_compressed(os.path.join(HERE, f"ref_p128_seed0_{name}.npz"),
+                                ops=G["ops"].astype(np.int8), bits=G["bits"].astype(np.int8),
+                                phase=G["phase"], inputs_hash=np.uint64(O.fnv1a(inputs)),
+                                ref_ns=G["ns"])
+    for f in sorted(os.listdir(HERE)):
+        print(f, os.path.getsize(os.path.join(HERE, f)))
+
+
+if __name__ == "__main__":
+    main()

@@ -206,3 +206,59 @@ def test_bootsgate_batch_dedup_and_alias(p128):
         assert np.array_equal(got, ref)
     finally:
         H.close()
+
+
+def _gold(name):
+    import os
+    return np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", name), allow_pickle=False)
+
+
+def test_keyswitch_matches_reference_vectors(p128):
+    """Reference lweKeySwitch outputs (seed-0 keys, which this library's key
+    generation reproduces bit-exactly) on the GPU."""
+    T, P, sk, keys = p128
+    g = _gold("ref_p128_seed0_keyswitch.npz")
+    got = T.keyswitch(sk.cloud, P, g["inputs"])
+    assert np.array_equal(got, g["outputs"])
+
+
+@pytest.mark.parametrize("name", ["nand1000", "rand6_256", "mux64"])
+def test_gates_match_reference_outputs(name):
+    """All reference gate outputs of the fixtures: same decrypted bits (and
+    truth table) from a fresh seed-0 key set and the replayed encryptions;
+    phases within 2^-5 of the reference's (the difference is bootstrapping
+    noise, see test_oracle_cpu.test_gates_match_reference)."""
+    import tfhe_amd as T
+    g = _gold(f"ref_p128_seed0_{name}.npz")
+    P = T.default_params(110)
+    sk = T.SecretKey(P, seed=0)
+    ops, bits = g["ops"].astype(np.int32), g["bits"].astype(np.int32)
+    nin = 3 if name.startswith("mux") else 2
+    cts = np.stack([sk.encrypt(bits[gi][:nin]) for gi in range(len(ops))])
+    assert O.fnv1a(np.ascontiguousarray(cts)) == int(g["inputs_hash"])
+    out = T.run_gates(sk.cloud, ops, cts[:, 0], cts[:, 1], cts[:, 2] if nin == 3 else None)
+    ph = sk.phases(out)
+    exp = np.array([O.TRUTH[O.GATES[o]](*b) for o, b in zip(ops, bits)])
+    assert np.array_equal((ph > 0).astype(int), exp)
+    assert np.array_equal((g["phase"] > 0).astype(int), exp)
+    diff = (ph.astype(np.int64) - g["phase"].astype(np.int64) + 2 ** 31) % 2 ** 32 - 2 ** 31
+    assert np.abs(diff).max() < 2 ** 27
+    # the noise distributions agree: output phase spread around +-1/8
+    dev_gpu = np.abs(ph.astype(np.int64)) - 2 ** 29
+    dev_ref = np.abs(g["phase"].astype(np.int64)) - 2 ** 29
+    if len(ops) >= 256:
+        ratio = dev_gpu.std() / dev_ref.std()
+        assert 0.8 < ratio < 1.25, ratio
+    sk.close()
+
+
+def test_c_program_gpu(tmp_path):
+    import os
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    exe = str(tmp_path / "c_binding")
+    subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", "-o", exe, os.path.join(root, "tests", "c", "c_binding.c"),
+                    "-I", os.path.join(root, "include"), "-L", os.path.join(root, "tfhe_amd"), "-ltfhe_amd",
+                    "-Wl,-rpath," + os.path.join(root, "tfhe_amd")], check=True)
+    r = subprocess.run([exe, "gpu"], capture_output=True, text=True, timeout=150)
+    assert r.returncode == 0, r.stdout + r.stderr

@@ -1,0 +1,97 @@
+"""Host-side checks of libtfhe_amd.so that need no GPU: every symbol the
+public header declares is exported, key generation and encryption replay the
+reference RNG bit-exactly (same as the oracle), the non-bootstrapped gates,
+argument errors, and a C program built against include/tfhe_amd.h."""
+import ctypes
+import os
+import re
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+import oracle_py as O
+import tfhe_amd as T
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "tfhe_amd.h")
+
+
+def declared_symbols():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"^TFHE_AMD_EXPORT[^;(]*?\b(\w+)\s*\(", text, re.M)))
+
+
+def test_every_declared_symbol_is_exported():
+    syms = declared_symbols()
+    assert len(syms) > 60
+    L = T.lib()
+    missing = [s for s in syms if not hasattr(L, s)]
+    assert not missing, missing
+
+
+def test_keygen_matches_oracle_bit_exact():
+    P = T.default_params(110)
+    sk = T.SecretKey(P, seed=0)
+    K = sk.export()
+    R = O.keygen(O.p128(), 0)
+    for k in ("lwe_key", "tlwe_key", "ksk", "bk"):
+        assert np.array_equal(K[k], getattr(R, k)), k
+    # encryption continues the same generator
+    bits = np.array([1, 0, 0, 1, 1], np.int32)
+    cts = sk.encrypt(bits)
+    for b, c in zip(bits, cts):
+        assert np.array_equal(c, O.boots_encrypt(R, b))
+    assert np.array_equal(sk.decrypt(cts), bits)
+    sk.close()
+
+
+def test_parameter_presets():
+    P = T.default_params(110)
+    g = P.ptr.contents.tgsw_params.contents
+    assert (P.n, P.N, P.k, P.l, P.Bgbit, P.ks_t, P.ks_basebit) == (630, 1024, 1, 3, 7, 8, 2)
+    assert [g.h[i] for i in range(3)] == [1 << 25, 1 << 18, 1 << 11]
+    assert g.offset == (1 << 31) + (1 << 24) + (1 << 17)
+    P80 = T.default_params(80)
+    assert (P80.n, P80.l, P80.Bgbit) == (500, 2, 10)
+    Pb = T.big_params()
+    assert (Pb.n, Pb.N, Pb.l, Pb.Bgbit) == (750, 2048, 2, 10)
+
+
+def test_lambda_above_128_aborts():
+    code = ("import sys; sys.path.insert(0, %r); import tfhe_amd as T; "
+            "T.lib().new_default_gate_bootstrapping_parameters(200)") % ROOT
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True)
+    assert r.returncode != 0 and "only implemented for 80bit and 128bit" in r.stderr
+
+
+def test_batch_rejects_bad_gate_code_without_touching_the_gpu():
+    P = T.default_params(110)
+    sk = T.SecretKey(P, seed=1)
+    H = T.HostCiphertexts(P, 2)
+    ops = np.array([0, 42], np.int32)
+    arr = (T.LSP * 2)(H.item(0), H.item(1))
+    rc = T.lib().bootsGate_batch(2, ops.ctypes.data_as(T.i32p), arr, arr, arr, None, sk.cloud)
+    assert rc == -1 and b"bad gate code" in T.lib().tfhe_amd_last_error()
+    H.close()
+    sk.close()
+
+
+def test_modswitch_and_dtot32_match_oracle():
+    L = T.lib()
+    rng = np.random.default_rng(2)
+    for v in rng.integers(-2 ** 31, 2 ** 31, size=300):
+        for m in (8, 2048, 4096, 7):
+            assert L.modSwitchFromTorus32(int(v), m) == O.lib().orc_modswitch_from_torus32(int(v), m)
+    for d in (0.0, 0.5, -0.5, 0.25, -0.25, 0.123456, -3.75):
+        assert L.dtot32(d) == O.lib().orc_dtot32(d)
+
+
+def test_c_program_host_part(tmp_path):
+    exe = str(tmp_path / "c_binding")
+    subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", "-o", exe, os.path.join(ROOT, "tests", "c", "c_binding.c"),
+                    "-I", os.path.join(ROOT, "include"), "-L", os.path.join(ROOT, "tfhe_amd"), "-ltfhe_amd",
+                    "-Wl,-rpath," + os.path.join(ROOT, "tfhe_amd")], check=True)
+    r = subprocess.run([exe, "host"], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
