@@ -43,6 +43,19 @@ def parse():
     return ap.parse_args()
 
 
+def profiled_traffic():
+    """HBM bytes per batch launch of the blind-rotation kernel, from the
+    calibrated rocprofv3 FETCH_SIZE/WRITE_SIZE passes summarised in profiles/."""
+    path = os.path.join(ROOT, "profiles", "br4_kernel_p128_batch4096.md")
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return int(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(count, seed):
     """Reference spqlios-FMA CPU path (oracle/_ref, built from the reference's
     own sources) timed on this host; falls back to the oracle port."""
@@ -197,8 +210,8 @@ def main():
         "roofline": {"bound": "mfma", "achieved": round(achieved_tflops, 3) if achieved_tflops else None,
                      "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(achieved_tflops / FP64_PEAK_TFLOPS, 4) if achieved_tflops else None,
-                     "traffic": None,
-                     "kernel": "br_kernel (blind rotation, FP64)",
+                     "traffic": profiled_traffic() if B == 4096 else None,
+                     "kernel": "br4_kernel (blind rotation, FP64); traffic from profiles/br4_kernel_p128_batch4096.md",
                      "note": "FP64 peak: MI355X vector and matrix FP64 are both 78.6 TFLOP/s"},
     }
     try:
