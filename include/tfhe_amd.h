@@ -241,6 +241,33 @@ TFHE_AMD_EXPORT int32_t tfhe_amd_gates_device_woks(const TFheGateBootstrappingCl
 TFHE_AMD_EXPORT int32_t tfhe_amd_keyswitch_device(const TFheGateBootstrappingCloudKeySet* ck, int32_t count,
                                                   const int32_t* d_in, int32_t* d_out, void* stream);
 
+/* ---- netlists (level scheduler; no reference counterpart) -------------
+ * A netlist is a set of bootstrapped gates (codes TFHE_AMD_NAND..MUX) over
+ * wires 0..n_wires-1; wire w is slot w of a device ciphertext pool. Gate g
+ * reads in0[g], in1[g] (and in2[g] for MUX; in2 may be NULL otherwise) and
+ * writes out[g]; each wire is written by at most one gate, wires written by no
+ * gate are inputs. Gates are levelised (schedule 0: as soon as possible,
+ * 1: as late as possible, except that gates whose outputs nothing reads keep
+ * their earliest level); each level is one batch launch, and with
+ * use_graph the whole sequence is captured once into a hipGraph and replayed
+ * (the graph is re-captured when the pool or stream changes). */
+typedef struct tfhe_amd_netlist tfhe_amd_netlist;
+TFHE_AMD_EXPORT tfhe_amd_netlist* tfhe_amd_netlist_create(const TFheGateBootstrappingCloudKeySet* ck, int32_t n_wires,
+                                                          int32_t n_gates, const int32_t* op, const int32_t* in0,
+                                                          const int32_t* in1, const int32_t* in2, const int32_t* out,
+                                                          int32_t schedule);
+/* Host-only levelisation (no GPU): returns the number of levels (negative on
+ * error: bad gate code, wire out of range, wire driven twice, cycle) and the
+ * 1-based level of every gate. */
+TFHE_AMD_EXPORT int32_t tfhe_amd_netlist_levelize(int32_t n_wires, int32_t n_gates, const int32_t* op,
+                                                  const int32_t* in0, const int32_t* in1, const int32_t* in2,
+                                                  const int32_t* out, int32_t schedule, int32_t* level_of_gate);
+/* Number of levels; widths (if not NULL) receives the gates per level. */
+TFHE_AMD_EXPORT int32_t tfhe_amd_netlist_levels(const tfhe_amd_netlist* nl, int32_t* widths);
+/* Asynchronous on stream (NULL = the library's stream). */
+TFHE_AMD_EXPORT int32_t tfhe_amd_netlist_run(tfhe_amd_netlist* nl, int32_t* d_pool, void* stream, int32_t use_graph);
+TFHE_AMD_EXPORT void tfhe_amd_netlist_destroy(tfhe_amd_netlist* nl);
+
 /* Launch configuration (autotuner hook). mode: 0 = automatic, 1 = one
  * wavefront per bootstrap, 2 = two wavefronts per bootstrap. */
 TFHE_AMD_EXPORT void tfhe_amd_set_launch_config(int32_t br_mode, int32_t br_blocks_per_cu, int32_t ks_gates_per_block);

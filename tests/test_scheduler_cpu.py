@@ -1,0 +1,62 @@
+"""Host-side levelisation of the level scheduler (no GPU)."""
+import numpy as np
+import pytest
+
+from tfhe_amd.circuits import Circuit, levelize, ripple_carry_adders
+
+
+def test_adder_levels_asap_and_alap():
+    n_adders, bits = 512, 32
+    C, sums, zero = ripple_carry_adders(n_adders, bits)
+    n, lv = levelize(C, 0)
+    assert n == 2 * bits + 1
+    widths = np.bincount(lv, minlength=n + 1)[1:]
+    # ASAP: every a^b and a&b first, then per bit (s, t3) and the carry
+    assert widths[0] == 2 * n_adders * bits
+    assert all(w == 2 * n_adders for w in widths[1::2]) and all(w == n_adders for w in widths[2::2])
+    n2, lv2 = levelize(C, 1)
+    w2 = np.bincount(lv2, minlength=n2 + 1)[1:]
+    assert n2 == n and w2.sum() == 5 * n_adders * bits
+    assert w2.max() <= 3 * n_adders  # t1, t2 of a bit move next to the carry that needs them
+
+
+def test_levels_respect_dependencies():
+    rng = np.random.default_rng(0)
+    C = Circuit(10)
+    wires = list(range(10))
+    for _ in range(200):
+        op = int(rng.integers(0, 11))
+        a, b, c = (int(v) for v in rng.choice(wires, 3))
+        wires.append(C.gate(op, a, b, c if op == 10 else None))
+    for sched in (0, 1):
+        n, lv = levelize(C, sched)
+        prod = {w: g for g, w in enumerate(C.out)}
+        for g in range(len(C.op)):
+            ins = [C.in0[g], C.in1[g]] + ([C.in2[g]] if C.op[g] == 10 else [])
+            for w in ins:
+                if w in prod:
+                    assert lv[prod[w]] < lv[g]
+        assert lv.min() >= 1 and lv.max() == n
+
+
+def test_levelize_errors():
+    C = Circuit(2)
+    C.gate(0, 0, 1)
+    C.in1[0] = 2  # reads its own output
+    with pytest.raises(RuntimeError, match="cycle"):
+        levelize(C)
+    C = Circuit(2)
+    C.gate(0, 0, 5)
+    with pytest.raises(RuntimeError, match="out of range"):
+        levelize(C)
+    C = Circuit(2)
+    C.gate(0, 0, 1)
+    C.gate(1, 0, 1)
+    C.out[1] = C.out[0]
+    with pytest.raises(RuntimeError, match="two gates"):
+        levelize(C)
+    C = Circuit(2)
+    C.gate(0, 0, 1)
+    C.op[0] = 17
+    with pytest.raises(RuntimeError, match="bad gate code"):
+        levelize(C)

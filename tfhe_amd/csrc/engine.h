@@ -5,6 +5,8 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include <mutex>
+
 #include "../../include/tfhe_amd.h"
 
 namespace tfhe_amd {
@@ -38,5 +40,16 @@ int engine_blind_rotate_host(tfhe_amd_device_key* dk, int32_t row0, int32_t n, c
 int engine_keyswitch_host(tfhe_amd_device_key* dk, const int32_t* in, int32_t* out);
 
 void engine_forget_host_key(const LweBootstrappingKeyFFT* bkfft);
+
+// For the level scheduler: device scratch a batch of `count` gates needs, the
+// engine lock, its stream, and a batch launch with caller-provided scratch
+// (call with engine_mutex() held; `capturing` disables event timing).
+size_t engine_batch_scratch_bytes(const tfhe_amd_device_key* dk, int count);
+std::mutex& engine_mutex();
+void* engine_stream();
+int engine_launch_batch_locked(tfhe_amd_device_key* dk, int count, const int32_t* d_op, const int32_t* d_in0,
+                               const int32_t* d_in1, const int32_t* d_in2, const int32_t* d_out, int32_t* d_pool,
+                               char* scr, void* stream, bool capturing);
+int engine_param_n(const tfhe_amd_device_key* dk);
 
 }  // namespace tfhe_amd

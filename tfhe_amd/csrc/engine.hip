@@ -779,7 +779,7 @@ size_t ext_bytes(const HostParams& P, int count) {
 // samples instead of running the keyswitch.
 int gates_device_locked(Ctx& C, tfhe_amd_device_key* dk, int count, const int32_t* d_op, const int32_t* d_in0,
                         const int32_t* d_in1, const int32_t* d_in2, const int32_t* d_out, int32_t* d_pool,
-                        int32_t* d_ext_out, int32_t mu, char* scr, hipStream_t s) {
+                        int32_t* d_ext_out, int32_t mu, char* scr, hipStream_t s, bool allow_prof = true) {
     if (count <= 0) return count == 0 ? 0 : set_error("negative gate count");
     const HostParams& P = dk->P;
     const size_t eb = ext_bytes(P, count);
@@ -801,7 +801,8 @@ int gates_device_locked(Ctx& C, tfhe_amd_device_key* dk, int count, const int32_
     A.mu = mu;
     A.ext = ext;
     Ctx::Ev ev{};
-    if (C.prof) {
+    const bool prof = C.prof && allow_prof;
+    if (prof) {
         for (auto& e : ev.e) HIP_TRY(hipEventCreate(&e));
         HIP_TRY(hipEventRecord(ev.e[0], s));
     }
@@ -814,7 +815,7 @@ int gates_device_locked(Ctx& C, tfhe_amd_device_key* dk, int count, const int32_
         launch_br(dk, A, 2 * count, s);
     }
     HIP_TRY(hipGetLastError());
-    if (C.prof) HIP_TRY(hipEventRecord(ev.e[1], s));
+    if (prof) HIP_TRY(hipEventRecord(ev.e[1], s));
     int rc = 0;
     if (d_ext_out) {
         hipLaunchKernelGGL(sum_halves_kernel, dim3(count), dim3(256), 0, s, ext, second, P.k * P.N, mu, d_ext_out);
@@ -822,7 +823,7 @@ int gates_device_locked(Ctx& C, tfhe_amd_device_key* dk, int count, const int32_
     } else {
         rc = run_keyswitch(dk, ext, second, count, d_pool, d_out, scr + eb, s);
     }
-    if (C.prof) {
+    if (prof) {
         HIP_TRY(hipEventRecord(ev.e[2], s));
         C.ev.push_back(ev);
     }
@@ -830,6 +831,28 @@ int gates_device_locked(Ctx& C, tfhe_amd_device_key* dk, int count, const int32_
 }
 
 }  // namespace
+
+// ------------------------------------------- internal entry points (scheduler) --
+
+size_t engine_batch_scratch_bytes(const tfhe_amd_device_key* dk, int count) {
+    return ext_bytes(dk->P, count) + ks_scratch_bytes(dk->P, count);
+}
+
+std::mutex& engine_mutex() { return ctx().mu; }
+
+void* engine_stream() {
+    Ctx& C = ctx();
+    return ensure_ready(C) ? nullptr : (void*)C.stream;
+}
+
+int engine_launch_batch_locked(tfhe_amd_device_key* dk, int count, const int32_t* d_op, const int32_t* d_in0,
+                               const int32_t* d_in1, const int32_t* d_in2, const int32_t* d_out, int32_t* d_pool,
+                               char* scr, void* stream, bool capturing) {
+    return gates_device_locked(ctx(), dk, count, d_op, d_in0, d_in1, d_in2, d_out, d_pool, nullptr, 1 << 29, scr,
+                               (hipStream_t)stream, !capturing);
+}
+
+int engine_param_n(const tfhe_amd_device_key* dk) { return dk->P.n; }
 
 // ----------------------------------------------------------- key upload --
 

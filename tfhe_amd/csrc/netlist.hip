@@ -1,0 +1,234 @@
+// Level scheduler for gate netlists (the callers of the gate API that the
+// reference's circuits are: test/test-addition-boot.cpp:30-111,
+// test/test-long-run.cpp:61-185). A netlist is a set of bootstrapped gates
+// over wires; wires are slots of a device ciphertext pool. Gates are
+// levelised (as soon as possible, or as late as possible), each level is one
+// batch launch (blind rotation + keyswitch), and the whole sequence of
+// launches is captured once into a hipGraph and replayed.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "engine.h"
+
+struct tfhe_amd_netlist {
+    const LweBootstrappingKeyFFT* bf = nullptr;
+    int32_t n_wires = 0, n_gates = 0;
+    std::vector<int32_t> level_count;    // gates per level
+    std::vector<size_t> level_offset;    // offset (in gates) of each level in the sorted order
+    int32_t* d_idx = nullptr;            // [5][n_gates]: op, in0, in1, in2, out (sorted by level)
+    char* d_scr = nullptr;               // scratch for the widest level
+    size_t scr_bytes = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int32_t* graph_pool = nullptr;
+    void* graph_stream = nullptr;
+};
+
+namespace tfhe_amd {
+namespace {
+
+int build_levels(tfhe_amd_netlist* nl, const int32_t* op, const int32_t* in0, const int32_t* in1,
+                 const int32_t* in2, const int32_t* out, int schedule, std::vector<int32_t>& order,
+                 std::vector<int32_t>& level) {
+    const int G = nl->n_gates, W = nl->n_wires;
+    std::vector<int32_t> producer(W, -1);
+    for (int g = 0; g < G; g++) {
+        if (op[g] < TFHE_AMD_NAND || op[g] > TFHE_AMD_MUX) return set_error("netlist gate %d: bad gate code %d", g, op[g]);
+        const int nin = op[g] == TFHE_AMD_MUX ? 3 : 2;
+        const int32_t ins[3] = {in0[g], in1[g], in2 ? in2[g] : 0};
+        for (int j = 0; j < nin; j++)
+            if (ins[j] < 0 || ins[j] >= W) return set_error("netlist gate %d: input wire %d out of range", g, ins[j]);
+        if (out[g] < 0 || out[g] >= W) return set_error("netlist gate %d: output wire out of range", g);
+        if (producer[out[g]] >= 0) return set_error("netlist: wire %d driven by two gates", out[g]);
+        producer[out[g]] = g;
+    }
+    // longest path from the inputs (as soon as possible), Kahn's algorithm
+    std::vector<std::vector<int32_t>> users(G);
+    std::vector<int32_t> pending(G, 0);
+    for (int g = 0; g < G; g++) {
+        const int nin = op[g] == TFHE_AMD_MUX ? 3 : 2;
+        const int32_t ins[3] = {in0[g], in1[g], in2 ? in2[g] : 0};
+        for (int j = 0; j < nin; j++) {
+            const int p = producer[ins[j]];
+            if (p >= 0) {
+                users[p].push_back(g);
+                pending[g]++;
+            }
+        }
+    }
+    level.assign(G, 1);
+    std::vector<int32_t> queue;
+    for (int g = 0; g < G; g++)
+        if (!pending[g]) queue.push_back(g);
+    size_t head = 0;
+    while (head < queue.size()) {
+        const int g = queue[head++];
+        for (int u : users[g]) {
+            level[u] = std::max(level[u], level[g] + 1);
+            if (--pending[u] == 0) queue.push_back(u);
+        }
+    }
+    if ((int)queue.size() != G) return set_error("netlist has a cycle");
+    int depth = 0;
+    for (int g = 0; g < G; g++) depth = std::max(depth, level[g]);
+    if (schedule == 1) {
+        // as late as possible: each gate just before its earliest user; gates
+        // nobody reads (circuit outputs) keep their earliest level
+        for (int q = G - 1; q >= 0; q--) {
+            const int g = queue[q];
+            if (users[g].empty()) continue;
+            int lv = depth;
+            for (int u : users[g]) lv = std::min(lv, level[u] - 1);
+            level[g] = lv;
+        }
+    }
+    order.resize(G);
+    for (int g = 0; g < G; g++) order[g] = g;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return level[a] < level[b]; });
+    nl->level_count.assign(depth, 0);
+    for (int g = 0; g < G; g++) nl->level_count[level[g] - 1]++;
+    nl->level_offset.assign(depth, 0);
+    for (int l = 1; l < depth; l++) nl->level_offset[l] = nl->level_offset[l - 1] + nl->level_count[l - 1];
+    return 0;
+}
+
+int launch_all(tfhe_amd_netlist* nl, tfhe_amd_device_key* dk, int32_t* d_pool, void* stream, bool capturing) {
+    const int G = nl->n_gates;
+    for (size_t l = 0; l < nl->level_count.size(); l++) {
+        const int cnt = nl->level_count[l];
+        if (!cnt) continue;
+        const int32_t* base = nl->d_idx + nl->level_offset[l];
+        if (int rc = engine_launch_batch_locked(dk, cnt, base, base + G, base + 2 * G, base + 3 * G, base + 4 * G,
+                                                d_pool, nl->d_scr, stream, capturing))
+            return rc;
+    }
+    return 0;
+}
+
+void drop_graph(tfhe_amd_netlist* nl) {
+    if (nl->exec) (void)hipGraphExecDestroy(nl->exec);
+    if (nl->graph) (void)hipGraphDestroy(nl->graph);
+    nl->exec = nullptr;
+    nl->graph = nullptr;
+    nl->graph_pool = nullptr;
+}
+
+}  // namespace
+}  // namespace tfhe_amd
+
+using namespace tfhe_amd;
+
+TFHE_AMD_EXPORT tfhe_amd_netlist* tfhe_amd_netlist_create(const TFheGateBootstrappingCloudKeySet* ck, int32_t n_wires,
+                                                          int32_t n_gates, const int32_t* op, const int32_t* in0,
+                                                          const int32_t* in1, const int32_t* in2, const int32_t* out,
+                                                          int32_t schedule) {
+    if (!ck || n_wires <= 0 || n_gates <= 0 || !op || !in0 || !in1 || !out) {
+        set_error("netlist_create: bad arguments");
+        return nullptr;
+    }
+    auto* nl = new tfhe_amd_netlist();
+    nl->bf = ck->bkFFT;
+    nl->n_wires = n_wires;
+    nl->n_gates = n_gates;
+    std::vector<int32_t> order, level;
+    if (build_levels(nl, op, in0, in1, in2, out, schedule, order, level)) {
+        delete nl;
+        return nullptr;
+    }
+    tfhe_amd_device_key* dk = engine_key(ck->bkFFT);
+    if (!dk) {
+        delete nl;
+        return nullptr;
+    }
+    std::vector<int32_t> idx(5 * (size_t)n_gates);
+    for (int k = 0; k < n_gates; k++) {
+        const int g = order[k];
+        idx[k] = op[g];
+        idx[n_gates + k] = in0[g];
+        idx[2 * (size_t)n_gates + k] = in1[g];
+        idx[3 * (size_t)n_gates + k] = (op[g] == TFHE_AMD_MUX && in2) ? in2[g] : in0[g];
+        idx[4 * (size_t)n_gates + k] = out[g];
+    }
+    int widest = 0;
+    for (int c : nl->level_count) widest = std::max(widest, c);
+    nl->scr_bytes = engine_batch_scratch_bytes(dk, widest);
+    if (hipMalloc(&nl->d_idx, idx.size() * sizeof(int32_t)) != hipSuccess ||
+        hipMalloc(&nl->d_scr, nl->scr_bytes) != hipSuccess ||
+        hipMemcpy(nl->d_idx, idx.data(), idx.size() * sizeof(int32_t), hipMemcpyHostToDevice) != hipSuccess) {
+        set_error("netlist_create: device allocation failed");
+        (void)hipFree(nl->d_idx);
+        (void)hipFree(nl->d_scr);
+        delete nl;
+        return nullptr;
+    }
+    return nl;
+}
+
+TFHE_AMD_EXPORT int32_t tfhe_amd_netlist_levelize(int32_t n_wires, int32_t n_gates, const int32_t* op,
+                                                  const int32_t* in0, const int32_t* in1, const int32_t* in2,
+                                                  const int32_t* out, int32_t schedule, int32_t* level_of_gate) {
+    if (n_wires <= 0 || n_gates <= 0 || !op || !in0 || !in1 || !out) return set_error("levelize: bad arguments");
+    tfhe_amd_netlist tmp;
+    tmp.n_wires = n_wires;
+    tmp.n_gates = n_gates;
+    std::vector<int32_t> order, level;
+    if (int rc = build_levels(&tmp, op, in0, in1, in2, out, schedule, order, level)) return rc;
+    if (level_of_gate) memcpy(level_of_gate, level.data(), sizeof(int32_t) * n_gates);
+    return (int32_t)tmp.level_count.size();
+}
+
+TFHE_AMD_EXPORT int32_t tfhe_amd_netlist_levels(const tfhe_amd_netlist* nl, int32_t* widths) {
+    if (!nl) return set_error("null netlist");
+    if (widths)
+        for (size_t l = 0; l < nl->level_count.size(); l++) widths[l] = nl->level_count[l];
+    return (int32_t)nl->level_count.size();
+}
+
+TFHE_AMD_EXPORT int32_t tfhe_amd_netlist_run(tfhe_amd_netlist* nl, int32_t* d_pool, void* stream, int32_t use_graph) {
+    if (!nl || !d_pool) return set_error("netlist_run: bad arguments");
+    tfhe_amd_device_key* dk = engine_key(nl->bf);
+    if (!dk) return -1;
+    std::lock_guard<std::mutex> g(engine_mutex());
+    void* s = stream ? stream : engine_stream();
+    if (!s) return -1;
+    if (!use_graph) return launch_all(nl, dk, d_pool, s, false);
+    if (!nl->exec || nl->graph_pool != d_pool || nl->graph_stream != s) {
+        drop_graph(nl);
+        hipStream_t hs = (hipStream_t)s;
+        if (hipStreamBeginCapture(hs, hipStreamCaptureModeThreadLocal) != hipSuccess)
+            return set_error("netlist_run: stream capture failed to start");
+        const int rc = launch_all(nl, dk, d_pool, s, true);
+        hipGraph_t graph = nullptr;
+        const hipError_t e = hipStreamEndCapture(hs, &graph);
+        if (rc) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return rc;
+        }
+        if (e != hipSuccess) return set_error("netlist_run: stream capture failed: %s", hipGetErrorString(e));
+        nl->graph = graph;
+        if (hipGraphInstantiate(&nl->exec, graph, nullptr, nullptr, 0) != hipSuccess) {
+            drop_graph(nl);
+            return set_error("netlist_run: graph instantiation failed");
+        }
+        nl->graph_pool = d_pool;
+        nl->graph_stream = s;
+    }
+    if (hipGraphLaunch(nl->exec, (hipStream_t)s) != hipSuccess) return set_error("netlist_run: graph launch failed");
+    return 0;
+}
+
+TFHE_AMD_EXPORT void tfhe_amd_netlist_destroy(tfhe_amd_netlist* nl) {
+    if (!nl) return;
+    {
+        std::lock_guard<std::mutex> g(engine_mutex());
+        (void)hipDeviceSynchronize();
+        drop_graph(nl);
+    }
+    (void)hipFree(nl->d_idx);
+    (void)hipFree(nl->d_scr);
+    delete nl;
+}
