@@ -38,6 +38,8 @@ def parse():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--adders", type=int, default=512, help="ripple-carry adders in the netlist section (0 = skip)")
     ap.add_argument("--adder-bits", type=int, default=32)
+    ap.add_argument("--scaling", default="64,256,1024,2048,8192",
+                    help="batch sizes for the per-kernel batch-size sweep ('' = skip)")
     ap.add_argument("--compare-modes", action="store_true",
                     help="also time the one-wavefront-per-bootstrap kernel on the same batch")
     ap.add_argument("--copy-calib-mb", type=int, default=0,
@@ -56,6 +58,50 @@ def profiled_traffic():
     except OSError:
         pass
     return None
+
+
+def bench_scaling(T, sk, P, sizes, seed):
+    """gates/s of each blind-rotation kernel across batch sizes (input for the
+    automatic kernel choice): 2 = four bootstraps per workgroup, 3 = two
+    wavefronts per bootstrap, 0 = automatic."""
+    rng = np.random.default_rng(seed + 3)
+    maxb = max(sizes)
+    ops = rng.integers(0, 6, size=maxb).astype(np.int32)
+    bits = rng.integers(0, 2, size=(2, maxb)).astype(np.int32)
+    w = P.n + 1
+    pool = np.concatenate([sk.encrypt(bits[0]), sk.encrypt(bits[1]), np.zeros((maxb, w), np.int32)])
+    d_pool = T.DeviceArray.from_numpy(pool)
+    idx = np.arange(maxb, dtype=np.int32)
+    d_ops, d_i0 = T.DeviceArray.from_numpy(ops), T.DeviceArray.from_numpy(idx)
+    d_i1, d_o = T.DeviceArray.from_numpy(idx + maxb), T.DeviceArray.from_numpy(idx + 2 * maxb)
+    out = {}
+    for B in sizes:
+        row = {}
+        for mode in (2, 3, 0):
+            T.lib().tfhe_amd_set_launch_config(mode, 0, 0)
+            T.check(T.lib().tfhe_amd_gates_device(sk.cloud, B, d_ops.ptr, d_i0.ptr, d_i1.ptr, d_i0.ptr, d_o.ptr,
+                                                  d_pool.ptr, None), "gates")
+            T.synchronize()
+            reps = max(1, min(8, 8192 // B))
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                T.check(T.lib().tfhe_amd_gates_device(sk.cloud, B, d_ops.ptr, d_i0.ptr, d_i1.ptr, d_i0.ptr, d_o.ptr,
+                                                      d_pool.ptr, None), "gates")
+            T.synchronize()
+            dt = (time.perf_counter() - t0) / reps
+            row[{2: "four_gate_lockstep", 3: "two_wave_split", 0: "auto"}[mode]] = {
+                "gates_per_s": round(B / dt, 1), "ms": round(dt * 1e3, 3)}
+        out[str(B)] = row
+        print(f"[bench] batch {B}: {row}", file=sys.stderr, flush=True)
+    T.lib().tfhe_amd_set_launch_config(0, 0, 0)
+    dec = sk.decrypt(d_pool.numpy()[2 * maxb:2 * maxb + max(sizes)])
+    truth = {0: lambda x, y: 1 - (x & y), 1: lambda x, y: x & y, 2: lambda x, y: x | y, 3: lambda x, y: x ^ y,
+             4: lambda x, y: 1 - (x ^ y), 5: lambda x, y: 1 - (x | y)}
+    exp = np.array([truth[o](x, y) for o, x, y in zip(ops, bits[0], bits[1])])
+    out["decrypt_errors"] = int((dec != exp).sum())
+    for d in (d_pool, d_ops, d_i0, d_i1, d_o):
+        d.free()
+    return out
 
 
 def bench_adders(T, sk, P, n_adders, bits, reps, seed):
@@ -224,6 +270,8 @@ def main():
     print(f"[bench] per-gate {per_gate:.1f} gates/s, p50 {np.percentile(lat_ms, 50):.3f} ms", file=sys.stderr,
           flush=True)
 
+    sizes = [int(x) for x in args.scaling.split(",") if x.strip()]
+    scaling = bench_scaling(T, sk, P, sizes, args.seed) if sizes else None
     adders = bench_adders(T, sk, P, args.adders, args.adder_bits, 2, args.seed) if args.adders > 0 else None
 
     br_avg_s = br_ms / max(launches, 1) / 1e3
@@ -251,6 +299,7 @@ def main():
         "batch_latency_ms": {"p50": round(ms_per_step, 4), "p99": round(ms_per_step, 4)},
         "decrypt_errors": errors,
         "kernel_variants_gates_per_s": mode_cmp,
+        "batch_size_sweep": scaling,
         "netlist_adders": adders,
         "kernel_ms": {"blind_rotate_avg": round(br_ms / max(launches, 1), 4),
                       "keyswitch_avg": round(ks_ms / max(launches, 1), 4)},

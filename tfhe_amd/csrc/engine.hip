@@ -473,6 +473,216 @@ __global__ __launch_bounds__(256, 2) void br4_kernel(BrArgs A, const int32_t* __
     if (lane == 0) e[N] = (int32_t)acc[1][0];
 }
 
+// largest LWE dimension the two-wavefront kernel's LDS table holds
+constexpr int BR2_NMAX = 768;
+
+// Blind rotation with two wavefronts per bootstrap (M = 512): wavefront c of a
+// bootstrap owns TLWE component c of the accumulator (in registers). Per key
+// row it rotates and decomposes its own component, transforms its l digit
+// polynomials and accumulates partial products for both output components;
+// the two wavefronts then swap the partial sum meant for the other component
+// through LDS and each inverse-transforms its own component. Half the latency
+// of one wavefront per bootstrap; a workgroup carries two bootstraps (four
+// wavefronts) that share the key digit blocks in LDS.
+template <int L>
+__global__ __launch_bounds__(256, 2) void br2_kernel(BrArgs A, const int32_t* __restrict__ mux_gate,
+                                                    const int32_t* __restrict__ n_jobs) {
+    constexpr int M = 512;
+    using F = Fft<M>;
+    constexpr int R = F::R, N = 2 * M, KPL = 2 * L;
+    constexpr int LOG2N2 = 11;
+    constexpr int NMAX = BR2_NMAX;
+    __shared__ c64 s_fft[4][F::LDS_C64];
+    __shared__ c64 s_bk[2][2 * M];  // key digit blocks of rows (c*L + p), c = 0, 1
+    __shared__ c64 s_tw1[R * 64];
+    __shared__ int16_t s_bara[2][NMAX];
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int w = tid >> 6;
+    const int q = w >> 1;  // bootstrap slot in the workgroup
+    const int c = w & 1;   // accumulator component owned by this wavefront
+    const int njobs = *n_jobs;
+    const int job = blockIdx.x * 2 + q;
+    if (blockIdx.x * 2 >= njobs) return;  // whole block idle (uniform)
+    const bool active = job < njobs;
+    const int n = A.n;
+    int16_t* bara = s_bara[q];
+    c64* fb = s_fft[w];
+    int32_t* fbi = (int32_t*)fb;
+
+    // acc[h*R + s] = ACC_c[64 s + lane + h M]
+    uint32_t acc[2 * R];
+#pragma unroll
+    for (int k = 0; k < 2 * R; k++) acc[k] = 0;
+    if (c == 1) {
+        if (active) {
+            const int gate = job < A.count ? job : mux_gate[job - A.count];
+            const int half = job < A.count ? 0 : 1;
+            const int op = A.op[gate];
+            int32_t k0, ca, cb, cc;
+            gate_coeffs(op, half, k0, ca, cb, cc);
+            const int32_t* a = A.pool + (size_t)A.in0[gate] * (n + 1);
+            const int32_t* b = (op == OP_IDENTITY) ? a : A.pool + (size_t)A.in1[gate] * (n + 1);
+            const int32_t* cp = (op == TFHE_AMD_MUX) ? A.pool + (size_t)A.in2[gate] * (n + 1) : a;
+            uint32_t barb = 0;
+            for (int i = lane; i <= n; i += 64) {
+                uint32_t v =
+                    (uint32_t)ca * (uint32_t)a[i] + (uint32_t)cb * (uint32_t)b[i] + (uint32_t)cc * (uint32_t)cp[i];
+                if (i == n) v += (uint32_t)k0;
+                // modSwitchFromTorus32(v, 2N), libtfhe/numeric-functions.cpp:59-65
+                const uint32_t bar = (v + (1u << (31 - LOG2N2))) >> (32 - LOG2N2);
+                if (i < n) bara[i] = (int16_t)bar;
+                else barb = bar;
+            }
+            barb = __shfl(barb, n & 63);
+            // ACC_1 = X^{2N-barb} * [mu...mu], lwe-bootstrapping-functions-fft.cpp:146-148
+#pragma unroll
+            for (int h = 0; h < 2; h++)
+#pragma unroll
+                for (int s = 0; s < R; s++) {
+                    const int j = 64 * s + lane + h * M;
+                    acc[h * R + s] = (((j + barb) & (2 * N - 1)) < N) ? (uint32_t)A.mu : 0u - (uint32_t)A.mu;
+                }
+        } else {
+            for (int i = lane; i < n; i += 64) bara[i] = 0;
+        }
+    }
+    for (int k = tid; k < R * 64; k += 256) s_tw1[k] = A.tw[(k & 63) * (R + 8) + (k >> 6)];
+    const c64* tw1 = s_tw1 + lane;
+    const c64 tw2 = A.tw[lane * (R + 8) + R + 1];
+    const uint32_t offset = A.offset;
+    const int Bgbit = A.Bgbit;
+    const uint32_t dmask = (1u << Bgbit) - 1u;
+    const int32_t dhalf = 1 << (Bgbit - 1);
+    __syncthreads();
+
+    for (int i = 0; i < n; i++) {
+        const int abar = bara[i];
+        c64 p0[R], p1[R];
+#pragma unroll
+        for (int s = 0; s < R; s++) p0[s] = p1[s] = c64{0.0, 0.0};
+        uint32_t tv[2 * R];
+#pragma unroll 1
+        for (int p = 0; p < L; p++) {
+            __syncthreads();  // previous MAC is done with s_bk
+            {
+                // key digit blocks (i, p) and (i, L + p) -> LDS, 8 x 16 B per thread
+                const c64* q0 = A.bkf + ((size_t)i * KPL + p) * 2 * M + tid;
+                const c64* q1 = A.bkf + ((size_t)i * KPL + L + p) * 2 * M + tid;
+                c64* d0 = s_bk[0] + (tid & ~63);
+                c64* d1 = s_bk[1] + (tid & ~63);
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    __builtin_amdgcn_global_load_lds((const void*)(q0 + u * 256),
+                                                     (__attribute__((address_space(3))) void*)(d0 + u * 256), 16, 0, 0);
+                    __builtin_amdgcn_global_load_lds((const void*)(q1 + u * 256),
+                                                     (__attribute__((address_space(3))) void*)(d1 + u * 256), 16, 0, 0);
+                }
+            }
+            c64 x[R];
+            if (active) {
+                if (p == 0) {
+                    // T = (X^abar - 1) * ACC_c + gadget offset, through this wavefront's LDS buffer
+#pragma unroll
+                    for (int h = 0; h < 2; h++)
+#pragma unroll
+                        for (int s = 0; s < R; s++) fbi[64 * s + lane + h * M] = (int32_t)acc[h * R + s];
+                    wave_sync();
+#pragma unroll
+                    for (int h = 0; h < 2; h++)
+#pragma unroll
+                        for (int s = 0; s < R; s++) {
+                            const int j = 64 * s + lane + h * M;
+                            const int idx = (j - abar) & (2 * N - 1);
+                            const uint32_t v = (uint32_t)fbi[idx & (N - 1)];
+                            const uint32_t rot = (idx & N) ? 0u - v : v;
+                            tv[h * R + s] = rot - acc[h * R + s] + offset;
+                        }
+                    wave_sync();
+                }
+                // digit p of the truncating gadget decomposition, tgsw-functions.cpp:337-348
+                const int shift = 32 - (p + 1) * Bgbit;
+#pragma unroll
+                for (int s = 0; s < R; s++) {
+                    const int32_t d0v = (int32_t)((tv[s] >> shift) & dmask) - dhalf;
+                    const int32_t d1v = (int32_t)((tv[R + s] >> shift) & dmask) - dhalf;
+                    x[s] = c64{(double)d0v, (double)d1v};
+                }
+                F::template fwd1<64>(x, lane, fb, tw1);
+                wave_sync();
+                F::fwd2(x, lane, fb, tw2);
+                wave_sync();
+                F::fwd3(x, lane, fb);
+                wave_sync();
+                F::fwd4(x, lane, fb);
+            }
+            __syncthreads();  // key blocks landed
+            if (active) {
+                const c64* bk = s_bk[c];
+#pragma unroll
+                for (int s = 0; s < R; s++) {
+                    cmac(p0[s], x[s], bk[s * 64 + lane]);
+                    cmac(p1[s], x[s], bk[M + s * 64 + lane]);
+                }
+            }
+        }
+        // swap partial sums: wavefront c keeps output c, hands over output 1-c
+        if (active) {
+            if (c) {
+#pragma unroll
+                for (int s = 0; s < R; s++) fb[s * 64 + lane] = p0[s];
+            } else {
+#pragma unroll
+                for (int s = 0; s < R; s++) fb[s * 64 + lane] = p1[s];
+            }
+        }
+        __syncthreads();
+        c64 x[R];
+        if (active) {
+            const c64* other = s_fft[w ^ 1];
+            if (c) {
+#pragma unroll
+                for (int s = 0; s < R; s++) x[s] = cadd(p1[s], other[s * 64 + lane]);
+            } else {
+#pragma unroll
+                for (int s = 0; s < R; s++) x[s] = cadd(p0[s], other[s * 64 + lane]);
+            }
+        }
+        __syncthreads();  // the partner has read this wavefront's buffer
+        if (active) {
+            // inverse transform of output component c and ACC_c += result
+            F::inv1(x, lane, fb);
+            wave_sync();
+            F::inv2(x, lane, fb, tw2);
+            wave_sync();
+            F::inv3(x, lane, fb);
+            wave_sync();
+            F::template inv4<64>(x, lane, fb, tw1);
+#pragma unroll
+            for (int s = 0; s < R; s++) {
+                acc[s] += (uint32_t)round_mod32(x[s].re);
+                acc[R + s] += (uint32_t)round_mod32(x[s].im);
+            }
+        }
+    }
+    if (!active) return;
+    // tLweExtractLweSampleIndex(.., 0, ..), libtfhe/lwe.cpp:41-56
+    int32_t* e = A.ext + (size_t)job * (N + 1);
+    if (c == 0) {
+#pragma unroll
+        for (int h = 0; h < 2; h++)
+#pragma unroll
+            for (int s = 0; s < R; s++) {
+                const int j = 64 * s + lane + h * M;
+                const uint32_t v = acc[h * R + s];
+                e[j == 0 ? 0 : N - j] = (int32_t)(j == 0 ? v : 0u - v);
+            }
+    } else if (lane == 0) {
+        e[N] = (int32_t)acc[0];
+    }
+}
+
 template <int M>
 __global__ __launch_bounds__(64) void bk_convert_kernel(const int32_t* __restrict__ bk, c64* __restrict__ bkf,
                                                         const c64* __restrict__ tw) {
@@ -620,6 +830,7 @@ struct Ctx {
     void* h_pinned = nullptr;
     size_t h_cap = 0;
     int br_mode = 0, br_bpc = 0;
+    int br_small_max = 1024;  // automatic mode: batches up to this many gates use two wavefronts per bootstrap
     // optional per-launch timing with HIP events on the launch stream
     bool prof = false;
     struct Ev { hipEvent_t e[3]; };
@@ -806,11 +1017,21 @@ int gates_device_locked(Ctx& C, tfhe_amd_device_key* dk, int count, const int32_
         for (auto& e : ev.e) HIP_TRY(hipEventCreate(&e));
         HIP_TRY(hipEventRecord(ev.e[0], s));
     }
-    const bool lockstep = dk->M == 512 && C.br_mode != 1;
-    if (lockstep) {
+    // kernel choice: 1 = one wavefront per bootstrap, 2 = four bootstraps per workgroup,
+    // 3 = two wavefronts per bootstrap; 0 = automatic (two wavefronts while the batch
+    // leaves CUs idle, four-gate lockstep for large batches)
+    int mode = C.br_mode;
+    if (dk->M != 512) mode = 1;
+    if (mode == 3 && dk->P.n > BR2_NMAX) mode = 2;
+    if (mode == 0) mode = (count <= C.br_small_max && dk->P.n <= BR2_NMAX) ? 3 : 2;
+    if (mode == 2) {
         const int blocks = (2 * count + 3) / 4;
         if (dk->P.l == 3) hipLaunchKernelGGL(br4_kernel<3>, dim3(blocks), dim3(256), 0, s, A, mux_gate, n_jobs);
         else hipLaunchKernelGGL(br4_kernel<2>, dim3(blocks), dim3(256), 0, s, A, mux_gate, n_jobs);
+    } else if (mode == 3) {
+        const int blocks = count;  // two bootstraps per workgroup, at most 2*count jobs
+        if (dk->P.l == 3) hipLaunchKernelGGL(br2_kernel<3>, dim3(blocks), dim3(256), 0, s, A, mux_gate, n_jobs);
+        else hipLaunchKernelGGL(br2_kernel<2>, dim3(blocks), dim3(256), 0, s, A, mux_gate, n_jobs);
     } else {
         launch_br(dk, A, 2 * count, s);
     }
@@ -1159,7 +1380,8 @@ TFHE_AMD_EXPORT void tfhe_amd_set_launch_config(int32_t br_mode, int32_t br_bloc
     std::lock_guard<std::mutex> g(C.mu);
     C.br_mode = br_mode;
     C.br_bpc = br_blocks_per_cu;
-    (void)ks_gates_per_block;
+    if (ks_gates_per_block > 0) C.br_small_max = ks_gates_per_block;
+    (void)0;
 }
 
 TFHE_AMD_EXPORT int32_t tfhe_amd_get_launch_config(int32_t* br_mode, int32_t* br_blocks_per_cu, int32_t* ks_gates_per_block) {
