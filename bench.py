@@ -40,6 +40,11 @@ def parse():
     ap.add_argument("--adder-bits", type=int, default=32)
     ap.add_argument("--scaling", default="64,256,1024,2048,8192",
                     help="batch sizes for the per-kernel batch-size sweep ('' = skip)")
+    ap.add_argument("--serve-clients", type=int, default=32, help="serving section: clients, one key each (0 = skip)")
+    ap.add_argument("--serve-rates", default="1e3,1e4,1e5", help="serving section: offered gates/s")
+    ap.add_argument("--serve-seconds", type=float, default=10.0, help="serving section: arrival window per rate")
+    ap.add_argument("--serve-min-requests", type=int, default=4)
+    ap.add_argument("--serve-max-requests", type=int, default=160)
     ap.add_argument("--compare-modes", action="store_true",
                     help="also time the one-wavefront-per-bootstrap kernel on the same batch")
     ap.add_argument("--copy-calib-mb", type=int, default=0,
@@ -78,7 +83,7 @@ def bench_scaling(T, sk, P, sizes, seed):
     for B in sizes:
         row = {}
         for mode in (2, 3, 0):
-            T.lib().tfhe_amd_set_launch_config(mode, 0, 0)
+            T.lib().tfhe_amd_set_launch_config(mode, -1)
             T.check(T.lib().tfhe_amd_gates_device(sk.cloud, B, d_ops.ptr, d_i0.ptr, d_i1.ptr, d_i0.ptr, d_o.ptr,
                                                   d_pool.ptr, None), "gates")
             T.synchronize()
@@ -93,7 +98,7 @@ def bench_scaling(T, sk, P, sizes, seed):
                 "gates_per_s": round(B / dt, 1), "ms": round(dt * 1e3, 3)}
         out[str(B)] = row
         print(f"[bench] batch {B}: {row}", file=sys.stderr, flush=True)
-    T.lib().tfhe_amd_set_launch_config(0, 0, 0)
+    T.lib().tfhe_amd_set_launch_config(0, -1)
     dec = sk.decrypt(d_pool.numpy()[2 * maxb:2 * maxb + max(sizes)])
     truth = {0: lambda x, y: 1 - (x & y), 1: lambda x, y: x & y, 2: lambda x, y: x | y, 3: lambda x, y: x ^ y,
              4: lambda x, y: 1 - (x ^ y), 5: lambda x, y: 1 - (x | y)}
@@ -101,6 +106,83 @@ def bench_scaling(T, sk, P, sizes, seed):
     out["decrypt_errors"] = int((dec != exp).sum())
     for d in (d_pool, d_ops, d_i0, d_i1, d_o):
         d.free()
+    return out
+
+
+def bench_serving(T, P, args):
+    """Poisson stream of random layered circuits (depth U[1,64], width U[1,512],
+    gate types uniform over all 11) from clients with their own keys resident in
+    HBM, served by the continuous-batching request loop. Per offered rate:
+    request latency p50/p99 (submit to outputs on the host) and gates/s."""
+    import threading
+    from tfhe_amd.circuits import Server, random_layered_circuit
+    n_cl = args.serve_clients
+    t0 = time.perf_counter()
+    keys = []
+    for i in range(n_cl):
+        sk = T.SecretKey(P, seed=1000 + i)
+        sk.upload()
+        keys.append(sk)
+        if (i + 1) % 8 == 0:
+            print(f"[bench] serving: {i + 1} client keys ({time.perf_counter() - t0:.1f} s)", file=sys.stderr,
+                  flush=True)
+    S = Server(n_streams=4, depth=2, max_batch=4096, wire_budget_bytes=4 << 30)
+    cids = [S.add_client(sk.cloud, P.n) for sk in keys]
+    mean_gates = 32.5 * 256.5
+    out = {"workload": f"{n_cl} clients (own P128 key each, resident), Poisson arrivals, random layered circuits "
+                       "depth U[1,64] x width U[1,512], 11 gate types uniform", "clients": n_cl,
+           "key_bytes_resident": S.stats()["key_bytes"], "rates": {}}
+    rng = np.random.default_rng(args.seed + 5)
+    for rate in [float(x) for x in args.serve_rates.split(",") if x.strip()]:
+        # arrival times (s) of a Poisson process of requests at rate / mean_gates
+        arr, t = [], 0.0
+        while len(arr) < args.serve_max_requests and (t < args.serve_seconds or len(arr) < args.serve_min_requests):
+            t += rng.exponential(mean_gates / rate)
+            arr.append(t)
+        reqs = []
+        for a in arr:
+            c = int(rng.integers(0, n_cl))
+            C, last = random_layered_circuit(rng, int(rng.integers(1, 65)), int(rng.integers(1, 513)))
+            bits = rng.integers(0, 2, size=len(last)).astype(np.int32)
+            reqs.append((a, c, C, bits, keys[c].encrypt(bits), last))
+        ids = [None] * len(reqs)
+        start = time.perf_counter()
+
+        def feed():
+            for j, (a, c, C, bits, cts, last) in enumerate(reqs):
+                d = start + a - time.perf_counter()
+                if d > 0:
+                    time.sleep(d)
+                ids[j] = S.submit(cids[c], C, np.arange(len(bits)), cts, last)
+
+        th = threading.Thread(target=feed)
+        th.start()
+        lat, outs = [], []
+        for j in range(len(reqs)):
+            while ids[j] is None:
+                time.sleep(0.0005)
+            o, l = S.wait(ids[j], -1)
+            outs.append(o)
+            lat.append(l)
+        end = time.perf_counter()
+        th.join()
+        gates = sum(len(r[2].op) for r in reqs)
+        errors = 0
+        for (a, c, C, bits, cts, last), o in zip(reqs, outs):
+            errors += int((keys[c].decrypt(o) != C.evaluate(bits)[last]).sum())
+        lat = np.array(lat)
+        st = S.stats()
+        row = {"requests": len(reqs), "gates": gates, "offered_gates_per_s": rate,
+               "gates_per_s": round(gates / (end - start), 1), "window_s": round(end - start, 2),
+               "p50_ms": round(float(np.percentile(lat, 50)), 2), "p99_ms": round(float(np.percentile(lat, 99)), 2),
+               "max_ms": round(float(lat.max()), 2), "output_bit_errors": errors,
+               "launches_total": st["launches"], "max_launch_gates": st["max_launch_gates"],
+               "wire_bytes_peak": st["wire_bytes_peak"]}
+        out["rates"][f"{rate:g}"] = row
+        print(f"[bench] serving rate {rate:g}: {row}", file=sys.stderr, flush=True)
+    S.close()
+    for sk in keys:
+        sk.close()
     return out
 
 
@@ -239,7 +321,7 @@ def main():
     if args.compare_modes:
         mode_cmp = {}
         for mode in (1, 2):
-            T.lib().tfhe_amd_set_launch_config(mode, 0, 0)
+            T.lib().tfhe_amd_set_launch_config(mode, -1)
             step()
             T.synchronize()
             t1 = time.perf_counter()
@@ -248,7 +330,7 @@ def main():
             T.synchronize()
             mode_cmp[{1: "one_wave_per_bootstrap", 2: "four_gate_lockstep"}[mode]] = round(
                 B * args.steps / (time.perf_counter() - t1), 2)
-        T.lib().tfhe_amd_set_launch_config(0, 0, 0)
+        T.lib().tfhe_amd_set_launch_config(0, -1)
         print(f"[bench] modes {mode_cmp}", file=sys.stderr, flush=True)
 
     # one gate per launch (same kernels, count = 1, synchronised each time)
@@ -270,9 +352,15 @@ def main():
     print(f"[bench] per-gate {per_gate:.1f} gates/s, p50 {np.percentile(lat_ms, 50):.3f} ms", file=sys.stderr,
           flush=True)
 
+    cross, tune = T.autotune(sk.cloud)
+    autotune = {"small_batch_max": cross,
+                "ms_four_gate_lockstep": {str(k): round(v[0], 3) for k, v in tune.items()},
+                "ms_two_wave_split": {str(k): round(v[1], 3) for k, v in tune.items()}}
+    print(f"[bench] autotune: two-wavefront kernel up to {cross} gates", file=sys.stderr, flush=True)
     sizes = [int(x) for x in args.scaling.split(",") if x.strip()]
     scaling = bench_scaling(T, sk, P, sizes, args.seed) if sizes else None
     adders = bench_adders(T, sk, P, args.adders, args.adder_bits, 2, args.seed) if args.adders > 0 else None
+    serving = bench_serving(T, P, args) if args.serve_clients > 0 else None
 
     br_avg_s = br_ms / max(launches, 1) / 1e3
     achieved_tflops = FLOP_PER_BOOTSTRAP_P128 * B / br_avg_s / 1e12 if br_avg_s > 0 else None
@@ -299,8 +387,10 @@ def main():
         "batch_latency_ms": {"p50": round(ms_per_step, 4), "p99": round(ms_per_step, 4)},
         "decrypt_errors": errors,
         "kernel_variants_gates_per_s": mode_cmp,
+        "autotune": autotune,
         "batch_size_sweep": scaling,
         "netlist_adders": adders,
+        "serving": serving,
         "kernel_ms": {"blind_rotate_avg": round(br_ms / max(launches, 1), 4),
                       "keyswitch_avg": round(ks_ms / max(launches, 1), 4)},
         "roofline": {"bound": "mfma", "achieved": round(achieved_tflops, 3) if achieved_tflops else None,

@@ -268,10 +268,59 @@ TFHE_AMD_EXPORT int32_t tfhe_amd_netlist_levels(const tfhe_amd_netlist* nl, int3
 TFHE_AMD_EXPORT int32_t tfhe_amd_netlist_run(tfhe_amd_netlist* nl, int32_t* d_pool, void* stream, int32_t use_graph);
 TFHE_AMD_EXPORT void tfhe_amd_netlist_destroy(tfhe_amd_netlist* nl);
 
-/* Launch configuration (autotuner hook). mode: 0 = automatic, 1 = one
- * wavefront per bootstrap, 2 = two wavefronts per bootstrap. */
-TFHE_AMD_EXPORT void tfhe_amd_set_launch_config(int32_t br_mode, int32_t br_blocks_per_cu, int32_t ks_gates_per_block);
-TFHE_AMD_EXPORT int32_t tfhe_amd_get_launch_config(int32_t* br_mode, int32_t* br_blocks_per_cu, int32_t* ks_gates_per_block);
+/* Serving loop (no counterpart in the reference; its callers are circuits
+ * like test/test-long-run.cpp:61-185 submitted by many clients). One
+ * scheduler thread admits requests in arrival order as their wires fit in a
+ * device wire arena of wire_budget_bytes, batches the next dependency level of
+ * every admitted request of a client into one launch (requests that arrive
+ * while others run join the next launch), spreads clients over n_streams
+ * streams with up to `depth` launches in flight per client, and copies each
+ * request's outputs back when its last level has finished. Keys stay resident
+ * (their HBM is counted in key_bytes); all clients of a server share one
+ * parameter set. 0 for n_streams / depth / max_batch picks 4 / 2 / 4096. */
+typedef struct tfhe_amd_server tfhe_amd_server;
+typedef struct {
+    int64_t submitted, completed, failed, gates, launches, max_launch_gates;
+    int32_t queued, active, clients, n_streams;
+    size_t wire_bytes_budget, wire_bytes_in_use, wire_bytes_peak, key_bytes;
+} tfhe_amd_server_stats;
+TFHE_AMD_EXPORT tfhe_amd_server* tfhe_amd_server_create(int32_t n_streams, int32_t depth, int32_t max_batch,
+                                                        size_t wire_budget_bytes);
+/* Registers a client's cloud key (uploaded now if needed); returns the client id. */
+TFHE_AMD_EXPORT int32_t tfhe_amd_server_add_client(tfhe_amd_server* srv, const TFheGateBootstrappingCloudKeySet* ck);
+/* Submits a netlist (same conventions as tfhe_amd_netlist_create) whose
+ * undriven wires input_wires[0..n_inputs) hold input_cts[n_inputs][n+1]
+ * (copied now). output_cts[n_outputs][n+1] receives the ciphertexts of
+ * output_wires when the request completes and must stay valid until then.
+ * Returns a request id >= 0, or -1 (bad netlist, wire read but neither input
+ * nor driven, request larger than the wire budget). */
+TFHE_AMD_EXPORT int64_t tfhe_amd_server_submit(tfhe_amd_server* srv, int32_t client, int32_t n_wires, int32_t n_gates,
+                                               const int32_t* op, const int32_t* in0, const int32_t* in1,
+                                               const int32_t* in2, const int32_t* out, int32_t n_inputs,
+                                               const int32_t* input_wires, const int32_t* input_cts,
+                                               int32_t n_outputs, const int32_t* output_wires, int32_t* output_cts);
+/* Waits for a request (timeout_ms < 0: no limit). 0 = done (latency_ms =
+ * submit to outputs written; the id is then retired), 1 = timed out, -1 = failed. */
+TFHE_AMD_EXPORT int32_t tfhe_amd_server_wait(tfhe_amd_server* srv, int64_t request, int32_t timeout_ms,
+                                             double* latency_ms);
+TFHE_AMD_EXPORT int32_t tfhe_amd_server_get_stats(tfhe_amd_server* srv, tfhe_amd_server_stats* stats);
+/* Finishes every submitted request, then frees the server. */
+TFHE_AMD_EXPORT void tfhe_amd_server_destroy(tfhe_amd_server* srv);
+
+/* Blind-rotation kernel choice. mode: 0 = automatic, 1 = one wavefront per
+ * bootstrap, 2 = four bootstraps per workgroup in lockstep, 3 = two
+ * wavefronts per bootstrap (P128-sized keys). In automatic mode batches of
+ * at most small_batch_max gates use mode 3, larger ones mode 2
+ * (small_batch_max < 0 keeps the current value). */
+TFHE_AMD_EXPORT void tfhe_amd_set_launch_config(int32_t br_mode, int32_t small_batch_max);
+TFHE_AMD_EXPORT int32_t tfhe_amd_get_launch_config(int32_t* br_mode, int32_t* small_batch_max);
+/* Autotuner: times modes 2 and 3 on this key at batch sizes 128, 256, 384,
+ * 512, 640, 768, 896, 1024, 1280, 1536 (milliseconds per batch, including
+ * the keyswitch, into ms_four / ms_two if not NULL, 10 entries each) and sets
+ * small_batch_max to the largest size up to which mode 3 is faster at every
+ * measured point. Synchronous; allocates and frees its own buffers. */
+TFHE_AMD_EXPORT int32_t tfhe_amd_autotune(const TFheGateBootstrappingCloudKeySet* ck, int32_t* small_batch_max,
+                                          float* ms_four, float* ms_two);
 
 /* Per-launch timing: when enabled, every device gate batch records HIP
  * events on its stream around the blind-rotation kernel and around the

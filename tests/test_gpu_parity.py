@@ -274,14 +274,14 @@ def test_all_blind_rotation_kernels_bitexact(p128, mode):
     rng = np.random.default_rng(20 + mode)
     ops = np.array([10, 0, 3, 10, 5, 1, 2], np.int32)  # 7 gates, 9 bootstraps
     bits, a, b, c = _encrypt_inputs(sk, rng, len(ops))
-    L.tfhe_amd_set_launch_config(mode, 0, 0)
+    L.tfhe_amd_set_launch_config(mode, -1)
     try:
         got = T.run_gates(sk.cloud, ops, a, b, c)
         big_ops = rng.integers(0, 11, size=301).astype(np.int32)
         bb, ba, bbb, bc = _encrypt_inputs(sk, rng, len(big_ops))
         big = T.run_gates(sk.cloud, big_ops, ba, bbb, bc)
     finally:
-        L.tfhe_amd_set_launch_config(0, 0, 0)
+        L.tfhe_amd_set_launch_config(0, -1)
     ref = O.gates(keys, ops, a, b, c, threads=16)
     assert np.array_equal(got, ref)
     exp = np.array([O.TRUTH[O.GATES[o]](x, y, z) for o, x, y, z in zip(big_ops, bb[0], bb[1], bb[2])])

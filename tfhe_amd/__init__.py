@@ -20,6 +20,7 @@ MUX = GATE_CODE["MUX"]
 i32 = ctypes.c_int32
 i32p = ctypes.POINTER(ctypes.c_int32)
 vp = ctypes.c_void_p
+f32p = ctypes.POINTER(ctypes.c_float)
 
 
 class LweParams(ctypes.Structure):
@@ -133,8 +134,9 @@ _SIGS = {
     "tfhe_amd_gates_device": (i32, [CKP, i32, vp, vp, vp, vp, vp, vp, vp]),
     "tfhe_amd_gates_device_woks": (i32, [CKP, i32, vp, vp, vp, vp, vp, vp, vp]),
     "tfhe_amd_keyswitch_device": (i32, [CKP, i32, vp, vp, vp]),
-    "tfhe_amd_set_launch_config": (None, [i32, i32, i32]),
-    "tfhe_amd_get_launch_config": (i32, [i32p, i32p, i32p]),
+    "tfhe_amd_set_launch_config": (None, [i32, i32]),
+    "tfhe_amd_get_launch_config": (i32, [i32p, i32p]),
+    "tfhe_amd_autotune": (i32, [CKP, i32p, f32p, f32p]),
     "tfhe_amd_profile_enable": (None, [i32]),
     "tfhe_amd_profile_read": (i32, [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double), i32p]),
     "tfhe_amd_copy_device": (i32, [vp, vp, ctypes.c_size_t, vp]),
@@ -299,6 +301,30 @@ def gates_device(cloud, ops, in0, in1, in2, out, pool, stream=None):
     count = ops.shape[0]
     check(lib().tfhe_amd_gates_device(cloud, count, ops.ptr, in0.ptr, in1.ptr, in2.ptr, out.ptr, pool.ptr, stream),
           "gates_device")
+
+
+AUTOTUNE_SIZES = (128, 256, 384, 512, 640, 768, 896, 1024, 1280, 1536)
+
+
+def autotune(cloud):
+    """Measures the two blind-rotation kernels on this key over AUTOTUNE_SIZES
+    and sets the automatic mode's small-batch crossover. Returns
+    (crossover, {size: (ms four-bootstrap kernel, ms two-wavefront kernel)})."""
+    cross = ctypes.c_int32()
+    four = (ctypes.c_float * len(AUTOTUNE_SIZES))()
+    two = (ctypes.c_float * len(AUTOTUNE_SIZES))()
+    check(lib().tfhe_amd_autotune(cloud, ctypes.byref(cross), four, two), "autotune")
+    return cross.value, {s: (four[i], two[i]) for i, s in enumerate(AUTOTUNE_SIZES)}
+
+
+def set_launch_config(mode, small_batch_max=-1):
+    lib().tfhe_amd_set_launch_config(int(mode), int(small_batch_max))
+
+
+def get_launch_config():
+    m, sb = ctypes.c_int32(), ctypes.c_int32()
+    lib().tfhe_amd_get_launch_config(ctypes.byref(m), ctypes.byref(sb))
+    return m.value, sb.value
 
 
 def profile_enable(on=True):

@@ -123,3 +123,104 @@ class Netlist:
         if self.ptr:
             _bind().tfhe_amd_netlist_destroy(self.ptr)
             self.ptr = None
+
+
+def random_layered_circuit(rng, depth, width, ops=tuple(range(11))):
+    """Layered random circuit (the shape of test/test-long-run.cpp:61-185's
+    random gate sequences, arranged in levels): `width` input wires, then
+    `depth` layers of `width` gates whose inputs are drawn uniformly from the
+    previous layer, gate types uniform over `ops` (MUX takes three inputs).
+    Returns the circuit and the wires of the last layer."""
+    C = Circuit(width)
+    layer = np.arange(width)
+    ops = np.asarray(ops, np.int64)
+    for _ in range(depth):
+        op = rng.choice(ops, size=width)
+        ins = rng.choice(layer, size=(3, width))
+        first = C.n_wires
+        C.op.extend(op.tolist())
+        C.in0.extend(ins[0].tolist())
+        C.in1.extend(ins[1].tolist())
+        C.in2.extend(np.where(op == MUX, ins[2], ins[0]).tolist())
+        C.out.extend(range(first, first + width))
+        C.n_wires += width
+        layer = np.arange(first, first + width)
+    return C, layer
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("submitted", ctypes.c_int64), ("completed", ctypes.c_int64), ("failed", ctypes.c_int64),
+                ("gates", ctypes.c_int64), ("launches", ctypes.c_int64), ("max_launch_gates", ctypes.c_int64),
+                ("queued", ctypes.c_int32), ("active", ctypes.c_int32), ("clients", ctypes.c_int32),
+                ("n_streams", ctypes.c_int32), ("wire_bytes_budget", ctypes.c_size_t),
+                ("wire_bytes_in_use", ctypes.c_size_t), ("wire_bytes_peak", ctypes.c_size_t),
+                ("key_bytes", ctypes.c_size_t)]
+
+
+_SERVER_SIGS = {
+    "tfhe_amd_server_create": (vp, [i32, i32, i32, ctypes.c_size_t]),
+    "tfhe_amd_server_add_client": (i32, [vp, CKP]),
+    "tfhe_amd_server_submit": (ctypes.c_int64, [vp, i32, i32, i32, i32p, i32p, i32p, i32p, i32p, i32, i32p, i32p,
+                                                i32, i32p, i32p]),
+    "tfhe_amd_server_wait": (i32, [vp, ctypes.c_int64, i32, ctypes.POINTER(ctypes.c_double)]),
+    "tfhe_amd_server_get_stats": (i32, [vp, ctypes.POINTER(ServerStats)]),
+    "tfhe_amd_server_destroy": (None, [vp]),
+}
+
+
+class Server:
+    """Continuous-batching request loop (tfhe_amd_server_*). submit() returns a
+    request id; wait() returns (output ciphertexts [n_out][n+1], latency ms)."""
+
+    def __init__(self, n_streams=4, depth=2, max_batch=4096, wire_budget_bytes=1 << 30):
+        L = lib()
+        for name, (res, args) in _SERVER_SIGS.items():
+            f = getattr(L, name)
+            f.restype, f.argtypes = res, args
+        self.L = L
+        self.ptr = L.tfhe_amd_server_create(n_streams, depth, max_batch, wire_budget_bytes)
+        if not self.ptr:
+            raise RuntimeError(L.tfhe_amd_last_error().decode())
+        self._live = {}
+        self.width = None
+
+    def add_client(self, cloud, n):
+        cid = self.L.tfhe_amd_server_add_client(self.ptr, cloud)
+        if cid < 0:
+            raise RuntimeError(self.L.tfhe_amd_last_error().decode())
+        self.width = n + 1
+        return cid
+
+    def submit(self, client, circuit, input_wires, input_cts, output_wires):
+        op, in0, in1, in2, out = circuit.arrays()
+        iw = np.ascontiguousarray(input_wires, np.int32)
+        ic = np.ascontiguousarray(input_cts, np.int32)
+        ow = np.ascontiguousarray(output_wires, np.int32)
+        dst = np.zeros((len(ow), self.width), np.int32)
+        p = lambda a: a.ctypes.data_as(i32p)
+        rid = self.L.tfhe_amd_server_submit(self.ptr, client, circuit.n_wires, len(op), p(op), p(in0), p(in1),
+                                            p(in2), p(out), len(iw), p(iw), p(ic), len(ow), p(ow), p(dst))
+        if rid < 0:
+            raise RuntimeError(self.L.tfhe_amd_last_error().decode())
+        self._live[rid] = dst
+        return rid
+
+    def wait(self, rid, timeout_ms=-1):
+        lat = ctypes.c_double()
+        rc = self.L.tfhe_amd_server_wait(self.ptr, rid, timeout_ms, ctypes.byref(lat))
+        if rc == 1:
+            return None
+        if rc != 0:
+            self._live.pop(rid, None)
+            raise RuntimeError(self.L.tfhe_amd_last_error().decode())
+        return self._live.pop(rid), lat.value
+
+    def stats(self):
+        st = ServerStats()
+        check(self.L.tfhe_amd_server_get_stats(self.ptr, ctypes.byref(st)), "server_get_stats")
+        return {name: getattr(st, name) for name, _ in ServerStats._fields_}
+
+    def close(self):
+        if self.ptr:
+            self.L.tfhe_amd_server_destroy(self.ptr)
+            self.ptr = None

@@ -6,6 +6,7 @@
 #include <stdint.h>
 
 #include <mutex>
+#include <vector>
 
 #include "../../include/tfhe_amd.h"
 
@@ -51,5 +52,15 @@ int engine_launch_batch_locked(tfhe_amd_device_key* dk, int count, const int32_t
                                const int32_t* d_in1, const int32_t* d_in2, const int32_t* d_out, int32_t* d_pool,
                                char* scr, void* stream, bool capturing);
 int engine_param_n(const tfhe_amd_device_key* dk);
+// True when two device keys have the same parameters (ciphertext width, scratch sizes).
+bool engine_same_params(const tfhe_amd_device_key* a, const tfhe_amd_device_key* b);
+size_t engine_key_bytes(const tfhe_amd_device_key* dk);
+
+// Levelisation of a gate netlist (schedule 0 = as soon as possible, 1 = as
+// late as possible): gate order sorted by level and the gates per level.
+// Validates gate codes, wire ranges, single drivers and acyclicity.
+int engine_levelize(int32_t n_wires, int32_t n_gates, const int32_t* op, const int32_t* in0, const int32_t* in1,
+                    const int32_t* in2, const int32_t* out, int schedule, std::vector<int32_t>& order,
+                    std::vector<int32_t>& level_count);
 
 }  // namespace tfhe_amd

@@ -829,8 +829,8 @@ struct Ctx {
     Scratch scratch;
     void* h_pinned = nullptr;
     size_t h_cap = 0;
-    int br_mode = 0, br_bpc = 0;
-    int br_small_max = 1024;  // automatic mode: batches up to this many gates use two wavefronts per bootstrap
+    int br_mode = 0;
+    int br_small_max = 512;  // automatic mode: batches up to this many gates use two wavefronts per bootstrap
     // optional per-launch timing with HIP events on the launch stream
     bool prof = false;
     struct Ev { hipEvent_t e[3]; };
@@ -1074,6 +1074,12 @@ int engine_launch_batch_locked(tfhe_amd_device_key* dk, int count, const int32_t
 }
 
 int engine_param_n(const tfhe_amd_device_key* dk) { return dk->P.n; }
+
+bool engine_same_params(const tfhe_amd_device_key* a, const tfhe_amd_device_key* b) {
+    return !memcmp(&a->P, &b->P, sizeof(HostParams));
+}
+
+size_t engine_key_bytes(const tfhe_amd_device_key* dk) { return dk->bytes; }
 
 // ----------------------------------------------------------- key upload --
 
@@ -1375,22 +1381,98 @@ TFHE_AMD_EXPORT int32_t tfhe_amd_keyswitch_device(const TFheGateBootstrappingClo
                          stream ? (hipStream_t)stream : C.stream);
 }
 
-TFHE_AMD_EXPORT void tfhe_amd_set_launch_config(int32_t br_mode, int32_t br_blocks_per_cu, int32_t ks_gates_per_block) {
+TFHE_AMD_EXPORT void tfhe_amd_set_launch_config(int32_t br_mode, int32_t small_batch_max) {
     Ctx& C = ctx();
     std::lock_guard<std::mutex> g(C.mu);
     C.br_mode = br_mode;
-    C.br_bpc = br_blocks_per_cu;
-    if (ks_gates_per_block > 0) C.br_small_max = ks_gates_per_block;
-    (void)0;
+    if (small_batch_max >= 0) C.br_small_max = small_batch_max;
 }
 
-TFHE_AMD_EXPORT int32_t tfhe_amd_get_launch_config(int32_t* br_mode, int32_t* br_blocks_per_cu, int32_t* ks_gates_per_block) {
+TFHE_AMD_EXPORT int32_t tfhe_amd_get_launch_config(int32_t* br_mode, int32_t* small_batch_max) {
     Ctx& C = ctx();
     std::lock_guard<std::mutex> g(C.mu);
     if (br_mode) *br_mode = C.br_mode;
-    if (br_blocks_per_cu) *br_blocks_per_cu = C.br_bpc;
-    if (ks_gates_per_block) *ks_gates_per_block = 0;
+    if (small_batch_max) *small_batch_max = C.br_small_max;
     return 0;
+}
+
+// Times the four-bootstrap and the two-wavefront blind rotations (each with
+// its keyswitch) on trivial ciphertexts over a ladder of batch sizes and sets
+// the automatic mode's crossover to the largest size below which the
+// two-wavefront kernel is faster at every measured point.
+TFHE_AMD_EXPORT int32_t tfhe_amd_autotune(const TFheGateBootstrappingCloudKeySet* ck, int32_t* small_batch_max,
+                                          float* ms_four, float* ms_two) {
+    if (!ck) return set_error("null cloud key");
+    tfhe_amd_device_key* dk = engine_key(ck->bkFFT);
+    if (!dk) return -1;
+    Ctx& C = ctx();
+    std::lock_guard<std::mutex> g(C.mu);
+    static const int sizes[] = {128, 256, 384, 512, 640, 768, 896, 1024, 1280, 1536};
+    constexpr int NS = sizeof(sizes) / sizeof(sizes[0]);
+    if (dk->M != 512 || dk->P.n > BR2_NMAX) {
+        C.br_small_max = 0;
+        if (small_batch_max) *small_batch_max = 0;
+        return 0;
+    }
+    const int maxB = sizes[NS - 1], w = dk->P.n + 1;
+    int32_t *d_pool = nullptr, *d_idx = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    std::vector<int32_t> idx(4 * (size_t)maxB);
+    for (int i = 0; i < maxB; i++) {
+        idx[i] = TFHE_AMD_NAND;
+        idx[maxB + i] = i;
+        idx[2 * maxB + i] = maxB + i;
+        idx[3 * maxB + i] = 2 * maxB + i;
+    }
+    auto cleanup = [&](int rc) {
+        (void)hipStreamSynchronize(C.stream);
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        (void)hipFree(d_pool);
+        (void)hipFree(d_idx);
+        return rc;
+    };
+    if (hipMalloc(&d_pool, sizeof(int32_t) * 3 * (size_t)maxB * w) != hipSuccess ||
+        hipMalloc(&d_idx, sizeof(int32_t) * idx.size()) != hipSuccess)
+        return cleanup(set_error("autotune: device allocation failed"));
+    if (hipMemsetAsync(d_pool, 0, sizeof(int32_t) * 3 * (size_t)maxB * w, C.stream) != hipSuccess ||
+        hipMemcpyAsync(d_idx, idx.data(), sizeof(int32_t) * idx.size(), hipMemcpyHostToDevice, C.stream) != hipSuccess ||
+        hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+        return cleanup(set_error("autotune: set-up failed"));
+    if (int rc = ensure_scratch(C, ext_bytes(dk->P, maxB) + ks_scratch_bytes(dk->P, maxB))) return cleanup(rc);
+    const int saved = C.br_mode;
+    float best[2][NS];
+    int rc = 0;
+    for (int m = 0; m < 2 && !rc; m++) {
+        C.br_mode = m ? 3 : 2;
+        for (int si = 0; si < NS && !rc; si++) {
+            const int B = sizes[si];
+            best[m][si] = 1e30f;
+            for (int rep = 0; rep < 4 && !rc; rep++) {
+                rc = (int)(hipEventRecord(e0, C.stream) != hipSuccess);
+                if (!rc)
+                    rc = gates_device_locked(C, dk, B, d_idx, d_idx + maxB, d_idx + 2 * maxB, d_idx + maxB,
+                                             d_idx + 3 * maxB, d_pool, nullptr, 1 << 29, (char*)C.scratch.d,
+                                             C.stream, false);
+                float ms = 0;
+                if (!rc && (hipEventRecord(e1, C.stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                            hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+                    rc = set_error("autotune: event timing failed");
+                if (!rc && rep) best[m][si] = std::min(best[m][si], ms);  // first run warms up
+            }
+        }
+    }
+    C.br_mode = saved;
+    if (rc) return cleanup(rc < 0 ? rc : set_error("autotune: event record failed"));
+    int cross = 0;
+    for (int si = 0; si < NS && best[1][si] < best[0][si]; si++) cross = sizes[si];
+    C.br_small_max = cross;
+    if (small_batch_max) *small_batch_max = cross;
+    for (int si = 0; si < NS; si++) {
+        if (ms_four) ms_four[si] = best[0][si];
+        if (ms_two) ms_two[si] = best[1][si];
+    }
+    return cleanup(0);
 }
 
 TFHE_AMD_EXPORT void tfhe_amd_profile_enable(int32_t on) {

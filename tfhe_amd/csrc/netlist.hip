@@ -118,6 +118,19 @@ void drop_graph(tfhe_amd_netlist* nl) {
 }
 
 }  // namespace
+
+int engine_levelize(int32_t n_wires, int32_t n_gates, const int32_t* op, const int32_t* in0, const int32_t* in1,
+                    const int32_t* in2, const int32_t* out, int schedule, std::vector<int32_t>& order,
+                    std::vector<int32_t>& level_count) {
+    tfhe_amd_netlist tmp;
+    tmp.n_wires = n_wires;
+    tmp.n_gates = n_gates;
+    std::vector<int32_t> level;
+    if (int rc = build_levels(&tmp, op, in0, in1, in2, out, schedule, order, level)) return rc;
+    level_count = tmp.level_count;
+    return 0;
+}
+
 }  // namespace tfhe_amd
 
 using namespace tfhe_amd;
