@@ -45,6 +45,8 @@ def parse():
     ap.add_argument("--serve-seconds", type=float, default=10.0, help="serving section: arrival window per rate")
     ap.add_argument("--serve-min-requests", type=int, default=4)
     ap.add_argument("--serve-max-requests", type=int, default=160)
+    ap.add_argument("--pbig-launches", type=int, default=64,
+                    help="N=2048 mixed-batch section: launches (0 = skip)")
     ap.add_argument("--compare-modes", action="store_true",
                     help="also time the one-wavefront-per-bootstrap kernel on the same batch")
     ap.add_argument("--copy-calib-mb", type=int, default=0,
@@ -183,6 +185,77 @@ def bench_serving(T, P, args):
     S.close()
     for sk in keys:
         sk.close()
+    return out
+
+
+def bench_pbig(T, args):
+    """N = 2048 parameter set (n=750, l=2, Bgbit=10, stdevs 2^-17 / 2^-44,
+    keyswitch t=8 basebit=2): 16384 MUX + 16384 two-input gates (uniform over
+    the 10 two-input types), shuffled, over `launches` consecutive launches
+    whose sizes are drawn uniformly from [1, 8192] and scaled to sum to the
+    32768 gates. Inputs are fresh encryptions resident in HBM."""
+    P = T.big_params()
+    sk = T.SecretKey(P, seed=args.seed)
+    sk.upload()
+    rng = np.random.default_rng(args.seed + 4)
+    G = 32768
+    ops = np.concatenate([np.full(G // 2, 10), rng.integers(0, 10, size=G // 2)]).astype(np.int32)
+    rng.shuffle(ops)
+    u = rng.integers(1, 8193, size=args.pbig_launches).astype(np.float64)
+    sizes = np.maximum(1, np.floor(u * G / u.sum())).astype(np.int64)
+    sizes[-1] += G - sizes.sum()
+    while sizes[-1] < 1:  # keep every launch non-empty
+        j = int(np.argmax(sizes[:-1]))
+        sizes[j] -= 1
+        sizes[-1] += 1
+    bits = rng.integers(0, 2, size=(3, G)).astype(np.int32)
+    w = P.n + 1
+    pool = np.concatenate([sk.encrypt(bits[0]), sk.encrypt(bits[1]), sk.encrypt(bits[2]), np.zeros((G, w), np.int32)])
+    d_pool = T.DeviceArray.from_numpy(pool)
+    idx = np.arange(G, dtype=np.int32)
+    d_ops = T.DeviceArray.from_numpy(ops)
+    d_idx = T.DeviceArray.from_numpy(np.concatenate([idx, idx + G, idx + 2 * G, idx + 3 * G]))
+    i32 = 4
+
+    def run():
+        off = 0
+        for sz in sizes:
+            sz = int(sz)
+            T.check(T.lib().tfhe_amd_gates_device(
+                sk.cloud, sz, d_ops.ptr + i32 * off, d_idx.ptr + i32 * off, d_idx.ptr + i32 * (G + off),
+                d_idx.ptr + i32 * (2 * G + off), d_idx.ptr + i32 * (3 * G + off), d_pool.ptr, None), "gates")
+            off += sz
+
+    run()
+    T.synchronize()
+    T.profile_read()
+    T.profile_enable(True)
+    t0 = time.perf_counter()
+    run()
+    T.synchronize()
+    dt = time.perf_counter() - t0
+    br_ms, ks_ms, n_l = T.profile_read()
+    T.profile_enable(False)
+    dec = sk.decrypt(d_pool.numpy()[3 * G:])
+    truth = [lambda a, b, c: 1 - (a & b), lambda a, b, c: a & b, lambda a, b, c: a | b, lambda a, b, c: a ^ b,
+             lambda a, b, c: 1 - (a ^ b), lambda a, b, c: 1 - (a | b), lambda a, b, c: (1 - a) & b,
+             lambda a, b, c: a & (1 - b), lambda a, b, c: (1 - a) | b, lambda a, b, c: a | (1 - b),
+             lambda a, b, c: b if a else c]
+    exp = np.array([truth[o](x, y, z) for o, x, y, z in zip(ops, bits[0], bits[1], bits[2])])
+    errors = int((dec != exp).sum())
+    boots = G + G // 2
+    flop = 750 * 409600 * boots
+    out = {"workload": "N=2048 n=750 l=2 Bgbit=10 ks t=8 basebit=2: 16384 MUX + 16384 two-input gates over "
+                       f"{len(sizes)} launches, sizes U[1,8192] scaled to sum to 32768",
+           "gates_per_s": round(G / dt, 1), "bootstraps_per_s": round(boots / dt, 1), "ms_total": round(dt * 1e3, 2),
+           "launch_sizes": {"min": int(sizes.min()), "median": int(np.median(sizes)), "max": int(sizes.max())},
+           "blind_rotate_ms": round(br_ms, 2), "keyswitch_ms": round(ks_ms, 2),
+           "fp64_tflops_blind_rotate": round(flop / (br_ms / 1e3) / 1e12, 3) if br_ms > 0 else None,
+           "decrypt_errors": errors}
+    print(f"[bench] N=2048 mixed: {out}", file=sys.stderr, flush=True)
+    for d in (d_pool, d_ops, d_idx):
+        d.free()
+    sk.close()
     return out
 
 
@@ -361,6 +434,7 @@ def main():
     scaling = bench_scaling(T, sk, P, sizes, args.seed) if sizes else None
     adders = bench_adders(T, sk, P, args.adders, args.adder_bits, 2, args.seed) if args.adders > 0 else None
     serving = bench_serving(T, P, args) if args.serve_clients > 0 else None
+    pbig = bench_pbig(T, args) if args.pbig_launches > 0 else None
 
     br_avg_s = br_ms / max(launches, 1) / 1e3
     achieved_tflops = FLOP_PER_BOOTSTRAP_P128 * B / br_avg_s / 1e12 if br_avg_s > 0 else None
@@ -391,6 +465,7 @@ def main():
         "batch_size_sweep": scaling,
         "netlist_adders": adders,
         "serving": serving,
+        "pbig_mixed": pbig,
         "kernel_ms": {"blind_rotate_avg": round(br_ms / max(launches, 1), 4),
                       "keyswitch_avg": round(ks_ms / max(launches, 1), 4)},
         "roofline": {"bound": "mfma", "achieved": round(achieved_tflops, 3) if achieved_tflops else None,

@@ -1,0 +1,92 @@
+"""GPU parity for the N = 2048 parameter set (n = 750, l = 2, Bgbit = 10,
+keyswitch t = 8 / basebit = 2): keyswitch and blind rotation bit-exact against
+the oracle, gates bit-exact against the oracle on the same keys, and decrypted
+bits equal to the plaintext gates over a large mixed batch."""
+import ctypes
+
+import numpy as np
+import pytest
+
+import oracle_py as O
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def pbig(gpu):
+    T = gpu
+    P = T.big_params()
+    sk = T.SecretKey(P, seed=3)
+    keys = O.Keys(O.pbig(), **sk.export())
+    sk.upload()
+    yield T, P, sk, keys
+    sk.close()
+
+
+def rand_i32(rng, shape):
+    return rng.integers(-2 ** 31, 2 ** 31, size=shape, dtype=np.int64).astype(np.int32)
+
+
+def test_pbig_keyswitch_bitexact(pbig):
+    T, P, sk, keys = pbig
+    rng = np.random.default_rng(41)
+    ext = rand_i32(rng, (33, P.N + 1))
+    got = T.keyswitch(sk.cloud, P, ext)
+    for g in (0, 1, 17, 32):
+        assert np.array_equal(got[g], O.keyswitch(keys, ext[g])), f"row {g}"
+
+
+@pytest.mark.parametrize("row,nrows", [(0, 1), (749, 1), (300, 3)])
+def test_pbig_blind_rotate_exact(pbig, row, nrows):
+    T, P, sk, keys = pbig
+    L = T.lib()
+    rng = np.random.default_rng(row + 13 * nrows)
+    acc0 = rand_i32(rng, (2, P.N))
+    bara = rng.integers(1, 2 * P.N, size=nrows).astype(np.int32)
+    if nrows > 1:
+        bara[1] = P.N
+    acc = L.new_TLweSample(P.ptr.contents.tgsw_params.contents.tlwe_params)
+    base = sk.cloud.contents.bkFFT.contents.bkFFT
+    row_ptr = ctypes.cast(ctypes.addressof(base.contents) + row * ctypes.sizeof(base._type_), type(base))
+    try:
+        for c in range(2):
+            ctypes.memmove(acc.contents.a[c].coefsT, acc0[c].ctypes.data, 4 * P.N)
+        L.tfhe_blindRotate_FFT(acc, row_ptr, bara.ctypes.data_as(T.i32p), nrows, P.ptr.contents.tgsw_params)
+        got = np.zeros((2, P.N), np.int32)
+        for c in range(2):
+            ctypes.memmove(got[c].ctypes.data, acc.contents.a[c].coefsT, 4 * P.N)
+    finally:
+        L.delete_TLweSample(acc)
+    ref = acc0.copy()
+    OP = O.pbig()
+    OP.n = nrows
+    O.lib().orc_blind_rotate(ctypes.byref(OP), O.ptr(np.ascontiguousarray(keys.bk[row:row + nrows])), O.ptr(bara),
+                             O.ptr(ref))
+    assert np.array_equal(got, ref), f"max diff {np.abs(got.astype(np.int64) - ref).max()}"
+
+
+def test_pbig_gates_bitexact_vs_oracle(pbig):
+    T, P, sk, keys = pbig
+    rng = np.random.default_rng(43)
+    ops = np.arange(11, dtype=np.int32)
+    bits = rng.integers(0, 2, size=(3, len(ops))).astype(np.int32)
+    a, b, c = (sk.encrypt(x) for x in bits)
+    got = T.run_gates(sk.cloud, ops, a, b, c)
+    ref = O.gates(keys, ops, a, b, c, threads=16)
+    assert np.array_equal(got, ref)
+    dec = sk.decrypt(got)
+    for g, op in enumerate(ops):
+        assert dec[g] == O.TRUTH[O.GATES[op]](bits[0][g], bits[1][g], bits[2][g])
+
+
+def test_pbig_mixed_batch_truth(pbig):
+    """1024 MUX + 1024 two-input gates in one batch decrypt correctly."""
+    T, P, sk, keys = pbig
+    rng = np.random.default_rng(44)
+    ops = np.concatenate([np.full(1024, 10), rng.integers(0, 10, size=1024)]).astype(np.int32)
+    rng.shuffle(ops)
+    bits = rng.integers(0, 2, size=(3, len(ops))).astype(np.int32)
+    a, b, c = (sk.encrypt(x) for x in bits)
+    got = T.run_gates(sk.cloud, ops, a, b, c)
+    exp = np.array([O.TRUTH[O.GATES[o]](x, y, z) for o, x, y, z in zip(ops, *bits)])
+    assert np.array_equal(sk.decrypt(got), exp)
