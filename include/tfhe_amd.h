@@ -268,6 +268,23 @@ TFHE_AMD_EXPORT int32_t tfhe_amd_netlist_levels(const tfhe_amd_netlist* nl, int3
 TFHE_AMD_EXPORT int32_t tfhe_amd_netlist_run(tfhe_amd_netlist* nl, int32_t* d_pool, void* stream, int32_t use_graph);
 TFHE_AMD_EXPORT void tfhe_amd_netlist_destroy(tfhe_amd_netlist* nl);
 
+/* Client-side batch encryption / decryption on the device, over contiguous
+ * ciphertext rows [count][n+1]. decrypt: bits[g] = (phase > 0) as
+ * bootsSymDecrypt (libtfhe/tfhe_gate_bootstrapping.cpp:162-165), phases[g] =
+ * b - <a, s> as lwePhase (libtfhe/lwe-functions.cpp:72-81); either output may
+ * be NULL. encrypt: bootsSymEncrypt (tfhe_gate_bootstrapping.cpp:154-159) of
+ * bits[g] with noise stdev alpha_min, randomness from Philox4x32-10 keyed by
+ * seed with ciphertext index first_index + g as counter (reproducible from
+ * (seed, index); not the reference generator's stream). The LWE key is copied
+ * to the device on first use and freed with the key set. Asynchronous on
+ * stream (NULL = the library's stream). */
+TFHE_AMD_EXPORT int32_t tfhe_amd_encrypt_device(const TFheGateBootstrappingSecretKeySet* key, int32_t count,
+                                                const int32_t* d_bits, int32_t* d_cts, uint64_t seed,
+                                                uint64_t first_index, void* stream);
+TFHE_AMD_EXPORT int32_t tfhe_amd_decrypt_device(const TFheGateBootstrappingSecretKeySet* key, int32_t count,
+                                                const int32_t* d_cts, int32_t* d_bits, int32_t* d_phases,
+                                                void* stream);
+
 /* Serving loop (no counterpart in the reference; its callers are circuits
  * like test/test-long-run.cpp:61-185 submitted by many clients). One
  * scheduler thread admits requests in arrival order as their wires fit in a
@@ -335,6 +352,8 @@ TFHE_AMD_EXPORT int32_t tfhe_amd_profile_read(double* br_ms, double* ks_ms, int3
 TFHE_AMD_EXPORT int32_t tfhe_amd_copy_device(void* dst, const void* src, size_t bytes, void* stream);
 TFHE_AMD_EXPORT void* tfhe_amd_malloc(size_t bytes);
 TFHE_AMD_EXPORT int32_t tfhe_amd_free(void* ptr);
+/* Host <-> device copies, ordered after the work queued on the library's
+ * stream; they have completed when the call returns. */
 TFHE_AMD_EXPORT int32_t tfhe_amd_memcpy_h2d(void* dst, const void* src, size_t bytes);
 TFHE_AMD_EXPORT int32_t tfhe_amd_memcpy_d2h(void* dst, const void* src, size_t bytes);
 TFHE_AMD_EXPORT int32_t tfhe_amd_synchronize(void);

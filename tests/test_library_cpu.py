@@ -95,3 +95,12 @@ def test_c_program_host_part(tmp_path):
                     "-Wl,-rpath," + os.path.join(ROOT, "tfhe_amd")], check=True)
     r = subprocess.run([exe, "host"], capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
+
+
+def test_philox_known_answers(tmp_path):
+    """The device encryption's generator reproduces Philox4x32-10's published
+    known-answer vectors (compiled for the host from the same header)."""
+    exe = str(tmp_path / "philox_kat")
+    src = os.path.join(os.path.dirname(os.path.abspath(__file__)), "c", "philox_kat.cpp")
+    subprocess.run(["g++", "-O2", "-std=c++17", "-o", exe, src], check=True)
+    assert subprocess.run([exe], capture_output=True, text=True, check=True).stdout.strip() == "OK"

@@ -134,6 +134,8 @@ _SIGS = {
     "tfhe_amd_gates_device": (i32, [CKP, i32, vp, vp, vp, vp, vp, vp, vp]),
     "tfhe_amd_gates_device_woks": (i32, [CKP, i32, vp, vp, vp, vp, vp, vp, vp]),
     "tfhe_amd_keyswitch_device": (i32, [CKP, i32, vp, vp, vp]),
+    "tfhe_amd_encrypt_device": (i32, [SKP, i32, vp, vp, ctypes.c_uint64, ctypes.c_uint64, vp]),
+    "tfhe_amd_decrypt_device": (i32, [SKP, i32, vp, vp, vp, vp]),
     "tfhe_amd_set_launch_config": (None, [i32, i32]),
     "tfhe_amd_get_launch_config": (i32, [i32p, i32p]),
     "tfhe_amd_autotune": (i32, [CKP, i32p, f32p, f32p]),
@@ -301,6 +303,21 @@ def gates_device(cloud, ops, in0, in1, in2, out, pool, stream=None):
     count = ops.shape[0]
     check(lib().tfhe_amd_gates_device(cloud, count, ops.ptr, in0.ptr, in1.ptr, in2.ptr, out.ptr, pool.ptr, stream),
           "gates_device")
+
+
+def encrypt_device(sk, d_bits, d_cts, seed, first_index=0, count=None, stream=None):
+    """Device encryption of count bits (DeviceArray) into ciphertext rows d_cts."""
+    count = d_bits.shape[0] if count is None else count
+    check(lib().tfhe_amd_encrypt_device(sk.ptr, count, d_bits.ptr, d_cts.ptr, seed, first_index, stream),
+          "encrypt_device")
+
+
+def decrypt_device(sk, d_cts, d_bits=None, d_phases=None, count=None, stream=None, offset_rows=0):
+    """Device decryption of count ciphertext rows starting at row offset_rows of d_cts."""
+    count = d_cts.shape[0] - offset_rows if count is None else count
+    src = d_cts.ptr + 4 * offset_rows * (sk.params.n + 1)
+    check(lib().tfhe_amd_decrypt_device(sk.ptr, count, src, d_bits.ptr if d_bits else None,
+                                        d_phases.ptr if d_phases else None, stream), "decrypt_device")
 
 
 AUTOTUNE_SIZES = (128, 256, 384, 512, 640, 768, 896, 1024, 1280, 1536)

@@ -39,13 +39,15 @@ def build(verbose_resources=False):
     net_o = os.path.join(BUILD, "netlist.o")
     host_o = os.path.join(BUILD, "host.o")
     srv_o = os.path.join(BUILD, "serve.o")
+    cli_o = os.path.join(BUILD, "client.o")
     extra = ["-Rpass-analysis=kernel-resource-usage"] if verbose_resources else []
     _run([HIPCC] + HIP_FLAGS + extra + ["-c", os.path.join(CSRC, "engine.hip"), "-o", eng_o])
     _run([HIPCC] + HIP_FLAGS + ["-c", os.path.join(CSRC, "netlist.hip"), "-o", net_o])
     _run([HIPCC] + HIP_FLAGS + ["-c", os.path.join(CSRC, "serve.hip"), "-o", srv_o])
+    _run([HIPCC] + HIP_FLAGS + ["-c", os.path.join(CSRC, "client.hip"), "-o", cli_o])
     _run([CXX] + HOST_FLAGS + ["-c", os.path.join(CSRC, "host.cpp"), "-o", host_o])
     tmp = OUT + ".tmp"
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, eng_o, net_o, srv_o, host_o,
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, eng_o, net_o, srv_o, cli_o, host_o,
           "-Wl,-Bsymbolic", "-Wl,--no-undefined", "-lpthread"])
     os.replace(tmp, OUT)
     return OUT

@@ -41,6 +41,8 @@ int engine_blind_rotate_host(tfhe_amd_device_key* dk, int32_t row0, int32_t n, c
 int engine_keyswitch_host(tfhe_amd_device_key* dk, const int32_t* in, int32_t* out);
 
 void engine_forget_host_key(const LweBootstrappingKeyFFT* bkfft);
+// Frees the device copy of an LWE secret key (client-side device encryption).
+void engine_forget_secret_key(const LweKey* k);
 
 // For the level scheduler: device scratch a batch of `count` gates needs, the
 // engine lock, its stream, and a batch launch with caller-provided scratch

@@ -1529,13 +1529,31 @@ TFHE_AMD_EXPORT int32_t tfhe_amd_free(void* ptr) {
     return 0;
 }
 
+// Both copies are ordered after the work queued on the library's stream and
+// complete before returning.
 TFHE_AMD_EXPORT int32_t tfhe_amd_memcpy_h2d(void* dst, const void* src, size_t bytes) {
-    HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    Ctx& C = ctx();
+    hipStream_t s;
+    {
+        std::lock_guard<std::mutex> g(C.mu);
+        if (int rc = ensure_ready(C)) return rc;
+        s = C.stream;
+    }
+    HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipStreamSynchronize(s));
     return 0;
 }
 
 TFHE_AMD_EXPORT int32_t tfhe_amd_memcpy_d2h(void* dst, const void* src, size_t bytes) {
-    HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    Ctx& C = ctx();
+    hipStream_t s;
+    {
+        std::lock_guard<std::mutex> g(C.mu);
+        if (int rc = ensure_ready(C)) return rc;
+        s = C.stream;
+    }
+    HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
     return 0;
 }
 

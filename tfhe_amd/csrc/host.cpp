@@ -512,6 +512,7 @@ TFHE_AMD_EXPORT TFheGateBootstrappingCloudKeySet* new_gate_bootstrapping_cloud_k
 
 TFHE_AMD_EXPORT void delete_gate_bootstrapping_secret_keyset(TFheGateBootstrappingSecretKeySet* keyset) {
     if (!keyset) return;
+    engine_forget_secret_key(keyset->lwe_key);
     delete_bkfft(keyset->cloud.bkFFT);
     delete_bk(keyset->cloud.bk);
     delete reinterpret_cast<SecretOwner*>(keyset);
