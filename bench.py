@@ -47,6 +47,7 @@ def parse():
     ap.add_argument("--serve-max-requests", type=int, default=160)
     ap.add_argument("--pbig-launches", type=int, default=64,
                     help="N=2048 mixed-batch section: launches (0 = skip)")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default small-batch crossover")
     ap.add_argument("--compare-modes", action="store_true",
                     help="also time the one-wavefront-per-bootstrap kernel on the same batch")
     ap.add_argument("--copy-calib-mb", type=int, default=0,
@@ -190,10 +191,13 @@ def bench_serving(T, P, args):
 
 def bench_pbig(T, args):
     """N = 2048 parameter set (n=750, l=2, Bgbit=10, stdevs 2^-17 / 2^-44,
-    keyswitch t=8 basebit=2): 16384 MUX + 16384 two-input gates (uniform over
-    the 10 two-input types), shuffled, over `launches` consecutive launches
-    whose sizes are drawn uniformly from [1, 8192] and scaled to sum to the
-    32768 gates. Inputs are fresh encryptions resident in HBM."""
+    keyswitch t=8 basebit=2): a mixed list of 16384 MUX + 16384 two-input gates
+    (uniform over the 10 two-input types), shuffled, fresh encryptions resident
+    in HBM, run as `launches` consecutive launches with sizes drawn uniformly
+    from [1, 8192]. The drawn sizes sum to more than 32768, so two splits are
+    reported: "drawn" takes the gates cyclically from the list (every launch has
+    its drawn size), "scaled" scales the sizes to sum to exactly 32768. Plus
+    one launch of 8192 gates from the same list."""
     P = T.big_params()
     sk = T.SecretKey(P, seed=args.seed)
     sk.upload()
@@ -201,56 +205,72 @@ def bench_pbig(T, args):
     G = 32768
     ops = np.concatenate([np.full(G // 2, 10), rng.integers(0, 10, size=G // 2)]).astype(np.int32)
     rng.shuffle(ops)
-    u = rng.integers(1, 8193, size=args.pbig_launches).astype(np.float64)
-    sizes = np.maximum(1, np.floor(u * G / u.sum())).astype(np.int64)
-    sizes[-1] += G - sizes.sum()
-    while sizes[-1] < 1:  # keep every launch non-empty
-        j = int(np.argmax(sizes[:-1]))
-        sizes[j] -= 1
-        sizes[-1] += 1
+    drawn = rng.integers(1, 8193, size=args.pbig_launches).astype(np.int64)
+    scaled = np.maximum(1, np.floor(drawn * G / drawn.sum())).astype(np.int64)
+    scaled[-1] += G - scaled.sum()
+    while scaled[-1] < 1:  # keep every launch non-empty
+        j = int(np.argmax(scaled[:-1]))
+        scaled[j] -= 1
+        scaled[-1] += 1
     bits = rng.integers(0, 2, size=(3, G)).astype(np.int32)
     w = P.n + 1
     pool = np.concatenate([sk.encrypt(bits[0]), sk.encrypt(bits[1]), sk.encrypt(bits[2]), np.zeros((G, w), np.int32)])
     d_pool = T.DeviceArray.from_numpy(pool)
-    idx = np.arange(G, dtype=np.int32)
-    d_ops = T.DeviceArray.from_numpy(ops)
+    # index tables twice over so a launch can wrap around the end of the list
+    idx = np.arange(2 * G, dtype=np.int32) % G
+    ops2 = np.concatenate([ops, ops])
+    d_ops = T.DeviceArray.from_numpy(ops2)
     d_idx = T.DeviceArray.from_numpy(np.concatenate([idx, idx + G, idx + 2 * G, idx + 3 * G]))
     i32 = 4
-
-    def run():
-        off = 0
-        for sz in sizes:
-            sz = int(sz)
-            T.check(T.lib().tfhe_amd_gates_device(
-                sk.cloud, sz, d_ops.ptr + i32 * off, d_idx.ptr + i32 * off, d_idx.ptr + i32 * (G + off),
-                d_idx.ptr + i32 * (2 * G + off), d_idx.ptr + i32 * (3 * G + off), d_pool.ptr, None), "gates")
-            off += sz
-
-    run()
-    T.synchronize()
-    T.profile_read()
-    T.profile_enable(True)
-    t0 = time.perf_counter()
-    run()
-    T.synchronize()
-    dt = time.perf_counter() - t0
-    br_ms, ks_ms, n_l = T.profile_read()
-    T.profile_enable(False)
-    dec = sk.decrypt(d_pool.numpy()[3 * G:])
     truth = [lambda a, b, c: 1 - (a & b), lambda a, b, c: a & b, lambda a, b, c: a | b, lambda a, b, c: a ^ b,
              lambda a, b, c: 1 - (a ^ b), lambda a, b, c: 1 - (a | b), lambda a, b, c: (1 - a) & b,
              lambda a, b, c: a & (1 - b), lambda a, b, c: (1 - a) | b, lambda a, b, c: a | (1 - b),
              lambda a, b, c: b if a else c]
     exp = np.array([truth[o](x, y, z) for o, x, y, z in zip(ops, bits[0], bits[1], bits[2])])
+
+    def run(sizes):
+        off = 0
+        for sz in sizes:
+            sz = int(sz)
+            T.check(T.lib().tfhe_amd_gates_device(
+                sk.cloud, sz, d_ops.ptr + i32 * off, d_idx.ptr + i32 * off, d_idx.ptr + i32 * (2 * G + off),
+                d_idx.ptr + i32 * (4 * G + off), d_idx.ptr + i32 * (6 * G + off), d_pool.ptr, None), "gates")
+            off = (off + sz) % G
+
+    def timed(sizes):
+        T.synchronize()
+        T.profile_read()
+        T.profile_enable(True)
+        t0 = time.perf_counter()
+        run(sizes)
+        T.synchronize()
+        dt = time.perf_counter() - t0
+        br_ms, ks_ms, _ = T.profile_read()
+        T.profile_enable(False)
+        gates = int(np.sum(sizes))
+        # bootstraps: MUX gates count twice
+        mux = 0
+        off = 0
+        for sz in sizes:
+            sel = (np.arange(off, off + int(sz)) % G)
+            mux += int((ops[sel] == 10).sum())
+            off = (off + int(sz)) % G
+        boots = gates + mux
+        return {"gates": gates, "gates_per_s": round(gates / dt, 1), "bootstraps_per_s": round(boots / dt, 1),
+                "ms_total": round(dt * 1e3, 2), "blind_rotate_ms": round(br_ms, 2), "keyswitch_ms": round(ks_ms, 2),
+                "fp64_tflops_blind_rotate": round(750 * 409600 * boots / (br_ms / 1e3) / 1e12, 3) if br_ms > 0 else None}
+
+    run(scaled)  # warm-up, and every gate of the list written once
+    T.synchronize()
+    dec = sk.decrypt(d_pool.numpy()[3 * G:])
     errors = int((dec != exp).sum())
-    boots = G + G // 2
-    flop = 750 * 409600 * boots
-    out = {"workload": "N=2048 n=750 l=2 Bgbit=10 ks t=8 basebit=2: 16384 MUX + 16384 two-input gates over "
-                       f"{len(sizes)} launches, sizes U[1,8192] scaled to sum to 32768",
-           "gates_per_s": round(G / dt, 1), "bootstraps_per_s": round(boots / dt, 1), "ms_total": round(dt * 1e3, 2),
-           "launch_sizes": {"min": int(sizes.min()), "median": int(np.median(sizes)), "max": int(sizes.max())},
-           "blind_rotate_ms": round(br_ms, 2), "keyswitch_ms": round(ks_ms, 2),
-           "fp64_tflops_blind_rotate": round(flop / (br_ms / 1e3) / 1e12, 3) if br_ms > 0 else None,
+    out = {"workload": "N=2048 n=750 l=2 Bgbit=10 ks t=8 basebit=2: 16384 MUX + 16384 two-input gates, "
+                       f"{len(drawn)} launches with sizes U[1,8192]",
+           "drawn": dict(timed(drawn), launch_sizes={"min": int(drawn.min()), "median": int(np.median(drawn)),
+                                                     "max": int(drawn.max())}),
+           "scaled": dict(timed(scaled), launch_sizes={"min": int(scaled.min()), "median": int(np.median(scaled)),
+                                                       "max": int(scaled.max())}),
+           "batch_8192": timed([8192]),
            "decrypt_errors": errors}
     print(f"[bench] N=2048 mixed: {out}", file=sys.stderr, flush=True)
     for d in (d_pool, d_ops, d_idx):
@@ -425,11 +445,13 @@ def main():
     print(f"[bench] per-gate {per_gate:.1f} gates/s, p50 {np.percentile(lat_ms, 50):.3f} ms", file=sys.stderr,
           flush=True)
 
-    cross, tune = T.autotune(sk.cloud)
-    autotune = {"small_batch_max": cross,
-                "ms_four_gate_lockstep": {str(k): round(v[0], 3) for k, v in tune.items()},
-                "ms_two_wave_split": {str(k): round(v[1], 3) for k, v in tune.items()}}
-    print(f"[bench] autotune: two-wavefront kernel up to {cross} gates", file=sys.stderr, flush=True)
+    autotune = None
+    if not args.no_autotune:
+        cross, tune = T.autotune(sk.cloud)
+        autotune = {"small_batch_max": cross,
+                    "ms_four_gate_lockstep": {str(k): round(v[0], 3) for k, v in tune.items()},
+                    "ms_two_wave_split": {str(k): round(v[1], 3) for k, v in tune.items()}}
+        print(f"[bench] autotune: two-wavefront kernel up to {cross} gates", file=sys.stderr, flush=True)
     sizes = [int(x) for x in args.scaling.split(",") if x.strip()]
     scaling = bench_scaling(T, sk, P, sizes, args.seed) if sizes else None
     adders = bench_adders(T, sk, P, args.adders, args.adder_bits, 2, args.seed) if args.adders > 0 else None

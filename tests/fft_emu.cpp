@@ -46,14 +46,84 @@ struct Emu {
     }
 };
 
+// N = 2048 on a pair of wavefronts (FftPair): X[h][lane][slot]
+struct EmuPair {
+    using F = Fft<512>;
+    using X = c64[2][64][8];
+    std::vector<c64> tw;
+    EmuPair() : tw(PAIR_TW_COUNT) { build_twiddles_pair(tw.data()); }
+    const c64* tw1(int h, int lane) const { return tw.data() + (h * 64 + lane) * 16; }
+    const c64* tw2(int h, int lane) const { return tw1(h, lane) + 8; }
+    void forward(const double* p, c64 (*x)[8]) {  // x = X[2*64][8]
+        c64 u[2][64][8];
+        for (int h = 0; h < 2; h++)
+            for (int b = 0; b < 64; b++)
+                for (int a = 0; a < 8; a++) {
+                    const int j = 512 * h + 64 * a + b;
+                    u[h][b][a] = c64{p[j], p[j + 1024]};
+                }
+        for (int b = 0; b < 64; b++) {
+            for (int a = 0; a < 8; a++) x[b][a] = u[0][b][a], x[64 + b][a] = u[1][b][a];
+            FftPair::pair_fwd<0>(x[b], u[1][b]);
+            FftPair::pair_fwd<1>(x[64 + b], u[0][b]);
+        }
+        for (int h = 0; h < 2; h++) {
+            std::vector<c64> lds(F::LDS_C64);
+            c64 (*xh)[8] = x + 64 * h;
+            for (int l = 0; l < 64; l++) {
+                if (h == 0) FftPair::fwd1<0>(xh[l], l, lds.data(), tw1(0, l));
+                else FftPair::fwd1<1>(xh[l], l, lds.data(), tw1(1, l));
+            }
+            for (int l = 0; l < 64; l++) F::fwd2(xh[l], l, lds.data(), tw2(h, l)[1]);
+            for (int l = 0; l < 64; l++) F::fwd3(xh[l], l, lds.data());
+            for (int l = 0; l < 64; l++) F::fwd4(xh[l], l, lds.data());
+        }
+    }
+    void inverse(c64 (*x)[8], double* p) {
+        for (int h = 0; h < 2; h++) {
+            std::vector<c64> lds(F::LDS_C64);
+            c64 (*xh)[8] = x + 64 * h;
+            for (int l = 0; l < 64; l++) F::inv1(xh[l], l, lds.data());
+            for (int l = 0; l < 64; l++) F::inv2(xh[l], l, lds.data(), tw2(h, l)[1]);
+            for (int l = 0; l < 64; l++) F::inv3(xh[l], l, lds.data());
+            for (int l = 0; l < 64; l++) {
+                if (h == 0) FftPair::inv4<0>(xh[l], l, lds.data(), tw1(0, l));
+                else FftPair::inv4<1>(xh[l], l, lds.data(), tw1(1, l));
+            }
+        }
+        for (int b = 0; b < 64; b++) {
+            c64 y0[8], y1[8];
+            for (int a = 0; a < 8; a++) y0[a] = x[b][a], y1[a] = x[64 + b][a];
+            c64 u0[8], u1[8];
+            for (int a = 0; a < 8; a++) u0[a] = y0[a], u1[a] = y1[a];
+            FftPair::pair_inv<0>(u0, y1);
+            FftPair::pair_inv<1>(u1, y0);
+            for (int a = 0; a < 8; a++) {
+                const int j0 = 64 * a + b, j1 = 512 + 64 * a + b;
+                p[j0] = u0[a].re, p[j0 + 1024] = u0[a].im;
+                p[j1] = u1[a].re, p[j1 + 1024] = u1[a].im;
+            }
+        }
+    }
+};
+
+template <class Em, int M, int SLOTS>
+int run_with(Em& E, int trials, unsigned seed, int dbits);
+
 template <int M>
 int run(int trials, unsigned seed, int dbits) {
-    const int N = 2 * M;
     Emu<M> E;
+    return run_with<Emu<M>, M, M / 64>(E, trials, seed, dbits);
+}
+
+template <class Em, int M, int SLOTS>
+int run_with(Em& E, int trials, unsigned seed, int dbits) {
+    const int N = 2 * M;
     std::mt19937_64 rng(seed);
     double maxerr = 0;
     long mismatches = 0;
-    static c64 X[64][M / 64], Y[64][M / 64];
+    constexpr int LANES = M / SLOTS;
+    static c64 X[LANES][SLOTS], Y[LANES][SLOTS];
     std::vector<double> pd(N), pt(N), out(N);
     std::vector<int32_t> d(N), t(N);
     for (int tr = 0; tr < trials; tr++) {
@@ -69,8 +139,8 @@ int run(int trials, unsigned seed, int dbits) {
         if (tr == 1) { for (int j = 0; j < N; j++) { t[j] = 1000 * (j == N - 1); pt[j] = t[j]; } }
         E.forward(pd.data(), X);
         E.forward(pt.data(), Y);
-        for (int l = 0; l < 64; l++)
-            for (int s = 0; s < M / 64; s++) X[l][s] = cmul(X[l][s], Y[l][s]);
+        for (int l = 0; l < LANES; l++)
+            for (int s = 0; s < SLOTS; s++) X[l][s] = cmul(X[l][s], Y[l][s]);
         E.inverse(X, out.data());
         // exact product mod 2^32
         std::vector<uint32_t> ex(N, 0);
@@ -106,6 +176,10 @@ int main(int argc, char** argv) {
     const int dbits = argc > 4 ? atoi(argv[4]) : 7;
     if (N == 1024) return run<512>(trials, seed, dbits);
     if (N == 2048) return run<1024>(trials, seed, dbits);
+    if (N == -2048) {  // pair-of-wavefronts transform
+        EmuPair E;
+        return run_with<EmuPair, 1024, 8>(E, trials, seed, dbits);
+    }
     fprintf(stderr, "N must be 1024 or 2048\n");
     return 2;
 }

@@ -90,3 +90,24 @@ def test_pbig_mixed_batch_truth(pbig):
     got = T.run_gates(sk.cloud, ops, a, b, c)
     exp = np.array([O.TRUTH[O.GATES[o]](x, y, z) for o, x, y, z in zip(ops, *bits)])
     assert np.array_equal(sk.decrypt(got), exp)
+
+
+def test_pbig_pair_kernel_matches_one_wave_kernel(pbig):
+    """The pair-transform kernel (four wavefronts per bootstrap, the default for
+    N = 2048) and the one-wavefront kernel give identical ciphertexts, MUX
+    included, for an odd batch."""
+    T, P, sk, keys = pbig
+    rng = np.random.default_rng(45)
+    ops = rng.integers(0, 11, size=37).astype(np.int32)
+    ops[:3] = 10
+    bits = rng.integers(0, 2, size=(3, len(ops))).astype(np.int32)
+    a, b, c = (sk.encrypt(x) for x in bits)
+    T.set_launch_config(1)
+    try:
+        one = T.run_gates(sk.cloud, ops, a, b, c)
+    finally:
+        T.set_launch_config(0)
+    pair = T.run_gates(sk.cloud, ops, a, b, c)
+    assert np.array_equal(one, pair)
+    exp = np.array([O.TRUTH[O.GATES[o]](x, y, z) for o, x, y, z in zip(ops, *bits)])
+    assert np.array_equal(sk.decrypt(pair), exp)

@@ -38,6 +38,8 @@ struct tfhe_amd_device_key {
     c64* bkf;      // [n][kpl][k+1][M]     FP64 transform of every TGSW row polynomial
     int32_t* ksk;  // [k*N][t][base-1][npad]  non-trivial keyswitch rows, coefficient b at column n
     c64* tw;       // TwiddleTable<M>::COUNT
+    c64* bkp;      // M = 1024 only: [n][kpl][k+1][2][512], the pair-of-wavefronts transform (FftPair)
+    c64* twp;      // M = 1024 only: PAIR_TW_COUNT
     size_t bytes;
 };
 
@@ -683,6 +685,299 @@ __global__ __launch_bounds__(256, 2) void br2_kernel(BrArgs A, const int32_t* __
     }
 }
 
+// largest LWE dimension the pair kernel's LDS table holds
+constexpr int BRP_NMAX = 768;
+
+// Blind rotation for N = 2048 with four wavefronts per bootstrap and two
+// bootstraps per 512-thread workgroup. Wavefront (g, c, h): bootstrap g,
+// accumulator component c, half h of the pair-of-wavefronts transform
+// (FftPair): it owns ACC_c coefficients 512h + 64a + lane (+1024), a < 8.
+// Per key row and digit p the two wavefronts of component c rotate and
+// decompose ACC_c, exchange their folded halves for the radix-2 step, run one
+// 512-point transform each and multiply-accumulate their bins against key rows
+// (c*l + p), which the workgroup copies into LDS once for both bootstraps;
+// then the component partners swap partial sums and each pair inverse-
+// transforms its own component.
+template <int L>
+__global__ __launch_bounds__(512, 1) void brp_kernel(BrArgs A, const c64* __restrict__ bkp, const c64* __restrict__ twp,
+                                                    const int32_t* __restrict__ mux_gate,
+                                                    const int32_t* __restrict__ n_jobs) {
+    using F = Fft<512>;
+    using P = FftPair;
+    constexpr int R = 8, N = 2048, KPL = 2 * L, LOG2N2 = 12;
+    constexpr int NMAX = BRP_NMAX;
+    __shared__ c64 s_fft[8][F::LDS_C64];
+    __shared__ c64 s_bk[2][2048];     // key digit blocks (i, p), (i, L + p): [col][h][slot][lane]
+    __shared__ c64 s_tw1[2][R * 64];  // [h][k1 * 64 + lane]
+    __shared__ int16_t s_bara[2][NMAX];
+    __shared__ int32_t s_barb[2];
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int w = tid >> 6;
+    const int g = w >> 2, c = (w >> 1) & 1, h = w & 1;
+    const int njobs = *n_jobs;
+    const int job = blockIdx.x * 2 + g;
+    if (blockIdx.x * 2 >= njobs) return;  // whole block idle (uniform)
+    const bool active = job < njobs;
+    const int n = A.n;
+    int16_t* bara = s_bara[g];
+    c64* fb = s_fft[w];
+    int32_t* fbi = (int32_t*)fb;
+
+    // gate linear form and modulus switch, one wavefront per bootstrap
+    if (c == 1 && h == 0) {
+        if (active) {
+            const int gate = job < A.count ? job : mux_gate[job - A.count];
+            const int half = job < A.count ? 0 : 1;
+            const int op = A.op[gate];
+            int32_t k0, ca, cb, cc;
+            gate_coeffs(op, half, k0, ca, cb, cc);
+            const int32_t* a = A.pool + (size_t)A.in0[gate] * (n + 1);
+            const int32_t* b = (op == OP_IDENTITY) ? a : A.pool + (size_t)A.in1[gate] * (n + 1);
+            const int32_t* cp = (op == TFHE_AMD_MUX) ? A.pool + (size_t)A.in2[gate] * (n + 1) : a;
+            for (int i = lane; i <= n; i += 64) {
+                uint32_t v =
+                    (uint32_t)ca * (uint32_t)a[i] + (uint32_t)cb * (uint32_t)b[i] + (uint32_t)cc * (uint32_t)cp[i];
+                if (i == n) v += (uint32_t)k0;
+                // modSwitchFromTorus32(v, 2N), libtfhe/numeric-functions.cpp:59-65
+                const uint32_t bar = (v + (1u << (31 - LOG2N2))) >> (32 - LOG2N2);
+                if (i < n) bara[i] = (int16_t)bar;
+                else s_barb[g] = (int32_t)bar;
+            }
+        } else {
+            for (int i = lane; i < n; i += 64) bara[i] = 0;
+            if (lane == 0) s_barb[g] = 0;
+        }
+    }
+    for (int k = tid; k < 2 * R * 64; k += 512) {
+        const int hh = k >> 9, k1 = (k >> 6) & 7, l = k & 63;
+        s_tw1[hh][k1 * 64 + l] = twp[(hh * 64 + l) * 16 + k1];
+    }
+    const c64* tw1 = s_tw1[h] + lane;
+    const c64 tw2 = twp[(h * 64 + lane) * 16 + 9];
+    const uint32_t offset = A.offset;
+    const int Bgbit = A.Bgbit;
+    const uint32_t dmask = (1u << Bgbit) - 1u;
+    const int32_t dhalf = 1 << (Bgbit - 1);
+    __syncthreads();
+
+    // acc[hh*8 + a] = ACC_c[512 h + 64 a + lane + 1024 hh]
+    uint32_t acc[2 * R];
+#pragma unroll
+    for (int k = 0; k < 2 * R; k++) acc[k] = 0;
+    if (c == 1 && active) {
+        // ACC_1 = X^{2N-barb} * [mu...mu], lwe-bootstrapping-functions-fft.cpp:146-148
+        const int barb = s_barb[g];
+#pragma unroll
+        for (int hh = 0; hh < 2; hh++)
+#pragma unroll
+            for (int a = 0; a < R; a++) {
+                const int j = 512 * h + 64 * a + lane + 1024 * hh;
+                acc[hh * R + a] = (((j + barb) & (2 * N - 1)) < N) ? (uint32_t)A.mu : 0u - (uint32_t)A.mu;
+            }
+    }
+
+    for (int i = 0; i < n; i++) {
+        const int abar = bara[i];
+        c64 p0[R], p1[R];
+#pragma unroll
+        for (int s = 0; s < R; s++) p0[s] = p1[s] = c64{0.0, 0.0};
+        uint32_t tv[2 * R];
+#pragma unroll 1
+        for (int p = 0; p < L; p++) {
+            __syncthreads();  // key blocks and every transform buffer are free
+            {
+                // key digit blocks (i, p) and (i, L + p) -> LDS, 8 x 16 B per thread
+                const c64* q0 = bkp + ((size_t)i * KPL + p) * 2048 + tid;
+                const c64* q1 = bkp + ((size_t)i * KPL + L + p) * 2048 + tid;
+                c64* d0 = s_bk[0] + (tid & ~63);
+                c64* d1 = s_bk[1] + (tid & ~63);
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    __builtin_amdgcn_global_load_lds((const void*)(q0 + u * 512),
+                                                     (__attribute__((address_space(3))) void*)(d0 + u * 512), 16, 0, 0);
+                    __builtin_amdgcn_global_load_lds((const void*)(q1 + u * 512),
+                                                     (__attribute__((address_space(3))) void*)(d1 + u * 512), 16, 0, 0);
+                }
+            }
+            if (p == 0) {
+                // T = (X^abar - 1) * ACC_c + gadget offset; coefficient k of ACC_c sits in
+                // the buffer of wavefront h' = (k >> 9) & 1 of the pair, at int ((k >> 10) << 9) + (k & 511)
+#pragma unroll
+                for (int hh = 0; hh < 2; hh++)
+#pragma unroll
+                    for (int a = 0; a < R; a++) fbi[hh * 512 + 64 * a + lane] = (int32_t)acc[hh * R + a];
+                __syncthreads();
+                if (active) {
+                    const int32_t* pair0 = (const int32_t*)s_fft[w & ~1];
+                    constexpr int STRIDE = F::LDS_C64 * 4;  // ints per transform buffer
+#pragma unroll
+                    for (int hh = 0; hh < 2; hh++)
+#pragma unroll
+                        for (int a = 0; a < R; a++) {
+                            const int j = 512 * h + 64 * a + lane + 1024 * hh;
+                            const int idx = (j - abar) & (2 * N - 1);
+                            const int k = idx & (N - 1);
+                            const uint32_t v = (uint32_t)pair0[((k >> 9) & 1) * STRIDE + ((k >> 10) << 9) + (k & 511)];
+                            const uint32_t rot = (idx & N) ? 0u - v : v;
+                            tv[hh * R + a] = rot - acc[hh * R + a] + offset;
+                        }
+                }
+                __syncthreads();
+            }
+            c64 x[R];
+            if (active) {
+                // digit p of the truncating gadget decomposition, tgsw-functions.cpp:337-348
+                const int shift = 32 - (p + 1) * Bgbit;
+#pragma unroll
+                for (int a = 0; a < R; a++) {
+                    const int32_t d0v = (int32_t)((tv[a] >> shift) & dmask) - dhalf;
+                    const int32_t d1v = (int32_t)((tv[R + a] >> shift) & dmask) - dhalf;
+                    x[a] = c64{(double)d0v, (double)d1v};
+                    fb[a * 64 + lane] = x[a];
+                }
+            }
+            __syncthreads();
+            if (active) {
+                const c64* o = s_fft[w ^ 1];
+                c64 ov[R];
+#pragma unroll
+                for (int a = 0; a < R; a++) ov[a] = o[a * 64 + lane];
+                if (h == 0) P::pair_fwd<0>(x, ov);
+                else P::pair_fwd<1>(x, ov);
+            }
+            __syncthreads();  // the partner has read this buffer
+            if (active) {
+                if (h == 0) P::template fwd1<0, 64>(x, lane, fb, tw1);
+                else P::template fwd1<1, 64>(x, lane, fb, tw1);
+                wave_sync();
+                F::fwd2(x, lane, fb, tw2);
+                wave_sync();
+                F::fwd3(x, lane, fb);
+                wave_sync();
+                F::fwd4(x, lane, fb);
+            }
+            __syncthreads();  // key blocks landed
+            if (active) {
+                const c64* bk = s_bk[c] + h * 512;
+#pragma unroll
+                for (int s = 0; s < R; s++) {
+                    cmac(p0[s], x[s], bk[s * 64 + lane]);
+                    cmac(p1[s], x[s], bk[1024 + s * 64 + lane]);
+                }
+            }
+        }
+        // swap partial sums with the other component's wavefront of the same half
+        if (active) {
+            if (c) {
+#pragma unroll
+                for (int s = 0; s < R; s++) fb[s * 64 + lane] = p0[s];
+            } else {
+#pragma unroll
+                for (int s = 0; s < R; s++) fb[s * 64 + lane] = p1[s];
+            }
+        }
+        __syncthreads();
+        c64 x[R];
+        if (active) {
+            const c64* other = s_fft[w ^ 2];
+            if (c) {
+#pragma unroll
+                for (int s = 0; s < R; s++) x[s] = cadd(p1[s], other[s * 64 + lane]);
+            } else {
+#pragma unroll
+                for (int s = 0; s < R; s++) x[s] = cadd(p0[s], other[s * 64 + lane]);
+            }
+        }
+        __syncthreads();
+        if (active) {
+            F::inv1(x, lane, fb);
+            wave_sync();
+            F::inv2(x, lane, fb, tw2);
+            wave_sync();
+            F::inv3(x, lane, fb);
+            wave_sync();
+            if (h == 0) P::template inv4<0, 64>(x, lane, fb, tw1);
+            else P::template inv4<1, 64>(x, lane, fb, tw1);
+            wave_sync();
+#pragma unroll
+            for (int s = 0; s < R; s++) fb[s * 64 + lane] = x[s];
+        }
+        __syncthreads();
+        if (active) {
+            const c64* o = s_fft[w ^ 1];
+            c64 ov[R];
+#pragma unroll
+            for (int a = 0; a < R; a++) ov[a] = o[a * 64 + lane];
+            if (h == 0) P::pair_inv<0>(x, ov);
+            else P::pair_inv<1>(x, ov);
+            // ACC_c += result (lwe-bootstrapping-functions-fft.cpp:67-76)
+#pragma unroll
+            for (int a = 0; a < R; a++) {
+                acc[a] += (uint32_t)round_mod32(x[a].re);
+                acc[R + a] += (uint32_t)round_mod32(x[a].im);
+            }
+        }
+    }
+    if (!active) return;
+    // tLweExtractLweSampleIndex(.., 0, ..), libtfhe/lwe.cpp:41-56
+    int32_t* e = A.ext + (size_t)job * (N + 1);
+    if (c == 0) {
+#pragma unroll
+        for (int hh = 0; hh < 2; hh++)
+#pragma unroll
+            for (int a = 0; a < R; a++) {
+                const int j = 512 * h + 64 * a + lane + 1024 * hh;
+                const uint32_t v = acc[hh * R + a];
+                e[j == 0 ? 0 : N - j] = (int32_t)(j == 0 ? v : 0u - v);
+            }
+    } else if (h == 0 && lane == 0) {
+        e[N] = (int32_t)acc[0];
+    }
+}
+
+// Pair-of-wavefronts transform of every bootstrapping-key polynomial (N = 2048):
+// one 128-thread workgroup per polynomial, output [poly][h][slot][lane].
+__global__ __launch_bounds__(128) void bk_convert_pair_kernel(const int32_t* __restrict__ bk, c64* __restrict__ bkp,
+                                                              const c64* __restrict__ twp) {
+    using F = Fft<512>;
+    using P = FftPair;
+    constexpr int R = 8;
+    __shared__ c64 s_fft[2][F::LDS_C64];
+    const int lane = threadIdx.x & 63, h = threadIdx.x >> 6;
+    const size_t poly = blockIdx.x;
+    const int32_t* p = bk + poly * 2048;
+    c64* fb = s_fft[h];
+    c64 x[R], tw1[R];
+#pragma unroll
+    for (int a = 0; a < R; a++) {
+        x[a] = c64{(double)p[512 * h + 64 * a + lane], (double)p[512 * h + 64 * a + lane + 1024]};
+        fb[a * 64 + lane] = x[a];
+    }
+    __syncthreads();
+    c64 ov[R];
+#pragma unroll
+    for (int a = 0; a < R; a++) ov[a] = s_fft[h ^ 1][a * 64 + lane];
+    if (h == 0) P::pair_fwd<0>(x, ov);
+    else P::pair_fwd<1>(x, ov);
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < R; k++) tw1[k] = twp[(h * 64 + lane) * 16 + k];
+    const c64 tw2 = twp[(h * 64 + lane) * 16 + 9];
+    if (h == 0) P::fwd1<0>(x, lane, fb, tw1);
+    else P::fwd1<1>(x, lane, fb, tw1);
+    wave_sync();
+    F::fwd2(x, lane, fb, tw2);
+    wave_sync();
+    F::fwd3(x, lane, fb);
+    wave_sync();
+    F::fwd4(x, lane, fb);
+    c64* out = bkp + poly * 1024 + h * 512;
+#pragma unroll
+    for (int s = 0; s < R; s++) out[s * 64 + lane] = x[s];
+}
+
 template <int M>
 __global__ __launch_bounds__(64) void bk_convert_kernel(const int32_t* __restrict__ bk, c64* __restrict__ bkf,
                                                         const c64* __restrict__ tw) {
@@ -1019,12 +1314,18 @@ int gates_device_locked(Ctx& C, tfhe_amd_device_key* dk, int count, const int32_
     }
     // kernel choice: 1 = one wavefront per bootstrap, 2 = four bootstraps per workgroup,
     // 3 = two wavefronts per bootstrap; 0 = automatic (two wavefronts while the batch
-    // leaves CUs idle, four-gate lockstep for large batches)
+    // leaves CUs idle, four-gate lockstep for large batches). N = 2048 uses the
+    // pair-transform kernel (4) unless mode 1 is forced.
     int mode = C.br_mode;
-    if (dk->M != 512) mode = 1;
+    if (dk->M != 512) mode = (mode == 1 || dk->P.n > BRP_NMAX || !dk->bkp) ? 1 : 4;
+    else if (mode == 4) mode = 2;
     if (mode == 3 && dk->P.n > BR2_NMAX) mode = 2;
     if (mode == 0) mode = (count <= C.br_small_max && dk->P.n <= BR2_NMAX) ? 3 : 2;
-    if (mode == 2) {
+    if (mode == 4) {
+        const int blocks = count;  // two bootstraps per workgroup, at most 2*count jobs
+        if (dk->P.l == 2) hipLaunchKernelGGL(brp_kernel<2>, dim3(blocks), dim3(512), 0, s, A, dk->bkp, dk->twp, mux_gate, n_jobs);
+        else hipLaunchKernelGGL(brp_kernel<3>, dim3(blocks), dim3(512), 0, s, A, dk->bkp, dk->twp, mux_gate, n_jobs);
+    } else if (mode == 2) {
         const int blocks = (2 * count + 3) / 4;
         if (dk->P.l == 3) hipLaunchKernelGGL(br4_kernel<3>, dim3(blocks), dim3(256), 0, s, A, mux_gate, n_jobs);
         else hipLaunchKernelGGL(br4_kernel<2>, dim3(blocks), dim3(256), 0, s, A, mux_gate, n_jobs);
@@ -1103,13 +1404,22 @@ static int upload_key(Ctx& C, const LweBootstrappingKeyFFT* bf, tfhe_amd_device_
     const size_t ksk_bytes = ksk_rows * npad * sizeof(int32_t);
     const int twc = (M == 512) ? TwiddleTable<512>::COUNT : TwiddleTable<1024>::COUNT;
     dk->bytes = bkf_bytes + ksk_bytes + twc * sizeof(c64);
+    dk->bkp = nullptr;
+    dk->twp = nullptr;
     auto fail = [&](int rc) {
         (void)hipFree(dk->bkf);
         (void)hipFree(dk->ksk);
         (void)hipFree(dk->tw);
+        (void)hipFree(dk->bkp);
+        (void)hipFree(dk->twp);
         delete dk;
         return rc;
     };
+    if (M == 1024 && k == 1 && n <= BRP_NMAX) {
+        if (hipMalloc(&dk->bkp, bkf_bytes) != hipSuccess || hipMalloc(&dk->twp, PAIR_TW_COUNT * sizeof(c64)) != hipSuccess)
+            return fail(set_error("hipMalloc(pair-transform key, %zu bytes) failed", bkf_bytes));
+        dk->bytes += bkf_bytes + PAIR_TW_COUNT * sizeof(c64);
+    }
     if (hipMalloc(&dk->bkf, bkf_bytes) != hipSuccess) return fail(set_error("hipMalloc(bootstrapping key, %zu bytes) failed", bkf_bytes));
     if (hipMalloc(&dk->ksk, ksk_bytes) != hipSuccess) return fail(set_error("hipMalloc(keyswitch key, %zu bytes) failed", ksk_bytes));
     if (hipMalloc(&dk->tw, twc * sizeof(c64)) != hipSuccess) return fail(set_error("hipMalloc(twiddles) failed"));
@@ -1139,10 +1449,20 @@ static int upload_key(Ctx& C, const LweBootstrappingKeyFFT* bf, tfhe_amd_device_
     hipError_t e = hipMemcpyAsync(d_coef, coef.data(), coef.size() * sizeof(int32_t), hipMemcpyHostToDevice, C.stream);
     if (e == hipSuccess) e = hipMemcpyAsync(dk->tw, tw.data(), twc * sizeof(c64), hipMemcpyHostToDevice, C.stream);
     if (e == hipSuccess) e = hipMemcpyAsync(dk->ksk, kh.data(), ksk_bytes, hipMemcpyHostToDevice, C.stream);
+    std::vector<c64> twp;
+    if (dk->twp) {
+        twp.resize(PAIR_TW_COUNT);
+        build_twiddles_pair(twp.data());
+        if (e == hipSuccess) e = hipMemcpyAsync(dk->twp, twp.data(), PAIR_TW_COUNT * sizeof(c64), hipMemcpyHostToDevice, C.stream);
+    }
     if (e == hipSuccess) {
         if (M == 512) hipLaunchKernelGGL(bk_convert_kernel<512>, dim3((unsigned)npoly), dim3(64), 0, C.stream, d_coef, dk->bkf, dk->tw);
         else hipLaunchKernelGGL(bk_convert_kernel<1024>, dim3((unsigned)npoly), dim3(64), 0, C.stream, d_coef, dk->bkf, dk->tw);
         e = hipGetLastError();
+        if (e == hipSuccess && dk->bkp) {
+            hipLaunchKernelGGL(bk_convert_pair_kernel, dim3((unsigned)npoly), dim3(128), 0, C.stream, d_coef, dk->bkp, dk->twp);
+            e = hipGetLastError();
+        }
     }
     if (e == hipSuccess) e = hipStreamSynchronize(C.stream);
     (void)hipFree(d_coef);
@@ -1173,6 +1493,8 @@ static void free_key(Ctx& C, tfhe_amd_device_key* dk) {
     (void)hipFree(dk->bkf);
     (void)hipFree(dk->ksk);
     (void)hipFree(dk->tw);
+    (void)hipFree(dk->bkp);
+    (void)hipFree(dk->twp);
     delete dk;
 }
 

@@ -271,6 +271,107 @@ struct Fft {
     }
 };
 
+// exp(i pi m / 32) for any integer m (a compile-time constant after unrolling)
+TA_HD constexpr double cos32(int m) {
+    return (m = ((m % 64) + 64) % 64) <= 16 ? kCos32[m]
+           : m <= 32                         ? -kCos32[32 - m]
+           : m <= 48                         ? -kCos32[m - 32]
+                                             : kCos32[64 - m];
+}
+TA_HD constexpr c64 e32(int m) { return c64{cos32(m), cos32(16 - m)}; }
+
+// Negacyclic transform for N = 2048 on a PAIR of wavefronts. Folding as above
+// gives u_j = p_j + i p_{j+1024} (j < 1024) and the wanted spectrum is
+// X_m = sum_j u_j psi^j w^{jm} (psi = exp(i pi/2048), w = exp(2 pi i/1024)).
+// Wavefront h of the pair holds u_j for j = 512h + 64a + b (lane b, slot a).
+// One radix-2 step across the pair (through LDS) forms
+//   y_h[j'] = u_{j'} + (-1)^h e^{i pi/4} u_{j'+512},       j' = 64a + b < 512,
+// and then X_{2k+h} = sum_{j'} y_h[j'] psi^{(1+4h) j'} w_512^{j'k}: a 512-point
+// DFT of y_h with the twist psi^{(1+4h) j'}, run by wavefront h alone with the
+// Fft<512> stages (the twist split into the compile-time part psi^{(1+4h) 64a}
+// and the per-lane part psi^{(1+4h) b}, which is folded into tw1). So
+// wavefront h ends with the odd (h = 1) or even (h = 0) bins. The inverse runs
+// the steps backwards; the pair step's 1/2 is folded into the final scale.
+struct FftPair {
+    using F = Fft<512>;
+    static constexpr int R = 8;
+    static constexpr double SQ = kH;  // e^{i pi/4} = (SQ, SQ)
+
+    // y from own u (u) and the partner's u (o)
+    template <int H>
+    TA_HD static void pair_fwd(c64* x, const c64* o) {
+#pragma unroll
+        for (int a = 0; a < R; a++) {
+            const c64 u = x[a], v = o[a];
+            if (H == 0) {  // u_{j'} + e^{i pi/4} u_{j'+512}; v = u_{j'+512}
+                x[a] = c64{u.re + SQ * (v.re - v.im), u.im + SQ * (v.re + v.im)};
+            } else {  // u_{j'} - e^{i pi/4} u_{j'+512}; u = u_{j'+512}, v = u_{j'}
+                x[a] = c64{v.re - SQ * (u.re - u.im), v.im - SQ * (u.re + u.im)};
+            }
+        }
+    }
+    // stage 1 of wavefront h: twist, 8-point DFT, per-lane twiddle, transpose write
+    template <int H, int TS = 1>
+    TA_HD static void fwd1(c64* x, int b, c64* lds, const c64* tw1) {
+#pragma unroll
+        for (int a = 1; a < R; a++) x[a] = cmul(x[a], e32((1 + 4 * H) * a));
+        dif<R, 1>(x);
+#pragma unroll
+        for (int j = 0; j < R; j++) {
+            const int k1 = bitrev(j, 3);
+            lds[F::a1(k1, b)] = cmul(x[j], tw1[k1 * TS]);
+        }
+    }
+    // last inverse stage of wavefront h: y_h / 2 (scaled by 1/1024)
+    template <int H, int TS = 1>
+    TA_HD static void inv4(c64* x, int b, const c64* lds, const c64* tw1) {
+#pragma unroll
+        for (int j = 0; j < R; j++) {
+            const int k1 = bitrev(j, 3);
+            x[j] = cmulc(lds[F::a1(k1, b)], tw1[k1 * TS]);
+        }
+        dit<R, -1>(x);
+#pragma unroll
+        for (int a = 0; a < R; a++) {
+            const c64 t = e32((1 + 4 * H) * a);
+            x[a] = cmul(x[a], c64{t.re / 1024.0, -t.im / 1024.0});
+        }
+    }
+    // u from own y/2 and the partner's y/2: u_{j'} = y0 + y1 (h = 0),
+    // u_{j'+512} = (y0 - y1) e^{-i pi/4} (h = 1)
+    template <int H>
+    TA_HD static void pair_inv(c64* x, const c64* o) {
+#pragma unroll
+        for (int a = 0; a < R; a++) {
+            const c64 y = x[a], v = o[a];
+            if (H == 0) {
+                x[a] = cadd(y, v);
+            } else {
+                const c64 d = csub(v, y);  // y0 - y1
+                x[a] = c64{SQ * (d.re + d.im), SQ * (d.im - d.re)};
+            }
+        }
+    }
+};
+
+// Twiddles of the pair transform: [h][lane][16]: tw1_h[k1] = psi^{(1+4h) b} w_512^{b k1}
+// (psi = exp(i pi/2048)), then the 8 stage-2a twiddles w_64^{(b&7) k2a}.
+constexpr int PAIR_TW_COUNT = 2 * 64 * 16;
+inline void build_twiddles_pair(c64* out) {
+    const long double pi = 3.141592653589793238462643383279502884L;
+    auto ex = [&](long double ang) { return c64{(double)cosl(ang), (double)sinl(ang)}; };
+    for (int h = 0; h < 2; h++)
+        for (int b = 0; b < 64; b++) {
+            c64* t = out + (h * 64 + b) * 16;
+            for (int k1 = 0; k1 < 8; k1++) {
+                // exp(i pi ((1+4h) b + 8 b k1) / 2048)
+                const long e = ((long)(1 + 4 * h) * b + 8L * b * k1) % 4096;
+                t[k1] = ex(pi * (long double)e / 2048.0L);
+            }
+            for (int k2a = 0; k2a < 8; k2a++) t[8 + k2a] = ex(2.0L * pi * (long double)(((b & 7) * k2a) % 64) / 64.0L);
+        }
+}
+
 // Per-lane twiddle tables, built on the host in long double and rounded once.
 // Layout: for each lane, R entries of tw1 then 8 entries of tw2; then the R
 // uniform twist entries and R inverse-twist entries.
