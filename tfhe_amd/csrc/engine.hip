@@ -164,7 +164,6 @@ __global__ __launch_bounds__(64, 2) void br_kernel(BrArgs A) {
     const c64 tw2 = A.tw[lane * (R + 8) + R + 1];
     const uint32_t offset = A.offset;
     const int Bgbit = A.Bgbit;
-    const uint32_t dmask = (1u << Bgbit) - 1u;
     const int32_t dhalf = 1 << (Bgbit - 1);
 
     // Key rows of the current digit polynomial, loaded one digit ahead of use
@@ -208,8 +207,8 @@ __global__ __launch_bounds__(64, 2) void br_kernel(BrArgs A) {
             c64 x[R];
 #pragma unroll
             for (int s = 0; s < R; s++) {
-                const int32_t d0 = (int32_t)((tv[s] >> shift) & dmask) - dhalf;
-                const int32_t d1 = (int32_t)((tv[R + s] >> shift) & dmask) - dhalf;
+                const int32_t d0 = (int32_t)__builtin_amdgcn_ubfe(tv[s], shift, Bgbit) - dhalf;
+                const int32_t d1 = (int32_t)__builtin_amdgcn_ubfe(tv[R + s], shift, Bgbit) - dhalf;
                 x[s] = c64{(double)d0, (double)d1};
             }
             F::fwd1(x, lane, s_fft, tw1);
@@ -369,7 +368,6 @@ __global__ __launch_bounds__(256, 2) void br4_kernel(BrArgs A, const int32_t* __
     const c64 tw2 = A.tw[lane * (R + 8) + R + 1];
     const uint32_t offset = A.offset;
     const int Bgbit = A.Bgbit;
-    const uint32_t dmask = (1u << Bgbit) - 1u;
     const int32_t dhalf = 1 << (Bgbit - 1);
 
     __syncthreads();
@@ -403,24 +401,26 @@ __global__ __launch_bounds__(256, 2) void br4_kernel(BrArgs A, const int32_t* __
 #pragma unroll
                         for (int s = 0; s < R; s++) fbi[64 * s + lane + h * M] = (int32_t)(c ? acc[1][h * R + s] : acc[0][h * R + s]);
                     wave_sync();
+                    // all 16 rotated reads in flight, then the differences
 #pragma unroll
-                    for (int h = 0; h < 2; h++)
+                    for (int q = 0; q < 2 * R; q++) {
+                        const int j = 64 * (q % R) + lane + (q / R) * M;
+                        tv[q] = (uint32_t)fbi[(j - abar) & (N - 1)];
+                    }
 #pragma unroll
-                        for (int s = 0; s < R; s++) {
-                            const int j = 64 * s + lane + h * M;
-                            const int idx = (j - abar) & (2 * N - 1);
-                            const uint32_t v = (uint32_t)fbi[idx & (N - 1)];
-                            const uint32_t rot = (idx & N) ? 0u - v : v;
-                            tv[h * R + s] = rot - (c ? acc[1][h * R + s] : acc[0][h * R + s]) + offset;
-                        }
+                    for (int q = 0; q < 2 * R; q++) {
+                        const int j = 64 * (q % R) + lane + (q / R) * M;
+                        const uint32_t rot = ((j - abar) & N) ? 0u - tv[q] : tv[q];
+                        tv[q] = rot - (c ? acc[1][q] : acc[0][q]) + offset;
+                    }
                     wave_sync();
                 }
                 // digit p of the truncating gadget decomposition, tgsw-functions.cpp:337-348
                 const int shift = 32 - (p + 1) * Bgbit;
 #pragma unroll
                 for (int s = 0; s < R; s++) {
-                    const int32_t d0 = (int32_t)((tv[s] >> shift) & dmask) - dhalf;
-                    const int32_t d1 = (int32_t)((tv[R + s] >> shift) & dmask) - dhalf;
+                    const int32_t d0 = (int32_t)__builtin_amdgcn_ubfe(tv[s], shift, Bgbit) - dhalf;
+                    const int32_t d1 = (int32_t)__builtin_amdgcn_ubfe(tv[R + s], shift, Bgbit) - dhalf;
                     x[s] = c64{(double)d0, (double)d1};
                 }
                 F::template fwd1<64>(x, lane, fb, tw1);
@@ -433,11 +433,16 @@ __global__ __launch_bounds__(256, 2) void br4_kernel(BrArgs A, const int32_t* __
             }
             __syncthreads();  // s_bk holds digit (i, r) for every wavefront
             if (active) {
+                // eight key values in flight at a time, then their multiply-accumulates
+                c64 kv[R];
 #pragma unroll
-                for (int s = 0; s < R; s++) {
-                    cmac(r0[s], x[s], s_bk[s * 64 + lane]);
-                    cmac(r1[s], x[s], s_bk[M + s * 64 + lane]);
-                }
+                for (int s = 0; s < R; s++) kv[s] = s_bk[s * 64 + lane];
+#pragma unroll
+                for (int s = 0; s < R; s++) cmac(r0[s], x[s], kv[s]);
+#pragma unroll
+                for (int s = 0; s < R; s++) kv[s] = s_bk[M + s * 64 + lane];
+#pragma unroll
+                for (int s = 0; s < R; s++) cmac(r1[s], x[s], kv[s]);
             }
         }
         if (active) {
@@ -555,7 +560,6 @@ __global__ __launch_bounds__(256, 2) void br2_kernel(BrArgs A, const int32_t* __
     const c64 tw2 = A.tw[lane * (R + 8) + R + 1];
     const uint32_t offset = A.offset;
     const int Bgbit = A.Bgbit;
-    const uint32_t dmask = (1u << Bgbit) - 1u;
     const int32_t dhalf = 1 << (Bgbit - 1);
     __syncthreads();
 
@@ -607,8 +611,8 @@ __global__ __launch_bounds__(256, 2) void br2_kernel(BrArgs A, const int32_t* __
                 const int shift = 32 - (p + 1) * Bgbit;
 #pragma unroll
                 for (int s = 0; s < R; s++) {
-                    const int32_t d0v = (int32_t)((tv[s] >> shift) & dmask) - dhalf;
-                    const int32_t d1v = (int32_t)((tv[R + s] >> shift) & dmask) - dhalf;
+                    const int32_t d0v = (int32_t)__builtin_amdgcn_ubfe(tv[s], shift, Bgbit) - dhalf;
+                    const int32_t d1v = (int32_t)__builtin_amdgcn_ubfe(tv[R + s], shift, Bgbit) - dhalf;
                     x[s] = c64{(double)d0v, (double)d1v};
                 }
                 F::template fwd1<64>(x, lane, fb, tw1);
@@ -758,7 +762,6 @@ __global__ __launch_bounds__(512, 1) void brp_kernel(BrArgs A, const c64* __rest
     const c64 tw2 = twp[(h * 64 + lane) * 16 + 9];
     const uint32_t offset = A.offset;
     const int Bgbit = A.Bgbit;
-    const uint32_t dmask = (1u << Bgbit) - 1u;
     const int32_t dhalf = 1 << (Bgbit - 1);
     __syncthreads();
 
@@ -832,8 +835,8 @@ __global__ __launch_bounds__(512, 1) void brp_kernel(BrArgs A, const c64* __rest
                 const int shift = 32 - (p + 1) * Bgbit;
 #pragma unroll
                 for (int a = 0; a < R; a++) {
-                    const int32_t d0v = (int32_t)((tv[a] >> shift) & dmask) - dhalf;
-                    const int32_t d1v = (int32_t)((tv[R + a] >> shift) & dmask) - dhalf;
+                    const int32_t d0v = (int32_t)__builtin_amdgcn_ubfe(tv[a], shift, Bgbit) - dhalf;
+                    const int32_t d1v = (int32_t)__builtin_amdgcn_ubfe(tv[R + a], shift, Bgbit) - dhalf;
                     x[a] = c64{(double)d0v, (double)d1v};
                     fb[a * 64 + lane] = x[a];
                 }

@@ -171,13 +171,17 @@ struct Fft {
     // tw1[k1 * TS] = psi^b w_M^{b k1} (TS = 1: per-lane array; TS = 64: shared table column)
     template <int TS = 1>
     TA_HD static void fwd1(c64* x, int b, c64* lds, const c64* tw1) {
+        // twiddles requested before the in-register DFT so their loads overlap it
+        c64 t[R];
+#pragma unroll
+        for (int k = 0; k < R; k++) t[k] = tw1[k * TS];
 #pragma unroll
         for (int a = 1; a < R; a++) x[a] = cmul(x[a], twist(a));
         dif<R, 1>(x);
 #pragma unroll
         for (int j = 0; j < R; j++) {
             const int k1 = bitrev(j, LOGR);
-            lds[a1(k1, b)] = cmul(x[j], tw1[k1 * TS]);
+            lds[a1(k1, b)] = cmul(x[j], t[k1]);
         }
     }
     // Powers w^0..w^7 of w = w_64^{b0}, b0 = lane & 7 (the stage-2a twiddles),
