@@ -18,6 +18,7 @@
 
 #include <stddef.h>
 #include <stdint.h>
+#include <stdio.h>
 
 #ifdef __cplusplus
 #define TFHE_AMD_EXPORT extern "C" __attribute__((visibility("default")))
@@ -267,6 +268,34 @@ TFHE_AMD_EXPORT int32_t tfhe_amd_netlist_levels(const tfhe_amd_netlist* nl, int3
 /* Asynchronous on stream (NULL = the library's stream). */
 TFHE_AMD_EXPORT int32_t tfhe_amd_netlist_run(tfhe_amd_netlist* nl, int32_t* d_pool, void* stream, int32_t use_graph);
 TFHE_AMD_EXPORT void tfhe_amd_netlist_destroy(tfhe_amd_netlist* nl);
+
+/* ---- Key, parameter and ciphertext files: include/tfhe_io.h:30-97, 379-485 --
+ * The reference's wire format (libtfhe/tfhe_io.cpp): text parameter
+ * sections plus native-order binary key material, so files written by either
+ * library load in the other. Readers return NULL on malformed input (reason in
+ * tfhe_amd_last_error); writers abort on bad arguments, as the reference. The
+ * std::ostream / std::istream variants of the reference are C++-only and not
+ * part of this C ABI. */
+TFHE_AMD_EXPORT void export_tfheGateBootstrappingParameterSet_toFile(FILE* F,
+                                                                     const TFheGateBootstrappingParameterSet* params);
+TFHE_AMD_EXPORT TFheGateBootstrappingParameterSet* new_tfheGateBootstrappingParameterSet_fromFile(FILE* F);
+TFHE_AMD_EXPORT void export_tfheGateBootstrappingCloudKeySet_toFile(FILE* F, const TFheGateBootstrappingCloudKeySet* key);
+TFHE_AMD_EXPORT TFheGateBootstrappingCloudKeySet* new_tfheGateBootstrappingCloudKeySet_fromFile(FILE* F);
+TFHE_AMD_EXPORT void export_tfheGateBootstrappingSecretKeySet_toFile(FILE* F,
+                                                                     const TFheGateBootstrappingSecretKeySet* key);
+TFHE_AMD_EXPORT TFheGateBootstrappingSecretKeySet* new_tfheGateBootstrappingSecretKeySet_fromFile(FILE* F);
+TFHE_AMD_EXPORT void export_gate_bootstrapping_ciphertext_toFile(FILE* F, const LweSample* sample,
+                                                                 const TFheGateBootstrappingParameterSet* params);
+TFHE_AMD_EXPORT void import_gate_bootstrapping_ciphertext_fromFile(FILE* F, LweSample* sample,
+                                                                   const TFheGateBootstrappingParameterSet* params);
+TFHE_AMD_EXPORT void export_lweSample_toFile(FILE* F, const LweSample* sample, const LweParams* params);
+TFHE_AMD_EXPORT void import_lweSample_fromFile(FILE* F, LweSample* sample, const LweParams* params);
+TFHE_AMD_EXPORT void export_lweParams_toFile(FILE* F, const LweParams* params);
+TFHE_AMD_EXPORT LweParams* new_lweParams_fromFile(FILE* F);
+TFHE_AMD_EXPORT void export_lweKey_toFile(FILE* F, const LweKey* key);
+/* The returned key owns its parameters: free it with delete_LweKey (include/lwekey.h:18-19). */
+TFHE_AMD_EXPORT LweKey* new_lweKey_fromFile(FILE* F);
+TFHE_AMD_EXPORT void delete_LweKey(LweKey* key);
 
 /* Client-side batch encryption / decryption on the device, over contiguous
  * ciphertext rows [count][n+1]. decrypt: bits[g] = (phase > 0) as

@@ -13,6 +13,11 @@
 //   gates SEED COUNT OPS OUTFILE  encrypt random bits (global RNG, in order) and run gates
 //   ks    SEED COUNT OUTFILE    keyswitch random extracted-dimension samples
 //   bench SEED COUNT THREADS    time bootsNAND on fresh encryptions; prints one JSON line
+//   files SEED DIR COUNT        write params.gb, secret.key, cloud.key, lwekey.bin, COUNT
+//                               ciphertexts cts.bin (bits in bits.txt) and nand.bin = NAND(ct0, ct1)
+//                               with the reference's file functions (tfhe_io.cpp)
+//   readfiles SEED DIR COUNT    read those files with the reference's readers; print the
+//                               decrypted bits, NAND(ct0, ct1) decrypted, and key hashes
 #include <tfhe.h>
 #include <tfhe_io.h>
 #include <lwesamples.h>
@@ -102,6 +107,18 @@ static void put_sample(FILE* f, const LweSample* s, int n) {
     write_all(f, &s->b, sizeof(int32_t));
 }
 
+static uint64_t fnv(const void* p, size_t bytes, uint64_t h = 1469598103934665603ULL) {
+    const unsigned char* c = (const unsigned char*)p;
+    for (size_t i = 0; i < bytes; i++) h = (h ^ c[i]) * 1099511628211ULL;
+    return h;
+}
+
+static FILE* open_or_die(const std::string& path, const char* mode) {
+    FILE* f = fopen(path.c_str(), mode);
+    if (!f) { perror(path.c_str()); exit(2); }
+    return f;
+}
+
 int main(int argc, char** argv) {
     if (argc < 3) {
         fprintf(stderr, "usage: ref_harness keys|gates|ks|bench SEED ...\n");
@@ -116,6 +133,83 @@ int main(int argc, char** argv) {
     // library's global generator is consumed only by key generation and encryption
     std::mt19937_64 bitgen(0x5eed0000ULL + seed);
 
+    if (cmd == "files" && argc >= 5) {
+        const std::string dir = argv[3];
+        const int count = atoi(argv[4]);
+        FILE* f = open_or_die(dir + "/params.gb", "wb");
+        export_tfheGateBootstrappingParameterSet_toFile(f, P);
+        fclose(f);
+        f = open_or_die(dir + "/secret.key", "wb");
+        export_tfheGateBootstrappingSecretKeySet_toFile(f, key);
+        fclose(f);
+        f = open_or_die(dir + "/cloud.key", "wb");
+        export_tfheGateBootstrappingCloudKeySet_toFile(f, &key->cloud);
+        fclose(f);
+        f = open_or_die(dir + "/lwekey.bin", "wb");
+        export_lweKey_toFile(f, key->lwe_key);
+        fclose(f);
+        LweSample* ct = new_gate_bootstrapping_ciphertext_array(count, P);
+        f = open_or_die(dir + "/cts.bin", "wb");
+        FILE* fb = open_or_die(dir + "/bits.txt", "w");
+        for (int g = 0; g < count; g++) {
+            const int32_t bit = (int32_t)(bitgen() & 1);
+            bootsSymEncrypt(&ct[g], bit, key);
+            export_gate_bootstrapping_ciphertext_toFile(f, &ct[g], P);
+            fprintf(fb, "%d", bit);
+        }
+        fclose(f);
+        fclose(fb);
+        LweSample* r = new_gate_bootstrapping_ciphertext(P);
+        bootsNAND(r, &ct[0], &ct[1], &key->cloud);
+        f = open_or_die(dir + "/nand.bin", "wb");
+        export_gate_bootstrapping_ciphertext_toFile(f, r, P);
+        fclose(f);
+        return 0;
+    }
+    if (cmd == "readfiles" && argc >= 5) {
+        const std::string dir = argv[3];
+        const int count = atoi(argv[4]);
+        FILE* f = open_or_die(dir + "/params.gb", "rb");
+        TFheGateBootstrappingParameterSet* p2 = new_tfheGateBootstrappingParameterSet_fromFile(f);
+        fclose(f);
+        f = open_or_die(dir + "/secret.key", "rb");
+        TFheGateBootstrappingSecretKeySet* sk = new_tfheGateBootstrappingSecretKeySet_fromFile(f);
+        fclose(f);
+        f = open_or_die(dir + "/cloud.key", "rb");
+        TFheGateBootstrappingCloudKeySet* ck = new_tfheGateBootstrappingCloudKeySet_fromFile(f);
+        fclose(f);
+        LweSample* ct = new_gate_bootstrapping_ciphertext_array(count, p2);
+        f = open_or_die(dir + "/cts.bin", "rb");
+        printf("bits ");
+        for (int g = 0; g < count; g++) {
+            import_gate_bootstrapping_ciphertext_fromFile(f, &ct[g], p2);
+            printf("%d", bootsSymDecrypt(&ct[g], sk));
+        }
+        fclose(f);
+        LweSample* r = new_gate_bootstrapping_ciphertext(p2);
+        bootsNAND(r, &ct[0], &ct[1], ck);
+        printf("\nnand %d\n", bootsSymDecrypt(r, sk));
+        printf("params %d %d %d %d %d %d %d %.10g %.10g %.10g %.10g\n", p2->in_out_params->n,
+               p2->tgsw_params->tlwe_params->N, p2->tgsw_params->tlwe_params->k, p2->tgsw_params->l,
+               p2->tgsw_params->Bgbit, p2->ks_t, p2->ks_basebit, p2->in_out_params->alpha_min,
+               p2->in_out_params->alpha_max, p2->tgsw_params->tlwe_params->alpha_min,
+               p2->tgsw_params->tlwe_params->alpha_max);
+        // key hashes over the cloud key read from cloud.key
+        const LweKeySwitchKey* ks = ck->bk->ks;
+        const int n2 = p2->in_out_params->n;
+        uint64_t hk = 1469598103934665603ULL;
+        for (int i = 0; i < ks->n * ks->t * ks->base; i++) {
+            hk = fnv(ks->ks0_raw[i].a, sizeof(int32_t) * n2, hk);
+            hk = fnv(&ks->ks0_raw[i].b, sizeof(int32_t), hk);
+        }
+        uint64_t hb = 1469598103934665603ULL;
+        const int kpl = p2->tgsw_params->kpl, N2 = p2->tgsw_params->tlwe_params->N, k2 = p2->tgsw_params->tlwe_params->k;
+        for (int i = 0; i < n2; i++)
+            for (int j = 0; j < kpl; j++)
+                for (int c = 0; c <= k2; c++) hb = fnv(ck->bk->bk[i].all_sample[j].a[c].coefsT, sizeof(int32_t) * N2, hb);
+        printf("ks_hash %llu\nbk_hash %llu\n", (unsigned long long)hk, (unsigned long long)hb);
+        return 0;
+    }
     if (cmd == "keys" && argc >= 4) {
         dump_keys(key, argv[3]);
         return 0;

@@ -4,8 +4,9 @@
 Needs oracle/_ref/ref_harness (make -C oracle ref, from the reference's own
 sources); run in a container where those sources exist. The vectors are data
 only: hashes of the reference's seed-0 keys, a few reference keyswitch
-input/output pairs, and the reference's output phases for gates on
-encryptions that the oracle replays from the same seed.
+input/output pairs, the reference's output phases for gates on encryptions
+that the oracle replays from the same seed, and the small files of the
+reference's key-file format (parameters, LWE key, ciphertexts) for seed 11.
 
     python tests/golden/make_golden.py
 """
@@ -64,6 +65,15 @@ def main():
                                 ops=G["ops"].astype(np.int8), bits=G["bits"].astype(np.int8),
                                 phase=G["phase"], inputs_hash=np.uint64(O.fnv1a(inputs)),
                                 ref_ns=G["ns"])
+        # key-file format: the small files the reference writes for seed 11
+        # (parameters, LWE key, 16 ciphertexts of recorded bits, NAND of the
+        # first two); the key sets themselves are too large to keep
+        io_dir = os.path.join(HERE, "ref_io_p128_seed11")
+        os.makedirs(io_dir, exist_ok=True)
+        run("files", 11, tmp, 16)
+        for f in ("params.gb", "lwekey.bin", "cts.bin", "nand.bin", "bits.txt"):
+            with open(os.path.join(tmp, f), "rb") as src, open(os.path.join(io_dir, f), "wb") as dst:
+                dst.write(src.read())
     for f in sorted(os.listdir(HERE)):
         print(f, os.path.getsize(os.path.join(HERE, f)))
 

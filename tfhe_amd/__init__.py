@@ -134,6 +134,17 @@ _SIGS = {
     "tfhe_amd_gates_device": (i32, [CKP, i32, vp, vp, vp, vp, vp, vp, vp]),
     "tfhe_amd_gates_device_woks": (i32, [CKP, i32, vp, vp, vp, vp, vp, vp, vp]),
     "tfhe_amd_keyswitch_device": (i32, [CKP, i32, vp, vp, vp]),
+    "export_tfheGateBootstrappingParameterSet_toFile": (None, [vp, PP]),
+    "new_tfheGateBootstrappingParameterSet_fromFile": (PP, [vp]),
+    "export_tfheGateBootstrappingCloudKeySet_toFile": (None, [vp, CKP]),
+    "new_tfheGateBootstrappingCloudKeySet_fromFile": (CKP, [vp]),
+    "export_tfheGateBootstrappingSecretKeySet_toFile": (None, [vp, SKP]),
+    "new_tfheGateBootstrappingSecretKeySet_fromFile": (SKP, [vp]),
+    "export_gate_bootstrapping_ciphertext_toFile": (None, [vp, LSP, PP]),
+    "import_gate_bootstrapping_ciphertext_fromFile": (None, [vp, LSP, PP]),
+    "export_lweKey_toFile": (None, [vp, vp]),
+    "new_lweKey_fromFile": (vp, [vp]),
+    "delete_LweKey": (None, [vp]),
     "tfhe_amd_encrypt_device": (i32, [SKP, i32, vp, vp, ctypes.c_uint64, ctypes.c_uint64, vp]),
     "tfhe_amd_decrypt_device": (i32, [SKP, i32, vp, vp, vp, vp]),
     "tfhe_amd_set_launch_config": (None, [i32, i32]),
@@ -215,12 +226,27 @@ def big_params():
 class SecretKey:
     """Wraps a TFheGateBootstrappingSecretKeySet (host keys; device key uploaded on first use)."""
 
-    def __init__(self, params, seed=None):
-        if seed is not None:
-            set_seed(seed)
+    def __init__(self, params, seed=None, ptr=None):
+        if ptr is None:
+            if seed is not None:
+                set_seed(seed)
+            ptr = lib().new_random_gate_bootstrapping_secret_keyset(params.ptr)
         self.params = params
-        self.ptr = lib().new_random_gate_bootstrapping_secret_keyset(params.ptr)
+        self.ptr = ptr
         self.cloud = lib().tfhe_amd_cloud_of(self.ptr)
+
+    @classmethod
+    def load(cls, path):
+        """Secret key set from a key file in the reference's format."""
+        with _CFile(path, "rb") as f:
+            ptr = lib().new_tfheGateBootstrappingSecretKeySet_fromFile(f)
+        if not ptr:
+            raise TfheAmdError(lib().tfhe_amd_last_error().decode())
+        return cls(Params(ptr.contents.params), ptr=ptr)
+
+    def save(self, path):
+        with _CFile(path, "wb") as f:
+            lib().export_tfheGateBootstrappingSecretKeySet_toFile(f, self.ptr)
 
     def encrypt(self, bits):
         bits = np.ascontiguousarray(bits, np.int32)
@@ -253,6 +279,89 @@ class SecretKey:
         if self.ptr:
             lib().delete_gate_bootstrapping_secret_keyset(self.ptr)
             self.ptr = None
+
+
+class _CFile:
+    """A C FILE* for the key-file functions (opened with the C library's fopen)."""
+
+    _libc = None
+
+    def __init__(self, path, mode):
+        if _CFile._libc is None:
+            c = ctypes.CDLL(None)
+            c.fopen.restype, c.fopen.argtypes = vp, [ctypes.c_char_p, ctypes.c_char_p]
+            c.fclose.restype, c.fclose.argtypes = ctypes.c_int, [vp]
+            _CFile._libc = c
+        self.f = _CFile._libc.fopen(os.fsencode(path), mode.encode())
+        if not self.f:
+            raise OSError(f"cannot open {path}")
+
+    def __enter__(self):
+        return self.f
+
+    def __exit__(self, *exc):
+        _CFile._libc.fclose(self.f)
+
+
+def save_params(params, path):
+    with _CFile(path, "wb") as f:
+        lib().export_tfheGateBootstrappingParameterSet_toFile(f, params.ptr)
+
+
+def load_params(path):
+    with _CFile(path, "rb") as f:
+        ptr = lib().new_tfheGateBootstrappingParameterSet_fromFile(f)
+    if not ptr:
+        raise TfheAmdError(lib().tfhe_amd_last_error().decode())
+    return Params(ptr)
+
+
+def save_cloud_key(cloud, path):
+    with _CFile(path, "wb") as f:
+        lib().export_tfheGateBootstrappingCloudKeySet_toFile(f, cloud)
+
+
+def load_cloud_key(path):
+    with _CFile(path, "rb") as f:
+        ck = lib().new_tfheGateBootstrappingCloudKeySet_fromFile(f)
+    if not ck:
+        raise TfheAmdError(lib().tfhe_amd_last_error().decode())
+    return ck
+
+
+def save_ciphertexts(params, cts, path, variances=None):
+    """Ciphertext rows [m][n+1] as consecutive gate-bootstrapping ciphertext
+    records (export_gate_bootstrapping_ciphertext_toFile)."""
+    cts = np.ascontiguousarray(cts, np.int32)
+    n = params.n
+    s = lib().new_gate_bootstrapping_ciphertext(params.ptr)
+    try:
+        with _CFile(path, "wb") as f:
+            for g in range(len(cts)):
+                ctypes.memmove(s.contents.a, cts[g].ctypes.data, 4 * n)
+                s.contents.b = int(cts[g, n])
+                s.contents.current_variance = 0.0 if variances is None else float(variances[g])
+                lib().export_gate_bootstrapping_ciphertext_toFile(f, s, params.ptr)
+    finally:
+        lib().delete_gate_bootstrapping_ciphertext(s)
+
+
+def load_ciphertexts(params, path, count):
+    """(rows [count][n+1], variances) read with import_gate_bootstrapping_ciphertext_fromFile."""
+    n = params.n
+    out = np.zeros((count, n + 1), np.int32)
+    var = np.zeros(count)
+    s = lib().new_gate_bootstrapping_ciphertext(params.ptr)
+    try:
+        with _CFile(path, "rb") as f:
+            for g in range(count):
+                lib().import_gate_bootstrapping_ciphertext_fromFile(f, s, params.ptr)
+                ctypes.memmove(out[g].ctypes.data, s.contents.a, 4 * n)
+                out[g, n] = s.contents.b
+                var[g] = s.contents.current_variance
+    finally:
+        lib().delete_gate_bootstrapping_ciphertext(s)
+    return out, var
 
 
 def cloud_from_arrays(params, bk, ksk):

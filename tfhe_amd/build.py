@@ -38,6 +38,7 @@ def build(verbose_resources=False):
     eng_o = os.path.join(BUILD, "engine.o")
     net_o = os.path.join(BUILD, "netlist.o")
     host_o = os.path.join(BUILD, "host.o")
+    io_o = os.path.join(BUILD, "io.o")
     srv_o = os.path.join(BUILD, "serve.o")
     cli_o = os.path.join(BUILD, "client.o")
     extra = ["-Rpass-analysis=kernel-resource-usage"] if verbose_resources else []
@@ -46,8 +47,9 @@ def build(verbose_resources=False):
     _run([HIPCC] + HIP_FLAGS + ["-c", os.path.join(CSRC, "serve.hip"), "-o", srv_o])
     _run([HIPCC] + HIP_FLAGS + ["-c", os.path.join(CSRC, "client.hip"), "-o", cli_o])
     _run([CXX] + HOST_FLAGS + ["-c", os.path.join(CSRC, "host.cpp"), "-o", host_o])
+    _run([CXX] + HOST_FLAGS + ["-c", os.path.join(CSRC, "io.cpp"), "-o", io_o])
     tmp = OUT + ".tmp"
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, eng_o, net_o, srv_o, cli_o, host_o,
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, eng_o, net_o, srv_o, cli_o, host_o, io_o,
           "-Wl,-Bsymbolic", "-Wl,--no-undefined", "-lpthread"])
     os.replace(tmp, OUT)
     return OUT

@@ -58,6 +58,19 @@ int engine_param_n(const tfhe_amd_device_key* dk);
 bool engine_same_params(const tfhe_amd_device_key* a, const tfhe_amd_device_key* b);
 size_t engine_key_bytes(const tfhe_amd_device_key* dk);
 
+// Host key-set constructors shared by the gate API and the key-file readers
+// (host.cpp). Arrays: bootstrapping key [n][kpl][k+1][N], keyswitch key
+// [k*N][t][base][n+1] (a, then b), every sample given the stated variance.
+TFheGateBootstrappingParameterSet* host_make_params(int32_t n, double lwe_alpha_min, double lwe_alpha_max, int32_t N,
+                                                    int32_t k, double tlwe_alpha_min, double tlwe_alpha_max,
+                                                    int32_t bk_l, int32_t bk_Bgbit, int32_t ks_t, int32_t ks_basebit);
+TFheGateBootstrappingCloudKeySet* host_cloud_from_arrays(const TFheGateBootstrappingParameterSet* P, const int32_t* bkc,
+                                                         const int32_t* ksk, double ks_var, double bk_var);
+TFheGateBootstrappingSecretKeySet* host_secret_from_arrays(const TFheGateBootstrappingParameterSet* P,
+                                                           const int32_t* bkc, const int32_t* ksk, double ks_var,
+                                                           double bk_var, const int32_t* lwe_key,
+                                                           const int32_t* tlwe_key);
+
 // Levelisation of a gate netlist (schedule 0 = as soon as possible, 1 = as
 // late as possible): gate order sorted by level and the gates per level.
 // Validates gate codes, wire ranges, single drivers and acyclicity.

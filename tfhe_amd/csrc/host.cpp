@@ -300,20 +300,28 @@ TFHE_AMD_EXPORT void delete_TGswParams(TGswParams* obj) {
     delete obj;
 }
 
-TFHE_AMD_EXPORT TFheGateBootstrappingParameterSet* new_gate_bootstrapping_parameters(
-    int32_t n, int32_t N, int32_t k, int32_t bk_l, int32_t bk_Bgbit, int32_t ks_t, int32_t ks_basebit,
-    double ks_stdev, double bk_stdev, double max_stdev) {
+namespace tfhe_amd {
+TFheGateBootstrappingParameterSet* host_make_params(int32_t n, double lwe_alpha_min, double lwe_alpha_max, int32_t N,
+                                                    int32_t k, double tlwe_alpha_min, double tlwe_alpha_max,
+                                                    int32_t bk_l, int32_t bk_Bgbit, int32_t ks_t, int32_t ks_basebit) {
     if (n <= 0 || N <= 0 || (N & (N - 1)) || k <= 0 || bk_l <= 0 || bk_Bgbit <= 0 || bk_l * bk_Bgbit > 32 ||
         ks_t <= 0 || ks_basebit <= 0 || ks_t * ks_basebit > 31) {
         set_error("invalid gate bootstrapping parameters");
         return nullptr;
     }
     std::lock_guard<std::mutex> g(g_param_mu);
-    g_lwe_params.emplace_back(new_LweParams(n, ks_stdev, max_stdev));
-    g_tlwe_params.emplace_back(new_TLweParams(N, k, bk_stdev, max_stdev));
+    g_lwe_params.emplace_back(new_LweParams(n, lwe_alpha_min, lwe_alpha_max));
+    g_tlwe_params.emplace_back(new_TLweParams(N, k, tlwe_alpha_min, tlwe_alpha_max));
     TGswParams* gp = new_TGswParams(bk_l, bk_Bgbit, g_tlwe_params.back().get());
     g_tgsw_params.push_back(gp);
     return new TFheGateBootstrappingParameterSet{ks_t, ks_basebit, g_lwe_params.back().get(), gp};
+}
+}  // namespace tfhe_amd
+
+TFHE_AMD_EXPORT TFheGateBootstrappingParameterSet* new_gate_bootstrapping_parameters(
+    int32_t n, int32_t N, int32_t k, int32_t bk_l, int32_t bk_Bgbit, int32_t ks_t, int32_t ks_basebit,
+    double ks_stdev, double bk_stdev, double max_stdev) {
+    return host_make_params(n, ks_stdev, max_stdev, N, k, bk_stdev, max_stdev, bk_l, bk_Bgbit, ks_t, ks_basebit);
 }
 
 // presets, libtfhe/tfhe_gate_bootstrapping.cpp:23-90
@@ -488,12 +496,8 @@ TFHE_AMD_EXPORT TFheGateBootstrappingSecretKeySet* new_random_gate_bootstrapping
     return &so->pub;
 }
 
-TFHE_AMD_EXPORT TFheGateBootstrappingCloudKeySet* new_gate_bootstrapping_cloud_keyset_from_arrays(
-    const TFheGateBootstrappingParameterSet* P, const int32_t* bkc, const int32_t* ksk) {
-    if (!P || !bkc || !ksk) {
-        set_error("null argument");
-        return nullptr;
-    }
+static const LweBootstrappingKey* bk_from_arrays(const TFheGateBootstrappingParameterSet* P, const int32_t* bkc,
+                                                 const int32_t* ksk, double ks_var, double bk_var) {
     const TGswParams* gp = P->tgsw_params;
     const TLweParams* tp = gp->tlwe_params;
     const int n = P->in_out_params->n, N = tp->N, k = tp->k, kpl = gp->kpl;
@@ -502,12 +506,50 @@ TFHE_AMD_EXPORT TFheGateBootstrappingCloudKeySet* new_gate_bootstrapping_cloud_k
     for (size_t r = 0; r < (size_t)nin * t * base; r++) {
         memcpy(ks->rows[r].a, ksk + r * (n + 1), sizeof(int32_t) * n);
         ks->rows[r].b = ksk[r * (n + 1) + n];
+        ks->rows[r].current_variance = ks_var;
     }
     BkOwner* bk = new_bk(P, &ks->pub);
     memcpy(bk->coefs.data(), bkc, sizeof(int32_t) * (size_t)n * kpl * (k + 1) * N);
+    for (auto& smp : bk->samples) smp.current_variance = bk_var;
+    return &bk->pub;
+}
+
+namespace tfhe_amd {
+TFheGateBootstrappingCloudKeySet* host_cloud_from_arrays(const TFheGateBootstrappingParameterSet* P, const int32_t* bkc,
+                                                         const int32_t* ksk, double ks_var, double bk_var) {
     auto* co = new CloudOwner();
-    co->pub = make_cloud(P, &bk->pub);
+    co->pub = make_cloud(P, bk_from_arrays(P, bkc, ksk, ks_var, bk_var));
     return &co->pub;
+}
+
+TFheGateBootstrappingSecretKeySet* host_secret_from_arrays(const TFheGateBootstrappingParameterSet* P,
+                                                           const int32_t* bkc, const int32_t* ksk, double ks_var,
+                                                           double bk_var, const int32_t* lwe_key,
+                                                           const int32_t* tlwe_key) {
+    const TGswParams* gp = P->tgsw_params;
+    const TLweParams* tp = gp->tlwe_params;
+    const int n = P->in_out_params->n, N = tp->N, k = tp->k;
+    auto* so = new SecretOwner();
+    so->lwe_bits.assign(lwe_key, lwe_key + n);
+    so->lwe_key = LweKey{P->in_out_params, so->lwe_bits.data()};
+    so->tlwe_bits.assign(tlwe_key, tlwe_key + (size_t)k * N);
+    so->tlwe_polys.resize(k);
+    for (int i = 0; i < k; i++) so->tlwe_polys[i] = IntPolynomial{N, so->tlwe_bits.data() + (size_t)i * N};
+    so->tgsw_key = TGswKey{gp, tp, so->tlwe_polys.data(), TLweKey{tp, so->tlwe_polys.data()}};
+    so->pub = TFheGateBootstrappingSecretKeySet{P, &so->lwe_key, &so->tgsw_key,
+                                                make_cloud(P, bk_from_arrays(P, bkc, ksk, ks_var, bk_var))};
+    return &so->pub;
+}
+}  // namespace tfhe_amd
+
+TFHE_AMD_EXPORT TFheGateBootstrappingCloudKeySet* new_gate_bootstrapping_cloud_keyset_from_arrays(
+    const TFheGateBootstrappingParameterSet* P, const int32_t* bkc, const int32_t* ksk) {
+    if (!P || !bkc || !ksk) {
+        set_error("null argument");
+        return nullptr;
+    }
+    return host_cloud_from_arrays(P, bkc, ksk, P->in_out_params->alpha_min * P->in_out_params->alpha_min,
+                                  P->tgsw_params->tlwe_params->alpha_min * P->tgsw_params->tlwe_params->alpha_min);
 }
 
 TFHE_AMD_EXPORT void delete_gate_bootstrapping_secret_keyset(TFheGateBootstrappingSecretKeySet* keyset) {
