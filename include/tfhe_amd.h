@@ -360,6 +360,11 @@ TFHE_AMD_EXPORT void tfhe_amd_server_destroy(tfhe_amd_server* srv);
  * (small_batch_max < 0 keeps the current value). */
 TFHE_AMD_EXPORT void tfhe_amd_set_launch_config(int32_t br_mode, int32_t small_batch_max);
 TFHE_AMD_EXPORT int32_t tfhe_amd_get_launch_config(int32_t* br_mode, int32_t* small_batch_max);
+/* Keyswitch kernels: 0 (default) = exact int8 matrix-core product of the
+ * one-hot digit matrix with the key's signed byte planes (keys with t = 8,
+ * basebit = 2); 1 = lanes-over-gates integer kernels (any t, basebit). Both
+ * give identical results. */
+TFHE_AMD_EXPORT void tfhe_amd_set_keyswitch_path(int32_t path);
 /* Autotuner: times modes 2 and 3 on this key at batch sizes 128, 256, 384,
  * 512, 640, 768, 896, 1024, 1280, 1536 (milliseconds per batch, including
  * the keyswitch, into ms_four / ms_two if not NULL, 10 entries each) and sets

@@ -37,13 +37,19 @@ def rand_i32(rng, shape):
     return rng.integers(-2 ** 31, 2 ** 31, size=shape, dtype=np.int64).astype(np.int32)
 
 
+@pytest.mark.parametrize("path", [0, 1])
 @pytest.mark.parametrize("count", [1, 3, 40, 600])
-def test_keyswitch_bitexact(p128, count):
+def test_keyswitch_bitexact(p128, count, path):
+    """Both keyswitch kernels (int8 matrix cores; lanes over gates) against the oracle."""
     T, P, sk, keys = p128
     rng = np.random.default_rng(100 + count)
     ext = rand_i32(rng, (count, P.N + 1))
     ext[0, :] = 0  # all-zero digits row: the output must be (0, b) = 0
-    got = T.keyswitch(sk.cloud, P, ext)
+    T.lib().tfhe_amd_set_keyswitch_path(path)
+    try:
+        got = T.keyswitch(sk.cloud, P, ext)
+    finally:
+        T.lib().tfhe_amd_set_keyswitch_path(0)
     idx = list(range(count)) if count <= 40 else list(rng.choice(count, 40, replace=False))
     for g in idx:
         assert np.array_equal(got[g], O.keyswitch(keys, ext[g])), f"gate {g}"

@@ -148,6 +148,7 @@ _SIGS = {
     "tfhe_amd_encrypt_device": (i32, [SKP, i32, vp, vp, ctypes.c_uint64, ctypes.c_uint64, vp]),
     "tfhe_amd_decrypt_device": (i32, [SKP, i32, vp, vp, vp, vp]),
     "tfhe_amd_set_launch_config": (None, [i32, i32]),
+    "tfhe_amd_set_keyswitch_path": (None, [i32]),
     "tfhe_amd_get_launch_config": (i32, [i32p, i32p]),
     "tfhe_amd_autotune": (i32, [CKP, i32p, f32p, f32p]),
     "tfhe_amd_profile_enable": (None, [i32]),

@@ -21,6 +21,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
+#include <atomic>
 #include <cstring>
 #include <mutex>
 #include <unordered_map>
@@ -40,6 +41,7 @@ struct tfhe_amd_device_key {
     c64* tw;       // TwiddleTable<M>::COUNT
     c64* bkp;      // M = 1024 only: [n][kpl][k+1][2][512], the pair-of-wavefronts transform (FftPair)
     c64* twp;      // M = 1024 only: PAIR_TW_COUNT
+    int8_t* kp;    // t = 8, basebit = 2: keyswitch key as signed byte planes [4][k*N][npad][32] (int8 matrix cores)
     size_t bytes;
 };
 
@@ -1110,6 +1112,121 @@ __global__ __launch_bounds__(256) void ks_reduce_kernel(const int32_t* __restric
     }
 }
 
+// ------------------------------------------------ keyswitch on matrix cores --
+//
+// The keyswitch out = (0, ..., 0, b) - sum_{i<kN, j<t} K[i][j][d_ij] is the
+// product OUT = B - D x K of the one-hot digit matrix D[g][(i,j,h)] = [d_gij = h]
+// with the key rows K[(i,j,h)][c] (h = 0 rows are zero). The key is split into
+// four signed byte planes, K = sum_b 2^{8b} s_b (mod 2^32), s_b in [-128, 127]
+// (balanced base-256 digits, the carry out of the top byte dropped), so every
+// plane product runs on the int8 matrix cores and is exact in int32
+// (|sum| <= kN t 128 < 2^31); the planes are recombined mod 2^32.
+// Layout for v_mfma_i32_16x16x64_i8 (lane l: matrix row l%16, k = 16(l/16) + 0..15):
+// the 64 k of one instruction are two input coefficients i, each with t = 8
+// digits x 4 values h, so a lane's 16 A-bytes are the one-hot words of four
+// digits and its 16 B-bytes are kp[b][i][c][16 (l/16 & 1) + 0..15].
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+
+// gate tiles of 16 per wavefront in the matrix-core keyswitch
+#ifndef KS_MMA_GT
+#define KS_MMA_GT 4
+#endif
+
+// 0: matrix cores when the key has byte planes; 1: always the lanes-over-gates kernels
+std::atomic<int> g_ks_path{0};
+
+// one thread per (i, c, j): the 4 values h of key row (i, j) at column c, for the 4 planes
+__global__ __launch_bounds__(256) void ks_planes_kernel(const int32_t* __restrict__ ksk, int nin, int npad,
+                                                        int8_t* __restrict__ kp) {
+    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (size_t)nin * npad * 8) return;
+    const int j = (int)(idx & 7);
+    const int c = (int)((idx >> 3) % npad);
+    const int i = (int)(idx / ((size_t)8 * npad));
+    const size_t plane = (size_t)nin * npad * 32;
+    int8_t* dst = kp + ((size_t)i * npad + c) * 32 + j * 4;
+#pragma unroll
+    for (int h = 0; h < 4; h++) {
+        const uint32_t v = h ? (uint32_t)ksk[((size_t)(i * 8 + j) * 3 + (h - 1)) * npad + c] : 0u;
+        int carry = 0;
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
+            int byte = (int)((v >> (8 * b)) & 255u) + carry;
+            carry = byte >= 128;
+            if (carry) byte -= 256;
+            dst[b * plane + h] = (int8_t)byte;
+        }
+    }
+}
+
+// A workgroup = 4 wavefronts; wavefront w owns 16 output columns (group
+// blockIdx.x*4 + w) of GT*16 gates (GT tiles of 16, from blockIdx.y*GT*16).
+// abar is the digit source a_i + prec_offset, [kN][cpad] with cpad a multiple
+// of 128.
+template <int GT>
+__global__ __launch_bounds__(256, 2) void ks_mma_kernel(const int8_t* __restrict__ kp, const uint32_t* __restrict__ abar,
+                                                     int cpad, int nin, int npad, int count,
+                                                     const int32_t* __restrict__ binit, int n,
+                                                     int32_t* __restrict__ out, const int32_t* __restrict__ out_idx) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int q = lane >> 4, r16 = lane & 15;
+    const int col = (blockIdx.x * 4 + w) * 16 + r16;
+    const int g0 = blockIdx.y * GT * 16;
+    v4i acc[GT][4];
+#pragma unroll
+    for (int gt = 0; gt < GT; gt++)
+#pragma unroll
+        for (int b = 0; b < 4; b++) acc[gt][b] = v4i{0, 0, 0, 0};
+    const size_t plane = (size_t)nin * npad * 32;
+    const int8_t* kb = kp + (size_t)col * 32 + 16 * (q & 1);
+    const uint32_t* ab = abar + g0 + r16;
+    const int sh = (q & 1) ? 16 : 24;  // digits j = 4(q&1) .. 4(q&1)+3 sit in this byte of a_i
+    // operands of the next pair of input coefficients are loaded while the
+    // current pair's products run
+    v4i bv[4];
+    uint32_t av8[GT];
+    auto load = [&](int i0) {
+        const int i = i0 + (q >> 1);
+#pragma unroll
+        for (int b = 0; b < 4; b++) bv[b] = *(const v4i*)(kb + b * plane + (size_t)i * npad * 32);
+#pragma unroll
+        for (int gt = 0; gt < GT; gt++) av8[gt] = ab[(size_t)i * cpad + gt * 16];
+    };
+    load(0);
+    for (int i0 = 0; i0 < nin; i0 += 2) {
+        v4i b_cur[4];
+        uint32_t a_cur[GT];
+#pragma unroll
+        for (int b = 0; b < 4; b++) b_cur[b] = bv[b];
+#pragma unroll
+        for (int gt = 0; gt < GT; gt++) a_cur[gt] = av8[gt];
+        if (i0 + 2 < nin) load(i0 + 2);
+#pragma unroll
+        for (int gt = 0; gt < GT; gt++) {
+            const uint32_t x = (a_cur[gt] >> sh) & 255u;
+            v4i av;
+#pragma unroll
+            for (int jj = 0; jj < 4; jj++) av[jj] = 1 << (((x >> (6 - 2 * jj)) & 3u) << 3);
+#pragma unroll
+            for (int b = 0; b < 4; b++)
+                acc[gt][b] = __builtin_amdgcn_mfma_i32_16x16x64_i8(av, b_cur[b], acc[gt][b], 0, 0, 0);
+        }
+    }
+    if (col > n) return;
+#pragma unroll
+    for (int gt = 0; gt < GT; gt++)
+#pragma unroll
+        for (int rr = 0; rr < 4; rr++) {
+            const int gate = g0 + gt * 16 + 4 * q + rr;
+            if (gate >= count) continue;
+            const uint32_t v = (uint32_t)acc[gt][0][rr] + ((uint32_t)acc[gt][1][rr] << 8) +
+                               ((uint32_t)acc[gt][2][rr] << 16) + ((uint32_t)acc[gt][3][rr] << 24);
+            const uint32_t o = (col == n ? (uint32_t)binit[gate] : 0u) - v;
+            out[(size_t)(out_idx ? out_idx[gate] : gate) * (n + 1) + col] = (int32_t)o;
+        }
+}
+
 // ---------------------------------------------------------------- runtime --
 
 namespace {
@@ -1244,8 +1361,12 @@ KsPlan ks_plan(const HostParams& P, int count) {
 size_t ks_scratch_bytes(const HostParams& P, int count) {
     const KsPlan k = ks_plan(P, count);
     const int npad = ((P.n + 1 + 63) / 64) * 64;
-    return align256(sizeof(uint32_t) * (size_t)P.k * P.N * k.cpad) + align256(sizeof(int32_t) * (size_t)k.cpad) +
-           align256(sizeof(int32_t) * (size_t)k.splits * k.cpad * npad);
+    const size_t lanes = align256(sizeof(uint32_t) * (size_t)P.k * P.N * k.cpad) +
+                         align256(sizeof(int32_t) * (size_t)k.cpad) +
+                         align256(sizeof(int32_t) * (size_t)k.splits * k.cpad * npad);
+    const size_t cpad_m = (size_t)(count + 127) / 128 * 128;
+    const size_t mma = align256(sizeof(uint32_t) * (size_t)P.k * P.N * cpad_m) + align256(sizeof(int32_t) * cpad_m);
+    return lanes > mma ? lanes : mma;
 }
 
 // Keyswitch of ext rows (sum of halves for MUX) into out rows, using the
@@ -1254,6 +1375,19 @@ int run_keyswitch(const tfhe_amd_device_key* dk, const int32_t* ext, const int32
                   const int32_t* out_idx, char* scr, hipStream_t s) {
     const HostParams& P = dk->P;
     const int nin = P.k * P.N;
+    const uint32_t prec_offset0 = 1u << (32 - (1 + P.ks_basebit * P.ks_t));
+    if (dk->kp && g_ks_path.load(std::memory_order_relaxed) == 0) {
+        // matrix-core path: prep (abar [kN][cpad], b) then one product kernel
+        const int cpad = (count + 127) / 128 * 128;
+        uint32_t* abar = (uint32_t*)scr;
+        int32_t* binit = (int32_t*)(scr + align256(sizeof(uint32_t) * (size_t)nin * cpad));
+        hipLaunchKernelGGL(ks_prep_kernel, dim3(count), dim3(256), 0, s, ext, second, nin, 1 << 29, prec_offset0, abar,
+                           cpad, binit);
+        hipLaunchKernelGGL(ks_mma_kernel<KS_MMA_GT>, dim3(dk->npad / 64, cpad / (16 * KS_MMA_GT)), dim3(256), 0, s,
+                           dk->kp, abar, cpad, nin, dk->npad, count, binit, P.n, out, out_idx);
+        HIP_TRY(hipGetLastError());
+        return 0;
+    }
     const KsPlan k = ks_plan(P, count);
     uint32_t* abar = (uint32_t*)scr;
     char* p = scr + align256(sizeof(uint32_t) * (size_t)nin * k.cpad);
@@ -1409,15 +1543,23 @@ static int upload_key(Ctx& C, const LweBootstrappingKeyFFT* bf, tfhe_amd_device_
     dk->bytes = bkf_bytes + ksk_bytes + twc * sizeof(c64);
     dk->bkp = nullptr;
     dk->twp = nullptr;
+    dk->kp = nullptr;
     auto fail = [&](int rc) {
         (void)hipFree(dk->bkf);
         (void)hipFree(dk->ksk);
         (void)hipFree(dk->tw);
         (void)hipFree(dk->bkp);
         (void)hipFree(dk->twp);
+        (void)hipFree(dk->kp);
         delete dk;
         return rc;
     };
+    const size_t kp_bytes = (size_t)4 * nin * npad * 32;
+    if (P.ks_t == 8 && P.ks_basebit == 2 && npad % 64 == 0 && nin % 2 == 0) {
+        if (hipMalloc(&dk->kp, kp_bytes) != hipSuccess)
+            return fail(set_error("hipMalloc(keyswitch byte planes, %zu bytes) failed", kp_bytes));
+        dk->bytes += kp_bytes;
+    }
     if (M == 1024 && k == 1 && n <= BRP_NMAX) {
         if (hipMalloc(&dk->bkp, bkf_bytes) != hipSuccess || hipMalloc(&dk->twp, PAIR_TW_COUNT * sizeof(c64)) != hipSuccess)
             return fail(set_error("hipMalloc(pair-transform key, %zu bytes) failed", bkf_bytes));
@@ -1466,6 +1608,12 @@ static int upload_key(Ctx& C, const LweBootstrappingKeyFFT* bf, tfhe_amd_device_
             hipLaunchKernelGGL(bk_convert_pair_kernel, dim3((unsigned)npoly), dim3(128), 0, C.stream, d_coef, dk->bkp, dk->twp);
             e = hipGetLastError();
         }
+        if (e == hipSuccess && dk->kp) {
+            const size_t threads = (size_t)nin * npad * 8;
+            hipLaunchKernelGGL(ks_planes_kernel, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, C.stream,
+                               dk->ksk, nin, npad, dk->kp);
+            e = hipGetLastError();
+        }
     }
     if (e == hipSuccess) e = hipStreamSynchronize(C.stream);
     (void)hipFree(d_coef);
@@ -1498,6 +1646,7 @@ static void free_key(Ctx& C, tfhe_amd_device_key* dk) {
     (void)hipFree(dk->tw);
     (void)hipFree(dk->bkp);
     (void)hipFree(dk->twp);
+    (void)hipFree(dk->kp);
     delete dk;
 }
 
@@ -1799,6 +1948,8 @@ TFHE_AMD_EXPORT int32_t tfhe_amd_autotune(const TFheGateBootstrappingCloudKeySet
     }
     return cleanup(0);
 }
+
+TFHE_AMD_EXPORT void tfhe_amd_set_keyswitch_path(int32_t path) { g_ks_path.store(path == 1 ? 1 : 0); }
 
 TFHE_AMD_EXPORT void tfhe_amd_profile_enable(int32_t on) {
     Ctx& C = ctx();
