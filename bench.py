@@ -44,6 +44,8 @@ def parse():
     ap.add_argument("--serve-rates", default="1e3,1e4,1e5", help="serving section: offered gates/s")
     ap.add_argument("--serve-seconds", type=float, default=10.0, help="serving section: arrival window per rate")
     ap.add_argument("--serve-min-requests", type=int, default=4)
+    ap.add_argument("--serve-streams", type=int, default=16, help="serving section: streams the clients share")
+    ap.add_argument("--serve-depth", type=int, default=2, help="serving section: launches in flight per client")
     ap.add_argument("--serve-max-requests", type=int, default=160)
     ap.add_argument("--pbig-launches", type=int, default=64,
                     help="N=2048 mixed-batch section: launches (0 = skip)")
@@ -129,11 +131,12 @@ def bench_serving(T, P, args):
         if (i + 1) % 8 == 0:
             print(f"[bench] serving: {i + 1} client keys ({time.perf_counter() - t0:.1f} s)", file=sys.stderr,
                   flush=True)
-    S = Server(n_streams=4, depth=2, max_batch=4096, wire_budget_bytes=4 << 30)
+    S = Server(n_streams=args.serve_streams, depth=args.serve_depth, max_batch=4096, wire_budget_bytes=4 << 30)
     cids = [S.add_client(sk.cloud, P.n) for sk in keys]
     mean_gates = 32.5 * 256.5
     out = {"workload": f"{n_cl} clients (own P128 key each, resident), Poisson arrivals, random layered circuits "
                        "depth U[1,64] x width U[1,512], 11 gate types uniform", "clients": n_cl,
+           "streams": args.serve_streams, "launches_in_flight_per_client": args.serve_depth,
            "key_bytes_resident": S.stats()["key_bytes"], "rates": {}}
     rng = np.random.default_rng(args.seed + 5)
     for rate in [float(x) for x in args.serve_rates.split(",") if x.strip()]:

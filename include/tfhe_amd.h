@@ -323,7 +323,7 @@ TFHE_AMD_EXPORT int32_t tfhe_amd_decrypt_device(const TFheGateBootstrappingSecre
  * streams with up to `depth` launches in flight per client, and copies each
  * request's outputs back when its last level has finished. Keys stay resident
  * (their HBM is counted in key_bytes); all clients of a server share one
- * parameter set. 0 for n_streams / depth / max_batch picks 4 / 2 / 4096. */
+ * parameter set. 0 for n_streams / depth / max_batch picks 8 / 2 / 4096. */
 typedef struct tfhe_amd_server tfhe_amd_server;
 typedef struct {
     int64_t submitted, completed, failed, gates, launches, max_launch_gates;

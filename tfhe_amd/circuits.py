@@ -172,7 +172,7 @@ class Server:
     """Continuous-batching request loop (tfhe_amd_server_*). submit() returns a
     request id; wait() returns (output ciphertexts [n_out][n+1], latency ms)."""
 
-    def __init__(self, n_streams=4, depth=2, max_batch=4096, wire_budget_bytes=1 << 30):
+    def __init__(self, n_streams=8, depth=2, max_batch=4096, wire_budget_bytes=1 << 30):
         L = lib()
         for name, (res, args) in _SERVER_SIGS.items():
             f = getattr(L, name)

@@ -410,7 +410,7 @@ void tfhe_amd_server::run() {
 
 TFHE_AMD_EXPORT tfhe_amd_server* tfhe_amd_server_create(int32_t n_streams, int32_t depth, int32_t max_batch,
                                                         size_t wire_budget_bytes) {
-    if (n_streams <= 0) n_streams = 4;
+    if (n_streams <= 0) n_streams = 8;
     if (depth <= 0) depth = 2;
     if (max_batch <= 0) max_batch = 4096;
     if (n_streams > 16 || depth > 8 || max_batch > (1 << 20) || wire_budget_bytes < (1 << 20)) {
