@@ -38,7 +38,7 @@ def parse():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--adders", type=int, default=512, help="ripple-carry adders in the netlist section (0 = skip)")
     ap.add_argument("--adder-bits", type=int, default=32)
-    ap.add_argument("--scaling", default="64,256,1024,2048,8192",
+    ap.add_argument("--scaling", default="1,16,64,128,256,512,1024,2048,8192",
                     help="batch sizes for the per-kernel batch-size sweep ('' = skip)")
     ap.add_argument("--serve-clients", type=int, default=32, help="serving section: clients, one key each (0 = skip)")
     ap.add_argument("--serve-rates", default="1e3,1e4,1e5", help="serving section: offered gates/s")
@@ -87,8 +87,8 @@ def bench_scaling(T, sk, P, sizes, seed):
     out = {}
     for B in sizes:
         row = {}
-        for mode in (2, 3, 0):
-            T.lib().tfhe_amd_set_launch_config(mode, -1)
+        for mode in (5, 3, 2, 0):
+            T.lib().tfhe_amd_set_launch_config(mode, -1, -1)
             T.check(T.lib().tfhe_amd_gates_device(sk.cloud, B, d_ops.ptr, d_i0.ptr, d_i1.ptr, d_i0.ptr, d_o.ptr,
                                                   d_pool.ptr, None), "gates")
             T.synchronize()
@@ -99,11 +99,11 @@ def bench_scaling(T, sk, P, sizes, seed):
                                                       d_pool.ptr, None), "gates")
             T.synchronize()
             dt = (time.perf_counter() - t0) / reps
-            row[{2: "four_gate_lockstep", 3: "two_wave_split", 0: "auto"}[mode]] = {
+            row[{2: "four_gate_lockstep", 3: "two_wave_split", 5: "digit_parallel", 0: "auto"}[mode]] = {
                 "gates_per_s": round(B / dt, 1), "ms": round(dt * 1e3, 3)}
         out[str(B)] = row
         print(f"[bench] batch {B}: {row}", file=sys.stderr, flush=True)
-    T.lib().tfhe_amd_set_launch_config(0, -1)
+    T.lib().tfhe_amd_set_launch_config(0, -1, -1)
     dec = sk.decrypt(d_pool.numpy()[2 * maxb:2 * maxb + max(sizes)])
     truth = {0: lambda x, y: 1 - (x & y), 1: lambda x, y: x & y, 2: lambda x, y: x | y, 3: lambda x, y: x ^ y,
              4: lambda x, y: 1 - (x ^ y), 5: lambda x, y: 1 - (x | y)}
@@ -417,7 +417,7 @@ def main():
     if args.compare_modes:
         mode_cmp = {}
         for mode in (1, 2):
-            T.lib().tfhe_amd_set_launch_config(mode, -1)
+            T.lib().tfhe_amd_set_launch_config(mode, -1, -1)
             step()
             T.synchronize()
             t1 = time.perf_counter()
@@ -426,7 +426,7 @@ def main():
             T.synchronize()
             mode_cmp[{1: "one_wave_per_bootstrap", 2: "four_gate_lockstep"}[mode]] = round(
                 B * args.steps / (time.perf_counter() - t1), 2)
-        T.lib().tfhe_amd_set_launch_config(0, -1)
+        T.lib().tfhe_amd_set_launch_config(0, -1, -1)
         print(f"[bench] modes {mode_cmp}", file=sys.stderr, flush=True)
 
     # one gate per launch (same kernels, count = 1, synchronised each time)
@@ -450,11 +450,11 @@ def main():
 
     autotune = None
     if not args.no_autotune:
-        cross, tune = T.autotune(sk.cloud)
-        autotune = {"small_batch_max": cross,
-                    "ms_four_gate_lockstep": {str(k): round(v[0], 3) for k, v in tune.items()},
-                    "ms_two_wave_split": {str(k): round(v[1], 3) for k, v in tune.items()}}
-        print(f"[bench] autotune: two-wavefront kernel up to {cross} gates", file=sys.stderr, flush=True)
+        lat, small, tune = T.autotune(sk.cloud)
+        autotune = {"latency_kernel_up_to": lat, "two_wave_kernel_up_to": small,
+                    "ms": {name: {str(k): round(v, 3) for k, v in t.items()} for name, t in tune.items()}}
+        print(f"[bench] autotune: 2l-wavefront kernel up to {lat} gates, two-wavefront up to {small}",
+              file=sys.stderr, flush=True)
     sizes = [int(x) for x in args.scaling.split(",") if x.strip()]
     scaling = bench_scaling(T, sk, P, sizes, args.seed) if sizes else None
     adders = bench_adders(T, sk, P, args.adders, args.adder_bits, 2, args.seed) if args.adders > 0 else None

@@ -355,23 +355,27 @@ TFHE_AMD_EXPORT void tfhe_amd_server_destroy(tfhe_amd_server* srv);
 
 /* Blind-rotation kernel choice. mode: 0 = automatic, 1 = one wavefront per
  * bootstrap, 2 = four bootstraps per workgroup in lockstep, 3 = two
- * wavefronts per bootstrap (P128-sized keys). In automatic mode batches of
- * at most small_batch_max gates use mode 3, larger ones mode 2
- * (small_batch_max < 0 keeps the current value). */
-TFHE_AMD_EXPORT void tfhe_amd_set_launch_config(int32_t br_mode, int32_t small_batch_max);
-TFHE_AMD_EXPORT int32_t tfhe_amd_get_launch_config(int32_t* br_mode, int32_t* small_batch_max);
+ * wavefronts per bootstrap, 5 = 2l wavefronts per bootstrap (lowest latency;
+ * modes 2, 3, 5 for N = 1024), 4 = the N = 2048 pair-transform kernel (the
+ * automatic choice for N = 2048). In automatic mode, for N = 1024, batches of
+ * at most latency_max gates use mode 5, then up to small_batch_max mode 3,
+ * larger ones mode 2 (a negative value keeps the current one). */
+TFHE_AMD_EXPORT void tfhe_amd_set_launch_config(int32_t br_mode, int32_t latency_max, int32_t small_batch_max);
+TFHE_AMD_EXPORT int32_t tfhe_amd_get_launch_config(int32_t* br_mode, int32_t* latency_max, int32_t* small_batch_max);
+/* Autotuner: times modes 5, 3 and 2 on this key at batch sizes 64, 128, 256,
+ * 384, 512, 640, 768, 896, 1024, 1280, 1536 (milliseconds per batch including
+ * the keyswitch, into ms[3][11] in that mode order, if not NULL) and sets
+ * latency_max to the largest size up to which mode 5 is the fastest at every
+ * measured point, small_batch_max to the largest size up to which mode 3 then
+ * beats mode 2 at every further point. Synchronous; allocates and frees its
+ * own buffers. */
+TFHE_AMD_EXPORT int32_t tfhe_amd_autotune(const TFheGateBootstrappingCloudKeySet* ck, int32_t* latency_max,
+                                          int32_t* small_batch_max, float* ms);
 /* Keyswitch kernels: 0 (default) = exact int8 matrix-core product of the
  * one-hot digit matrix with the key's signed byte planes (keys with t = 8,
  * basebit = 2); 1 = lanes-over-gates integer kernels (any t, basebit). Both
  * give identical results. */
 TFHE_AMD_EXPORT void tfhe_amd_set_keyswitch_path(int32_t path);
-/* Autotuner: times modes 2 and 3 on this key at batch sizes 128, 256, 384,
- * 512, 640, 768, 896, 1024, 1280, 1536 (milliseconds per batch, including
- * the keyswitch, into ms_four / ms_two if not NULL, 10 entries each) and sets
- * small_batch_max to the largest size up to which mode 3 is faster at every
- * measured point. Synchronous; allocates and frees its own buffers. */
-TFHE_AMD_EXPORT int32_t tfhe_amd_autotune(const TFheGateBootstrappingCloudKeySet* ck, int32_t* small_batch_max,
-                                          float* ms_four, float* ms_two);
 
 /* Per-launch timing: when enabled, every device gate batch records HIP
  * events on its stream around the blind-rotation kernel and around the

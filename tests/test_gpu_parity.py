@@ -270,24 +270,25 @@ def test_c_program_gpu(tmp_path):
     assert r.returncode == 0, r.stdout + r.stderr
 
 
-@pytest.mark.parametrize("mode", [1, 2, 3])
+@pytest.mark.parametrize("mode", [1, 2, 3, 5])
 def test_all_blind_rotation_kernels_bitexact(p128, mode):
-    """The three blind-rotation kernels (one wavefront per bootstrap, four
-    bootstraps per workgroup, two wavefronts per bootstrap) give the oracle's
-    ciphertexts bit-exactly, MUX included, for odd batch sizes."""
+    """The blind-rotation kernels (1: one wavefront per bootstrap, 2: four
+    bootstraps per workgroup, 3: two wavefronts per bootstrap, 5: 2l
+    wavefronts per bootstrap) give the oracle's ciphertexts bit-exactly, MUX
+    included, for odd batch sizes."""
     T, P, sk, keys = p128
     L = T.lib()
     rng = np.random.default_rng(20 + mode)
     ops = np.array([10, 0, 3, 10, 5, 1, 2], np.int32)  # 7 gates, 9 bootstraps
     bits, a, b, c = _encrypt_inputs(sk, rng, len(ops))
-    L.tfhe_amd_set_launch_config(mode, -1)
+    L.tfhe_amd_set_launch_config(mode, -1, -1)
     try:
         got = T.run_gates(sk.cloud, ops, a, b, c)
         big_ops = rng.integers(0, 11, size=301).astype(np.int32)
         bb, ba, bbb, bc = _encrypt_inputs(sk, rng, len(big_ops))
         big = T.run_gates(sk.cloud, big_ops, ba, bbb, bc)
     finally:
-        L.tfhe_amd_set_launch_config(0, -1)
+        L.tfhe_amd_set_launch_config(0, -1, -1)
     ref = O.gates(keys, ops, a, b, c, threads=16)
     assert np.array_equal(got, ref)
     exp = np.array([O.TRUTH[O.GATES[o]](x, y, z) for o, x, y, z in zip(big_ops, bb[0], bb[1], bb[2])])

@@ -147,10 +147,10 @@ _SIGS = {
     "delete_LweKey": (None, [vp]),
     "tfhe_amd_encrypt_device": (i32, [SKP, i32, vp, vp, ctypes.c_uint64, ctypes.c_uint64, vp]),
     "tfhe_amd_decrypt_device": (i32, [SKP, i32, vp, vp, vp, vp]),
-    "tfhe_amd_set_launch_config": (None, [i32, i32]),
+    "tfhe_amd_set_launch_config": (None, [i32, i32, i32]),
     "tfhe_amd_set_keyswitch_path": (None, [i32]),
-    "tfhe_amd_get_launch_config": (i32, [i32p, i32p]),
-    "tfhe_amd_autotune": (i32, [CKP, i32p, f32p, f32p]),
+    "tfhe_amd_get_launch_config": (i32, [i32p, i32p, i32p]),
+    "tfhe_amd_autotune": (i32, [CKP, i32p, i32p, f32p]),
     "tfhe_amd_profile_enable": (None, [i32]),
     "tfhe_amd_profile_read": (i32, [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double), i32p]),
     "tfhe_amd_copy_device": (i32, [vp, vp, ctypes.c_size_t, vp]),
@@ -430,28 +430,30 @@ def decrypt_device(sk, d_cts, d_bits=None, d_phases=None, count=None, stream=Non
                                         d_phases.ptr if d_phases else None, stream), "decrypt_device")
 
 
-AUTOTUNE_SIZES = (128, 256, 384, 512, 640, 768, 896, 1024, 1280, 1536)
+AUTOTUNE_SIZES = (64, 128, 256, 384, 512, 640, 768, 896, 1024, 1280, 1536)
+AUTOTUNE_MODES = ("digit_parallel", "two_wave_split", "four_gate_lockstep")
 
 
 def autotune(cloud):
-    """Measures the two blind-rotation kernels on this key over AUTOTUNE_SIZES
-    and sets the automatic mode's small-batch crossover. Returns
-    (crossover, {size: (ms four-bootstrap kernel, ms two-wavefront kernel)})."""
-    cross = ctypes.c_int32()
-    four = (ctypes.c_float * len(AUTOTUNE_SIZES))()
-    two = (ctypes.c_float * len(AUTOTUNE_SIZES))()
-    check(lib().tfhe_amd_autotune(cloud, ctypes.byref(cross), four, two), "autotune")
-    return cross.value, {s: (four[i], two[i]) for i, s in enumerate(AUTOTUNE_SIZES)}
+    """Measures the three blind-rotation kernels on this key over AUTOTUNE_SIZES
+    and sets the automatic mode's crossovers. Returns (latency_max,
+    small_batch_max, {mode name: {size: ms}})."""
+    lat, small = ctypes.c_int32(), ctypes.c_int32()
+    ms = (ctypes.c_float * (3 * len(AUTOTUNE_SIZES)))()
+    check(lib().tfhe_amd_autotune(cloud, ctypes.byref(lat), ctypes.byref(small), ms), "autotune")
+    table = {name: {sz: ms[m * len(AUTOTUNE_SIZES) + i] for i, sz in enumerate(AUTOTUNE_SIZES)}
+             for m, name in enumerate(AUTOTUNE_MODES)}
+    return lat.value, small.value, table
 
 
-def set_launch_config(mode, small_batch_max=-1):
-    lib().tfhe_amd_set_launch_config(int(mode), int(small_batch_max))
+def set_launch_config(mode, latency_max=-1, small_batch_max=-1):
+    lib().tfhe_amd_set_launch_config(int(mode), int(latency_max), int(small_batch_max))
 
 
 def get_launch_config():
-    m, sb = ctypes.c_int32(), ctypes.c_int32()
-    lib().tfhe_amd_get_launch_config(ctypes.byref(m), ctypes.byref(sb))
-    return m.value, sb.value
+    m, lat, sb = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+    lib().tfhe_amd_get_launch_config(ctypes.byref(m), ctypes.byref(lat), ctypes.byref(sb))
+    return m.value, lat.value, sb.value
 
 
 def profile_enable(on=True):

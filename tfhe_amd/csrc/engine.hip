@@ -482,6 +482,167 @@ __global__ __launch_bounds__(256, 2) void br4_kernel(BrArgs A, const int32_t* __
     if (lane == 0) e[N] = (int32_t)acc[1][0];
 }
 
+// largest LWE dimension the latency kernel's LDS table holds
+constexpr int BRL_NMAX = 1024;
+// largest LWE dimension for which the autotuner compares the N = 1024 kernels
+constexpr int BR4_ANY_NMAX = 768;
+
+// Blind rotation for latency (M = 512): one bootstrap per workgroup of 2l
+// wavefronts, wavefront (c, p) owning digit polynomial p of accumulator
+// component c. Per key row every wavefront rotates and decomposes its
+// component (accumulator in LDS), transforms its one digit polynomial and
+// multiplies it by its key row (c*l + p), held in registers and loaded at the
+// start of the row so the load runs under the transform; the 2l products for
+// each output component are summed by wavefront (c, 0), which runs the
+// inverse transform and updates ACC_c. Per row a bootstrap costs one forward
+// and one inverse transform of latency instead of l + 1.
+template <int L>
+__global__ __launch_bounds__(2 * L * 64, 1) void brl_kernel(BrArgs A, const int32_t* __restrict__ mux_gate,
+                                                           const int32_t* __restrict__ n_jobs) {
+    constexpr int M = 512;
+    using F = Fft<M>;
+    constexpr int R = F::R, N = 2 * M, KPL = 2 * L, W = 2 * L;
+    constexpr int LOG2N2 = 11;
+    __shared__ c64 s_fft[W][F::LDS_C64];  // transform buffer; [0, M) then holds the product for output 0
+    __shared__ c64 s_par[W][M];           // product for output 1
+    __shared__ int32_t s_acc[2][N];
+    __shared__ c64 s_tw1[R * 64];
+    __shared__ int16_t s_bara[BRL_NMAX];
+    __shared__ int32_t s_barb;
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int w = tid >> 6;
+    const int c = w / L, p = w % L;
+    const int job = blockIdx.x;
+    if (job >= *n_jobs) return;  // uniform per workgroup
+    const int n = A.n;
+    c64* fb = s_fft[w];
+
+    if (w == 0) {
+        const int gate = job < A.count ? job : mux_gate[job - A.count];
+        const int half = job < A.count ? 0 : 1;
+        const int op = A.op[gate];
+        int32_t k0, ca, cb, cc;
+        gate_coeffs(op, half, k0, ca, cb, cc);
+        const int32_t* a = A.pool + (size_t)A.in0[gate] * (n + 1);
+        const int32_t* b = (op == OP_IDENTITY) ? a : A.pool + (size_t)A.in1[gate] * (n + 1);
+        const int32_t* cp = (op == TFHE_AMD_MUX) ? A.pool + (size_t)A.in2[gate] * (n + 1) : a;
+        for (int i = lane; i <= n; i += 64) {
+            uint32_t v = (uint32_t)ca * (uint32_t)a[i] + (uint32_t)cb * (uint32_t)b[i] + (uint32_t)cc * (uint32_t)cp[i];
+            if (i == n) v += (uint32_t)k0;
+            // modSwitchFromTorus32(v, 2N), libtfhe/numeric-functions.cpp:59-65
+            const uint32_t bar = (v + (1u << (31 - LOG2N2))) >> (32 - LOG2N2);
+            if (i < n) s_bara[i] = (int16_t)bar;
+            else s_barb = (int32_t)bar;
+        }
+    }
+    for (int q = tid; q < R * 64; q += W * 64) s_tw1[q] = A.tw[(q & 63) * (R + 8) + (q >> 6)];
+    const c64* tw1 = s_tw1 + lane;
+    const c64 tw2 = A.tw[lane * (R + 8) + R + 1];
+    const uint32_t offset = A.offset;
+    const int Bgbit = A.Bgbit;
+    const int32_t dhalf = 1 << (Bgbit - 1);
+    __syncthreads();
+    {
+        // ACC = (0, X^{2N-barb} * [mu...mu]), lwe-bootstrapping-functions-fft.cpp:146-148
+        const int barb = s_barb;
+        for (int j = tid; j < N; j += W * 64) {
+            s_acc[0][j] = 0;
+            s_acc[1][j] = (((j + barb) & (2 * N - 1)) < N) ? A.mu : -A.mu;
+        }
+    }
+    __syncthreads();
+
+    const int32_t* accc = s_acc[c];
+    for (int i = 0; i < n; i++) {
+        const int abar = s_bara[i];
+        if (abar == 0) continue;  // lwe-bootstrapping-functions-fft.cpp:101 (uniform per workgroup)
+        // key row (i, c*l + p), both output components, into registers
+        c64 k0[R], k1[R];
+        {
+            const c64* q = A.bkf + ((size_t)i * KPL + c * L + p) * 2 * M + lane;
+#pragma unroll
+            for (int s = 0; s < R; s++) {
+                k0[s] = q[s * 64];
+                k1[s] = q[M + s * 64];
+            }
+        }
+        // T = (X^abar - 1) * ACC_c + gadget offset, then digit p
+        // (tgsw-functions.cpp:337-348)
+        uint32_t tv[2 * R];
+#pragma unroll
+        for (int qq = 0; qq < 2 * R; qq++) {
+            const int j = 64 * (qq % R) + lane + (qq / R) * M;
+            tv[qq] = (uint32_t)accc[(j - abar) & (N - 1)];
+        }
+#pragma unroll
+        for (int qq = 0; qq < 2 * R; qq++) {
+            const int j = 64 * (qq % R) + lane + (qq / R) * M;
+            const uint32_t rot = ((j - abar) & N) ? 0u - tv[qq] : tv[qq];
+            tv[qq] = rot - (uint32_t)accc[j] + offset;
+        }
+        const int shift = 32 - (p + 1) * Bgbit;
+        c64 x[R];
+#pragma unroll
+        for (int s = 0; s < R; s++) {
+            const int32_t d0 = (int32_t)__builtin_amdgcn_ubfe(tv[s], shift, Bgbit) - dhalf;
+            const int32_t d1 = (int32_t)__builtin_amdgcn_ubfe(tv[R + s], shift, Bgbit) - dhalf;
+            x[s] = c64{(double)d0, (double)d1};
+        }
+        F::template fwd1<64>(x, lane, fb, tw1);
+        wave_sync();
+        F::fwd2(x, lane, fb, tw2);
+        wave_sync();
+        F::fwd3(x, lane, fb);
+        wave_sync();
+        F::fwd4(x, lane, fb);
+        wave_sync();
+#pragma unroll
+        for (int s = 0; s < R; s++) {
+            fb[s * 64 + lane] = cmul(x[s], k0[s]);
+            s_par[w][s * 64 + lane] = cmul(x[s], k1[s]);
+        }
+        __syncthreads();  // all products written
+        c64 y[R];
+        if (p == 0) {
+            // output component c: sum of the 2l products
+#pragma unroll
+            for (int s = 0; s < R; s++) y[s] = c64{0.0, 0.0};
+#pragma unroll
+            for (int v = 0; v < W; v++) {
+                const c64* src = c ? s_par[v] : s_fft[v];
+#pragma unroll
+                for (int s = 0; s < R; s++) y[s] = cadd(y[s], src[s * 64 + lane]);
+            }
+        }
+        __syncthreads();  // products read before the inverse transform reuses the buffers
+        if (p == 0) {
+            F::inv1(y, lane, fb);
+            wave_sync();
+            F::inv2(y, lane, fb, tw2);
+            wave_sync();
+            F::inv3(y, lane, fb);
+            wave_sync();
+            F::template inv4<64>(y, lane, fb, tw1);
+            // ACC_c += result (lwe-bootstrapping-functions-fft.cpp:67-76)
+            int32_t* acc = s_acc[c];
+#pragma unroll
+            for (int s = 0; s < R; s++) {
+                const int j = 64 * s + lane;
+                acc[j] = (int32_t)((uint32_t)acc[j] + (uint32_t)round_mod32(y[s].re));
+                acc[j + M] = (int32_t)((uint32_t)acc[j + M] + (uint32_t)round_mod32(y[s].im));
+            }
+        }
+        __syncthreads();  // ACC updated for the next row
+    }
+    if (w != 0) return;
+    // tLweExtractLweSampleIndex(.., 0, ..), libtfhe/lwe.cpp:41-56
+    int32_t* e = A.ext + (size_t)job * (N + 1);
+    for (int j = lane; j < N; j += 64) e[j == 0 ? 0 : N - j] = j == 0 ? s_acc[0][0] : (int32_t)(0u - (uint32_t)s_acc[0][j]);
+    if (lane == 0) e[N] = s_acc[1][0];
+}
+
 // largest LWE dimension the two-wavefront kernel's LDS table holds
 constexpr int BR2_NMAX = 768;
 
@@ -1245,7 +1406,8 @@ struct Ctx {
     void* h_pinned = nullptr;
     size_t h_cap = 0;
     int br_mode = 0;
-    int br_small_max = 512;  // automatic mode: batches up to this many gates use two wavefronts per bootstrap
+    int brl_max = 256;        // automatic mode: batches up to this many gates use 2l wavefronts per bootstrap
+    int br_small_max = 1024;  // automatic mode: batches up to this many gates use two wavefronts per bootstrap
     // optional per-launch timing with HIP events on the launch stream
     bool prof = false;
     struct Ev { hipEvent_t e[3]; };
@@ -1450,15 +1612,23 @@ int gates_device_locked(Ctx& C, tfhe_amd_device_key* dk, int count, const int32_
         HIP_TRY(hipEventRecord(ev.e[0], s));
     }
     // kernel choice: 1 = one wavefront per bootstrap, 2 = four bootstraps per workgroup,
-    // 3 = two wavefronts per bootstrap; 0 = automatic (two wavefronts while the batch
-    // leaves CUs idle, four-gate lockstep for large batches). N = 2048 uses the
-    // pair-transform kernel (4) unless mode 1 is forced.
+    // 3 = two wavefronts per bootstrap, 5 = 2l wavefronts per bootstrap; 0 = automatic
+    // (5 for the smallest batches, 3 while the batch leaves CUs idle, 2 for large
+    // batches). N = 2048 uses the pair-transform kernel (4) unless mode 1 is forced.
     int mode = C.br_mode;
     if (dk->M != 512) mode = (mode == 1 || dk->P.n > BRP_NMAX || !dk->bkp) ? 1 : 4;
     else if (mode == 4) mode = 2;
     if (mode == 3 && dk->P.n > BR2_NMAX) mode = 2;
-    if (mode == 0) mode = (count <= C.br_small_max && dk->P.n <= BR2_NMAX) ? 3 : 2;
-    if (mode == 4) {
+    if (mode == 0) {
+        if (count <= C.brl_max && dk->P.n <= BRL_NMAX) mode = 5;
+        else mode = (count <= C.br_small_max && dk->P.n <= BR2_NMAX) ? 3 : 2;
+    }
+    if (mode == 5 && (dk->M != 512 || dk->P.n > BRL_NMAX)) mode = 3;
+    if (mode == 5) {
+        const int blocks = 2 * count;  // one bootstrap per workgroup, at most 2*count jobs
+        if (dk->P.l == 3) hipLaunchKernelGGL(brl_kernel<3>, dim3(blocks), dim3(384), 0, s, A, mux_gate, n_jobs);
+        else hipLaunchKernelGGL(brl_kernel<2>, dim3(blocks), dim3(256), 0, s, A, mux_gate, n_jobs);
+    } else if (mode == 4) {
         const int blocks = count;  // two bootstraps per workgroup, at most 2*count jobs
         if (dk->P.l == 2) hipLaunchKernelGGL(brp_kernel<2>, dim3(blocks), dim3(512), 0, s, A, dk->bkp, dk->twp, mux_gate, n_jobs);
         else hipLaunchKernelGGL(brp_kernel<3>, dim3(blocks), dim3(512), 0, s, A, dk->bkp, dk->twp, mux_gate, n_jobs);
@@ -1855,36 +2025,43 @@ TFHE_AMD_EXPORT int32_t tfhe_amd_keyswitch_device(const TFheGateBootstrappingClo
                          stream ? (hipStream_t)stream : C.stream);
 }
 
-TFHE_AMD_EXPORT void tfhe_amd_set_launch_config(int32_t br_mode, int32_t small_batch_max) {
+TFHE_AMD_EXPORT void tfhe_amd_set_launch_config(int32_t br_mode, int32_t latency_max, int32_t small_batch_max) {
     Ctx& C = ctx();
     std::lock_guard<std::mutex> g(C.mu);
     C.br_mode = br_mode;
+    if (latency_max >= 0) C.brl_max = latency_max;
     if (small_batch_max >= 0) C.br_small_max = small_batch_max;
 }
 
-TFHE_AMD_EXPORT int32_t tfhe_amd_get_launch_config(int32_t* br_mode, int32_t* small_batch_max) {
+TFHE_AMD_EXPORT int32_t tfhe_amd_get_launch_config(int32_t* br_mode, int32_t* latency_max, int32_t* small_batch_max) {
     Ctx& C = ctx();
     std::lock_guard<std::mutex> g(C.mu);
     if (br_mode) *br_mode = C.br_mode;
+    if (latency_max) *latency_max = C.brl_max;
     if (small_batch_max) *small_batch_max = C.br_small_max;
     return 0;
 }
 
-// Times the four-bootstrap and the two-wavefront blind rotations (each with
-// its keyswitch) on trivial ciphertexts over a ladder of batch sizes and sets
-// the automatic mode's crossover to the largest size below which the
-// two-wavefront kernel is faster at every measured point.
-TFHE_AMD_EXPORT int32_t tfhe_amd_autotune(const TFheGateBootstrappingCloudKeySet* ck, int32_t* small_batch_max,
-                                          float* ms_four, float* ms_two) {
+// Times the three N = 1024 blind-rotation kernels (each with its keyswitch) on
+// pseudo-random ciphertexts over a ladder of batch sizes and sets the automatic
+// mode's crossovers: the 2l-wavefront kernel up to the largest size up to
+// which it is the fastest at every measured point, then the two-wavefront
+// kernel up to the largest size up to which it beats the four-gate kernel at
+// every further point.
+TFHE_AMD_EXPORT int32_t tfhe_amd_autotune(const TFheGateBootstrappingCloudKeySet* ck, int32_t* latency_max,
+                                          int32_t* small_batch_max, float* ms) {
     if (!ck) return set_error("null cloud key");
     tfhe_amd_device_key* dk = engine_key(ck->bkFFT);
     if (!dk) return -1;
     Ctx& C = ctx();
     std::lock_guard<std::mutex> g(C.mu);
-    static const int sizes[] = {128, 256, 384, 512, 640, 768, 896, 1024, 1280, 1536};
+    static const int sizes[] = {64, 128, 256, 384, 512, 640, 768, 896, 1024, 1280, 1536};
     constexpr int NS = sizeof(sizes) / sizeof(sizes[0]);
-    if (dk->M != 512 || dk->P.n > BR2_NMAX) {
+    static const int modes[3] = {5, 3, 2};
+    if (dk->M != 512 || dk->P.n > BR4_ANY_NMAX) {
+        C.brl_max = 0;
         C.br_small_max = 0;
+        if (latency_max) *latency_max = 0;
         if (small_batch_max) *small_batch_max = 0;
         return 0;
     }
@@ -1909,16 +2086,24 @@ TFHE_AMD_EXPORT int32_t tfhe_amd_autotune(const TFheGateBootstrappingCloudKeySet
     if (hipMalloc(&d_pool, sizeof(int32_t) * 3 * (size_t)maxB * w) != hipSuccess ||
         hipMalloc(&d_idx, sizeof(int32_t) * idx.size()) != hipSuccess)
         return cleanup(set_error("autotune: device allocation failed"));
-    if (hipMemsetAsync(d_pool, 0, sizeof(int32_t) * 3 * (size_t)maxB * w, C.stream) != hipSuccess ||
+    // pseudo-random ciphertexts: every key row is used (a zero mask would let
+    // the kernels that skip rows with a zero rotation look faster than they are)
+    std::vector<int32_t> pool(3 * (size_t)maxB * w);
+    uint64_t st = 0x9e3779b97f4a7c15ULL;
+    for (auto& v : pool) {
+        st = st * 6364136223846793005ULL + 1442695040888963407ULL;
+        v = (int32_t)(st >> 32);
+    }
+    if (hipMemcpy(d_pool, pool.data(), sizeof(int32_t) * pool.size(), hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpyAsync(d_idx, idx.data(), sizeof(int32_t) * idx.size(), hipMemcpyHostToDevice, C.stream) != hipSuccess ||
         hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
         return cleanup(set_error("autotune: set-up failed"));
     if (int rc = ensure_scratch(C, ext_bytes(dk->P, maxB) + ks_scratch_bytes(dk->P, maxB))) return cleanup(rc);
     const int saved = C.br_mode;
-    float best[2][NS];
+    float best[3][NS];
     int rc = 0;
-    for (int m = 0; m < 2 && !rc; m++) {
-        C.br_mode = m ? 3 : 2;
+    for (int m = 0; m < 3 && !rc; m++) {
+        C.br_mode = modes[m];
         for (int si = 0; si < NS && !rc; si++) {
             const int B = sizes[si];
             best[m][si] = 1e30f;
@@ -1928,24 +2113,27 @@ TFHE_AMD_EXPORT int32_t tfhe_amd_autotune(const TFheGateBootstrappingCloudKeySet
                     rc = gates_device_locked(C, dk, B, d_idx, d_idx + maxB, d_idx + 2 * maxB, d_idx + maxB,
                                              d_idx + 3 * maxB, d_pool, nullptr, 1 << 29, (char*)C.scratch.d,
                                              C.stream, false);
-                float ms = 0;
+                float t = 0;
                 if (!rc && (hipEventRecord(e1, C.stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
-                            hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+                            hipEventElapsedTime(&t, e0, e1) != hipSuccess))
                     rc = set_error("autotune: event timing failed");
-                if (!rc && rep) best[m][si] = std::min(best[m][si], ms);  // first run warms up
+                if (!rc && rep) best[m][si] = std::min(best[m][si], t);  // first run warms up
             }
         }
     }
     C.br_mode = saved;
     if (rc) return cleanup(rc < 0 ? rc : set_error("autotune: event record failed"));
-    int cross = 0;
-    for (int si = 0; si < NS && best[1][si] < best[0][si]; si++) cross = sizes[si];
-    C.br_small_max = cross;
-    if (small_batch_max) *small_batch_max = cross;
-    for (int si = 0; si < NS; si++) {
-        if (ms_four) ms_four[si] = best[0][si];
-        if (ms_two) ms_two[si] = best[1][si];
-    }
+    int si = 0, lat = 0, small = 0;
+    for (; si < NS && best[0][si] <= best[1][si] && best[0][si] <= best[2][si]; si++) lat = sizes[si];
+    small = lat;
+    for (; si < NS && best[1][si] < best[2][si]; si++) small = sizes[si];
+    C.brl_max = lat;
+    C.br_small_max = small;
+    if (latency_max) *latency_max = lat;
+    if (small_batch_max) *small_batch_max = small;
+    if (ms)
+        for (int m = 0; m < 3; m++)
+            for (int k = 0; k < NS; k++) ms[m * NS + k] = best[m][k];
     return cleanup(0);
 }
 
