@@ -295,3 +295,28 @@ def test_all_blind_rotation_kernels_bitexact(p128, mode):
     assert np.array_equal(sk.decrypt(big), exp)
     sel = [0, 150, 300]
     assert np.array_equal(big[sel], O.gates(keys, big_ops[sel], ba[sel], bbb[sel], bc[sel], threads=8))
+
+
+def test_autotune_crossovers_and_auto_mode(p128):
+    """The autotuner returns ordered crossovers within its ladder, and the
+    automatic kernel choice then gives the same ciphertexts as a forced kernel
+    on both sides of each crossover."""
+    T, P, sk, keys = p128
+    lat, small, table = T.autotune(sk.cloud)
+    try:
+        assert 0 <= lat <= small <= max(T.AUTOTUNE_SIZES)
+        assert all(v > 0 for t in table.values() for v in t.values())
+        assert T.get_launch_config()[1:] == (lat, small)
+        rng = np.random.default_rng(31)
+        for count in sorted({max(1, lat), lat + 1, small, small + 1}):
+            ops = rng.integers(0, 11, size=count).astype(np.int32)
+            bits, a, b, c = _encrypt_inputs(sk, rng, count)
+            auto = T.run_gates(sk.cloud, ops, a, b, c)
+            T.set_launch_config(2)
+            try:
+                forced = T.run_gates(sk.cloud, ops, a, b, c)
+            finally:
+                T.set_launch_config(0)
+            assert np.array_equal(auto, forced), count
+    finally:
+        T.set_launch_config(0, 256, 1024)
