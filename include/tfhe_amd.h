@@ -314,6 +314,16 @@ TFHE_AMD_EXPORT int32_t tfhe_amd_decrypt_device(const TFheGateBootstrappingSecre
                                                 const int32_t* d_cts, int32_t* d_bits, int32_t* d_phases,
                                                 void* stream);
 
+/* Key generation on the device: a full secret key set (LWE and TLWE keys,
+ * keyswitch key, bootstrapping key) drawn from Philox4x32-10 keyed by seed,
+ * following the reference's procedure (libtfhe/tfhe_gate_bootstrapping.cpp:
+ * 97-109) but not its generator order, so these are not the keys
+ * new_random_gate_bootstrapping_secret_keyset gives for any seed. Takes
+ * milliseconds instead of the host's sequential generator; free with
+ * delete_gate_bootstrapping_secret_keyset. */
+TFHE_AMD_EXPORT TFheGateBootstrappingSecretKeySet* tfhe_amd_keygen_device(const TFheGateBootstrappingParameterSet* params,
+                                                                        uint64_t seed);
+
 /* Serving loop (no counterpart in the reference; its callers are circuits
  * like test/test-long-run.cpp:61-185 submitted by many clients). One
  * scheduler thread admits requests in arrival order as their wires fit in a

@@ -146,6 +146,7 @@ _SIGS = {
     "new_lweKey_fromFile": (vp, [vp]),
     "delete_LweKey": (None, [vp]),
     "tfhe_amd_encrypt_device": (i32, [SKP, i32, vp, vp, ctypes.c_uint64, ctypes.c_uint64, vp]),
+    "tfhe_amd_keygen_device": (SKP, [PP, ctypes.c_uint64]),
     "tfhe_amd_decrypt_device": (i32, [SKP, i32, vp, vp, vp, vp]),
     "tfhe_amd_set_launch_config": (None, [i32, i32, i32]),
     "tfhe_amd_set_keyswitch_path": (None, [i32]),
@@ -235,6 +236,14 @@ class SecretKey:
         self.params = params
         self.ptr = ptr
         self.cloud = lib().tfhe_amd_cloud_of(self.ptr)
+
+    @classmethod
+    def generate_on_device(cls, params, seed):
+        """Key set generated on the device (Philox4x32-10 keyed by seed)."""
+        ptr = lib().tfhe_amd_keygen_device(params.ptr, seed)
+        if not ptr:
+            raise TfheAmdError(lib().tfhe_amd_last_error().decode())
+        return cls(params, ptr=ptr)
 
     @classmethod
     def load(cls, path):

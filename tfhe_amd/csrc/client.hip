@@ -14,6 +14,7 @@
 #include <cmath>
 #include <map>
 #include <mutex>
+#include <vector>
 
 #include "engine.h"
 #include "philox.h"
@@ -76,6 +77,108 @@ __global__ void __launch_bounds__(256) encrypt_kernel(const int32_t* __restrict_
         const int32_t et = (int32_t)(int64_t)((e - (double)(int64_t)e) * 4294967296.0);
         const int32_t m = bits[g] ? mu : -mu;
         out[n] = (int32_t)((uint32_t)m + (uint32_t)et + s);
+    }
+}
+
+// ------------------------------------------------------- key generation --
+//
+// Keys drawn on the device from Philox4x32-10 keyed by a 64-bit seed; value
+// `idx` of stream `st` is word 0 of block (idx lo, idx hi, st, 0), a Gaussian
+// of stream `st` comes from the Box-Muller transform of the whole block. The
+// procedure follows the reference's key generation
+// (libtfhe/tfhe_gate_bootstrapping.cpp:97-109): LWE and TLWE secret bits, the
+// keyswitch key rows ks[i][j][h] = LWE_s(h * t_i / base^(j+1)) for h >= 1
+// (libtfhe/lwe-keyswitch-functions.cpp:163-215) and the TGSW encryptions of the
+// LWE key bits (libtfhe/tgsw-functions.cpp:114-140), with the exact negacyclic
+// product for the TLWE b polynomials. The reference's generator order is not
+// reproduced (new_random_gate_bootstrapping_secret_keyset does that on the host).
+
+__device__ inline uint32_t rnd32(uint64_t idx, uint32_t st, uint32_t k0, uint32_t k1) {
+    return philox4x32_10(Philox4{{(uint32_t)idx, (uint32_t)(idx >> 32), st, 0u}}, k0, k1).v[0];
+}
+
+__device__ inline int32_t gauss_torus(uint64_t idx, uint32_t st, double sigma, uint32_t k0, uint32_t k1) {
+    const Philox4 r = philox4x32_10(Philox4{{(uint32_t)idx, (uint32_t)(idx >> 32), st, 1u}}, k0, k1);
+    const double u1 = ((double)((((uint64_t)r.v[0] << 32) | r.v[1]) >> 11) + 1.0) * 0x1p-53;
+    const double u2 = (double)((((uint64_t)r.v[2] << 32) | r.v[3]) >> 11) * 0x1p-53;
+    const double e = sigma * sqrt(-2.0 * log(u1)) * cospi(2.0 * u2);
+    return (int32_t)(int64_t)((e - (double)(int64_t)e) * 4294967296.0);  // dtot32
+}
+
+constexpr uint32_t ST_LWE_KEY = 1, ST_TLWE_KEY = 2, ST_KS_MASK = 3, ST_KS_NOISE = 4, ST_BK_MASK = 5, ST_BK_NOISE = 6;
+
+__global__ void __launch_bounds__(256) keygen_bits_kernel(int n, int nk, uint32_t k0, uint32_t k1,
+                                                          int32_t* __restrict__ lwe, int32_t* __restrict__ tlwe) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx < n) lwe[idx] = (int32_t)(rnd32(idx, ST_LWE_KEY, k0, k1) & 1u);
+    if (idx < nk) tlwe[idx] = (int32_t)(rnd32(idx, ST_TLWE_KEY, k0, k1) & 1u);
+}
+
+// one wavefront per keyswitch row (i, j, h), layout [nin][t][base][n+1]; h = 0 rows are zero
+__global__ void __launch_bounds__(256) keygen_ksk_kernel(int nin, int t, int basebit, int n, double alpha,
+                                                         const int32_t* __restrict__ lwe,
+                                                         const int32_t* __restrict__ tlwe, uint32_t k0, uint32_t k1,
+                                                         int32_t* __restrict__ ksk) {
+    const int lane = threadIdx.x & 63;
+    const int base = 1 << basebit;
+    const int64_t row = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (row >= (int64_t)nin * t * base) return;
+    const int h = (int)(row % base), j = (int)((row / base) % t), i = (int)(row / ((int64_t)base * t));
+    int32_t* out = ksk + row * (n + 1);
+    if (h == 0) {
+        for (int p = lane; p <= n; p += 64) out[p] = 0;
+        return;
+    }
+    uint32_t s = 0;
+    for (int p = lane; p < n; p += 64) {
+        const uint32_t a = rnd32((uint64_t)row * n + p, ST_KS_MASK, k0, k1);
+        out[p] = (int32_t)a;
+        s += a * (uint32_t)lwe[p];
+    }
+    for (int d = 32; d >= 1; d >>= 1) s += (uint32_t)__shfl_xor((int)s, d, 64);
+    if (lane == 0) {
+        const uint32_t msg = ((uint32_t)tlwe[i] * (uint32_t)h) << (32 - (j + 1) * basebit);
+        out[n] = (int32_t)(msg + (uint32_t)gauss_torus((uint64_t)row, ST_KS_NOISE, alpha, k0, k1) + s);
+    }
+}
+
+// one workgroup per TGSW row (i, r): masks uniform, b = sum_c a_c * s_c + e
+// (exact negacyclic products with the binary TLWE key), then mu * h_p added to
+// coefficient 0 of component r / l (libtfhe/tgsw-functions.cpp:114-140).
+__global__ void __launch_bounds__(1024) keygen_bk_kernel(int N, int k, int l, int Bgbit, double alpha,
+                                                         const int32_t* __restrict__ lwe,
+                                                         const int32_t* __restrict__ tlwe, uint32_t k0, uint32_t k1,
+                                                         int32_t* __restrict__ bk) {
+    extern __shared__ int32_t sm[];  // a[k][N], s[k][N]
+    const int kpl = (k + 1) * l;
+    const int64_t row = blockIdx.x;  // i * kpl + r
+    const int i = (int)(row / kpl), r = (int)(row % kpl);
+    int32_t* poly = bk + row * (k + 1) * N;
+    for (int q = threadIdx.x; q < k * N; q += blockDim.x) {
+        const int32_t a = (int32_t)rnd32((uint64_t)row * k * N + q, ST_BK_MASK, k0, k1);
+        sm[q] = a;
+        sm[k * N + q] = tlwe[q];
+        poly[q] = a;
+    }
+    __syncthreads();
+    for (int jj = threadIdx.x; jj < N; jj += blockDim.x) {
+        uint32_t b = (uint32_t)gauss_torus((uint64_t)row * N + jj, ST_BK_NOISE, alpha, k0, k1);
+        for (int c = 0; c < k; c++) {
+            const int32_t* a = sm + c * N;
+            const int32_t* sc = sm + k * N + c * N;
+            // (a * s)_jj = sum_{t <= jj} s_t a_{jj-t} - sum_{t > jj} s_t a_{jj-t+N}
+            for (int tt = 0; tt < N; tt++) {
+                const uint32_t prod = (uint32_t)sc[tt] * (uint32_t)a[(jj - tt) & (N - 1)];
+                b += (tt <= jj) ? prod : 0u - prod;
+            }
+        }
+        poly[k * N + jj] = (int32_t)b;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int blk = r / l, p = r % l;
+        int32_t* c0 = poly + (size_t)blk * N;
+        c0[0] = (int32_t)((uint32_t)c0[0] + (uint32_t)lwe[i] * (1u << (32 - (p + 1) * Bgbit)));
     }
 }
 
@@ -153,4 +256,59 @@ TFHE_AMD_EXPORT int32_t tfhe_amd_decrypt_device(const TFheGateBootstrappingSecre
                        d_phases);
     const hipError_t e = hipGetLastError();
     return e == hipSuccess ? 0 : set_error("decrypt_device: launch failed: %s", hipGetErrorString(e));
+}
+
+TFHE_AMD_EXPORT TFheGateBootstrappingSecretKeySet* tfhe_amd_keygen_device(const TFheGateBootstrappingParameterSet* P,
+                                                                        uint64_t seed) {
+    if (!P) {
+        set_error("keygen_device: null parameters");
+        return nullptr;
+    }
+    const TGswParams* gp = P->tgsw_params;
+    const TLweParams* tp = gp->tlwe_params;
+    const int n = P->in_out_params->n, N = tp->N, k = tp->k, l = gp->l, kpl = gp->kpl;
+    const int t = P->ks_t, basebit = P->ks_basebit, base = 1 << basebit, nin = k * N;
+    const size_t ksk_words = (size_t)nin * t * base * (n + 1), bk_words = (size_t)n * kpl * (k + 1) * N;
+    hipStream_t s = (hipStream_t)engine_stream();
+    if (!s) return nullptr;
+    int32_t *d_lwe = nullptr, *d_tlwe = nullptr, *d_ksk = nullptr, *d_bk = nullptr;
+    std::vector<int32_t> lwe(n), tlwe(nin), ksk(ksk_words), bk(bk_words);
+    auto done = [&](bool ok) {
+        (void)hipFree(d_lwe);
+        (void)hipFree(d_tlwe);
+        (void)hipFree(d_ksk);
+        (void)hipFree(d_bk);
+        return ok;
+    };
+    const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
+    bool ok = hipMalloc(&d_lwe, sizeof(int32_t) * n) == hipSuccess &&
+              hipMalloc(&d_tlwe, sizeof(int32_t) * nin) == hipSuccess &&
+              hipMalloc(&d_ksk, sizeof(int32_t) * ksk_words) == hipSuccess &&
+              hipMalloc(&d_bk, sizeof(int32_t) * bk_words) == hipSuccess;
+    if (!ok) {
+        done(false);
+        set_error("keygen_device: device allocation failed");
+        return nullptr;
+    }
+    const int m = n > nin ? n : nin;
+    hipLaunchKernelGGL(keygen_bits_kernel, dim3((m + 255) / 256), dim3(256), 0, s, n, nin, k0, k1, d_lwe, d_tlwe);
+    const int64_t rows = (int64_t)nin * t * base;
+    hipLaunchKernelGGL(keygen_ksk_kernel, dim3((unsigned)((rows + 3) / 4)), dim3(256), 0, s, nin, t, basebit, n,
+                       P->in_out_params->alpha_min, d_lwe, d_tlwe, k0, k1, d_ksk);
+    hipLaunchKernelGGL(keygen_bk_kernel, dim3((unsigned)(n * kpl)), dim3(1024), sizeof(int32_t) * 2 * k * N, s, N, k, l,
+                       gp->Bgbit, tp->alpha_min, d_lwe, d_tlwe, k0, k1, d_bk);
+    ok = hipGetLastError() == hipSuccess &&
+         hipMemcpyAsync(lwe.data(), d_lwe, sizeof(int32_t) * n, hipMemcpyDeviceToHost, s) == hipSuccess &&
+         hipMemcpyAsync(tlwe.data(), d_tlwe, sizeof(int32_t) * nin, hipMemcpyDeviceToHost, s) == hipSuccess &&
+         hipMemcpyAsync(ksk.data(), d_ksk, sizeof(int32_t) * ksk_words, hipMemcpyDeviceToHost, s) == hipSuccess &&
+         hipMemcpyAsync(bk.data(), d_bk, sizeof(int32_t) * bk_words, hipMemcpyDeviceToHost, s) == hipSuccess &&
+         hipStreamSynchronize(s) == hipSuccess;
+    done(ok);
+    if (!ok) {
+        set_error("keygen_device: key generation failed on the device");
+        return nullptr;
+    }
+    const double ks_var = P->in_out_params->alpha_min * P->in_out_params->alpha_min;
+    const double bk_var = tp->alpha_min * tp->alpha_min;
+    return host_secret_from_arrays(P, bk.data(), ksk.data(), ks_var, bk_var, lwe.data(), tlwe.data());
 }
