@@ -111,3 +111,38 @@ def test_pbig_pair_kernel_matches_one_wave_kernel(pbig):
     assert np.array_equal(one, pair)
     exp = np.array([O.TRUTH[O.GATES[o]](x, y, z) for o, x, y, z in zip(ops, *bits)])
     assert np.array_equal(sk.decrypt(pair), exp)
+
+
+@pytest.fixture(scope="module")
+def n2048_l3(gpu):
+    """N = 2048 with three gadget digits (n = 630, Bgbit = 7, P128's noise): the l = 3
+    instantiation of the pair-transform kernel."""
+    T = gpu
+    P = T.Params(T.lib().new_gate_bootstrapping_parameters(630, 2048, 1, 3, 7, 8, 2, 2.0 ** -15, 2.0 ** -25,
+                                                           0.012467))
+    sk = T.SecretKey(P, seed=6)
+    keys = O.Keys(O.OrcParams(630, 2048, 1, 3, 7, 8, 2, 2.0 ** -15, 2.0 ** -25), **sk.export())
+    sk.upload()
+    yield T, P, sk, keys
+    sk.close()
+
+
+def test_n2048_l3_pair_kernel_bitexact(n2048_l3):
+    """Every gate type (and extra MUX, odd batch) through the l = 3 pair kernel:
+    bit-exact against the oracle and the one-wavefront kernel, and decrypting to
+    the plaintext gate."""
+    T, P, sk, keys = n2048_l3
+    rng = np.random.default_rng(46)
+    ops = np.array(list(range(11)) + [10, 10], np.int32)
+    bits = rng.integers(0, 2, size=(3, len(ops))).astype(np.int32)
+    a, b, c = (sk.encrypt(x) for x in bits)
+    pair = T.run_gates(sk.cloud, ops, a, b, c)
+    T.set_launch_config(1)
+    try:
+        one = T.run_gates(sk.cloud, ops, a, b, c)
+    finally:
+        T.set_launch_config(0)
+    assert np.array_equal(pair, one)
+    assert np.array_equal(pair, O.gates(keys, ops, a, b, c, threads=16))
+    exp = np.array([O.TRUTH[O.GATES[o]](x, y, z) for o, x, y, z in zip(ops, *bits)])
+    assert np.array_equal(sk.decrypt(pair), exp)

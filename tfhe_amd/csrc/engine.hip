@@ -22,6 +22,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <atomic>
+#include <type_traits>
 #include <cstring>
 #include <mutex>
 #include <unordered_map>
@@ -116,7 +117,7 @@ __device__ __forceinline__ void gate_coeffs(int op, int half, int32_t& k0, int32
 }
 
 template <int M, int L>
-__global__ __launch_bounds__(64, 2) void br_kernel(BrArgs A) {
+__global__ __launch_bounds__(64, M == 512 ? 2 : 1) void br_kernel(BrArgs A) {
     using F = Fft<M>;
     constexpr int R = F::R, N = 2 * M, KPL = 2 * L;
     constexpr int LOG2N2 = (M == 512) ? 11 : 12;  // log2(2N)
@@ -858,13 +859,18 @@ constexpr int BRP_NMAX = 768;
 // Blind rotation for N = 2048 with four wavefronts per bootstrap and two
 // bootstraps per 512-thread workgroup. Wavefront (g, c, h): bootstrap g,
 // accumulator component c, half h of the pair-of-wavefronts transform
-// (FftPair): it owns ACC_c coefficients 512h + 64a + lane (+1024), a < 8.
-// Per key row and digit p the two wavefronts of component c rotate and
-// decompose ACC_c, exchange their folded halves for the radix-2 step, run one
-// 512-point transform each and multiply-accumulate their bins against key rows
-// (c*l + p), which the workgroup copies into LDS once for both bootstraps;
-// then the component partners swap partial sums and each pair inverse-
-// transforms its own component.
+// (FftPair); it owns the quarters m = h and h + 2 of ACC_c (coefficients
+// 512 m + 64 a + lane, a < 8). At the start of a key row each wavefront also
+// holds its partner's quarters, so it writes the whole of ACC_c into its own
+// LDS buffer and the rotation, the gadget decomposition and the transform's
+// radix-2 step across the pair (y_h from all four quarters) need no exchange
+// inside the pair. Each wavefront then runs its 512-point transform and
+// multiply-accumulates its bins against key rows (c*l + p), which the
+// workgroup copies into LDS once for both bootstraps; the component partners
+// swap partial sums, each pair inverse-transforms its own component and
+// exchanges the two halves once, and both wavefronts of the pair form the
+// whole update (the partner's old quarters come through the idle key buffer).
+// Seven workgroup barriers per key row for l = 2.
 template <int L>
 __global__ __launch_bounds__(512, 1) void brp_kernel(BrArgs A, const c64* __restrict__ bkp, const c64* __restrict__ twp,
                                                     const int32_t* __restrict__ mux_gate,
@@ -891,6 +897,9 @@ __global__ __launch_bounds__(512, 1) void brp_kernel(BrArgs A, const c64* __rest
     int16_t* bara = s_bara[g];
     c64* fb = s_fft[w];
     int32_t* fbi = (int32_t*)fb;
+    // while no key block is in flight: slot of wavefront w for its old quarters of ACC_c
+    int32_t* const acc_out = (int32_t*)&s_bk[0][0] + w * (2 * R * 64);
+    const int32_t* const acc_in = (const int32_t*)&s_bk[0][0] + (w ^ 1) * (2 * R * 64);
 
     // gate linear form and modulus switch, one wavefront per bootstrap
     if (c == 1 && h == 0) {
@@ -928,31 +937,36 @@ __global__ __launch_bounds__(512, 1) void brp_kernel(BrArgs A, const c64* __rest
     const int32_t dhalf = 1 << (Bgbit - 1);
     __syncthreads();
 
-    // acc[hh*8 + a] = ACC_c[512 h + 64 a + lane + 1024 hh]
-    uint32_t acc[2 * R];
+    // own quarters acc[q * R + a] = ACC_c[512 (h + 2q) + 64 a + lane]; pacc: the partner's
+    // quarters (1 - h + 2q), valid from the end of one key row to the start of the next.
+    // A workgroup's idle second bootstrap (odd batch) runs the same steps on zeros.
+    uint32_t acc[2 * R], pacc[2 * R];
 #pragma unroll
-    for (int k = 0; k < 2 * R; k++) acc[k] = 0;
+    for (int k = 0; k < 2 * R; k++) acc[k] = pacc[k] = 0;
     if (c == 1 && active) {
         // ACC_1 = X^{2N-barb} * [mu...mu], lwe-bootstrapping-functions-fft.cpp:146-148
         const int barb = s_barb[g];
 #pragma unroll
-        for (int hh = 0; hh < 2; hh++)
+        for (int q = 0; q < 2; q++)
 #pragma unroll
             for (int a = 0; a < R; a++) {
-                const int j = 512 * h + 64 * a + lane + 1024 * hh;
-                acc[hh * R + a] = (((j + barb) & (2 * N - 1)) < N) ? (uint32_t)A.mu : 0u - (uint32_t)A.mu;
+                const int jo = 512 * (h + 2 * q) + 64 * a + lane, jp = 512 * (1 - h + 2 * q) + 64 * a + lane;
+                acc[q * R + a] = (((jo + barb) & (2 * N - 1)) < N) ? (uint32_t)A.mu : 0u - (uint32_t)A.mu;
+                pacc[q * R + a] = (((jp + barb) & (2 * N - 1)) < N) ? (uint32_t)A.mu : 0u - (uint32_t)A.mu;
             }
     }
 
     for (int i = 0; i < n; i++) {
         const int abar = bara[i];
-        c64 p0[R], p1[R];
-#pragma unroll
-        for (int s = 0; s < R; s++) p0[s] = p1[s] = c64{0.0, 0.0};
-        uint32_t tv[2 * R];
-#pragma unroll 1
-        for (int p = 0; p < L; p++) {
-            __syncthreads();  // key blocks and every transform buffer are free
+        c64 p0[R], p1[R];  // set by the first digit's product
+        // digits 1..L-1 of the own and the partner's quarters, two 16-bit fields per word;
+        // the next one to use is always dig[0]
+        uint32_t dig[L - 1][2][R];
+        // one gadget digit: key blocks to LDS, decomposition, transform, product.
+        // FIRST (digit 0) also rotates ACC_c and decomposes every digit.
+        auto digit = [&](const int p, auto first) {
+            constexpr bool FIRST = decltype(first)::value;
+            __syncthreads();  // key blocks free; the partner has read this buffer
             {
                 // key digit blocks (i, p) and (i, L + p) -> LDS, 8 x 16 B per thread
                 const c64* q0 = bkp + ((size_t)i * KPL + p) * 2048 + tid;
@@ -967,75 +981,115 @@ __global__ __launch_bounds__(512, 1) void brp_kernel(BrArgs A, const c64* __rest
                                                      (__attribute__((address_space(3))) void*)(d1 + u * 512), 16, 0, 0);
                 }
             }
-            if (p == 0) {
-                // T = (X^abar - 1) * ACC_c + gadget offset; coefficient k of ACC_c sits in
-                // the buffer of wavefront h' = (k >> 9) & 1 of the pair, at int ((k >> 10) << 9) + (k & 511)
+            // x: u_{j'} = d_{j'} + i d_{j'+1024} of the own quarters, ov: of the partner's
+            c64 x[R], ov[R];
+            if constexpr (FIRST) {
+                // T = (X^abar - 1) * ACC_c + gadget offset, from this wavefront's copy of ACC_c
+                int32_t* const fo = fbi + 512 * h + lane;
+                int32_t* const fp = fbi + 512 * (1 - h) + lane;
 #pragma unroll
-                for (int hh = 0; hh < 2; hh++)
+                for (int q = 0; q < 2; q++)
 #pragma unroll
-                    for (int a = 0; a < R; a++) fbi[hh * 512 + 64 * a + lane] = (int32_t)acc[hh * R + a];
-                __syncthreads();
-                if (active) {
-                    const int32_t* pair0 = (const int32_t*)s_fft[w & ~1];
-                    constexpr int STRIDE = F::LDS_C64 * 4;  // ints per transform buffer
+                    for (int a = 0; a < R; a++) {
+                        fo[1024 * q + 64 * a] = (int32_t)acc[q * R + a];
+                        fp[1024 * q + 64 * a] = (int32_t)pacc[q * R + a];
+                    }
+                wave_sync();
+                // rotated positions: base (own or partner's quarter 0 position, minus abar)
+                // plus a compile-time offset
+                const int bo = 512 * h + lane - abar, bp = 512 * (1 - h) + lane - abar;
+                uint32_t to[2 * R], tp[2 * R];
 #pragma unroll
-                    for (int hh = 0; hh < 2; hh++)
+                for (int q = 0; q < 2; q++)
 #pragma unroll
-                        for (int a = 0; a < R; a++) {
-                            const int j = 512 * h + 64 * a + lane + 1024 * hh;
-                            const int idx = (j - abar) & (2 * N - 1);
-                            const int k = idx & (N - 1);
-                            const uint32_t v = (uint32_t)pair0[((k >> 9) & 1) * STRIDE + ((k >> 10) << 9) + (k & 511)];
-                            const uint32_t rot = (idx & N) ? 0u - v : v;
-                            tv[hh * R + a] = rot - acc[hh * R + a] + offset;
-                        }
+                    for (int a = 0; a < R; a++) {
+                        const int io = (bo + 1024 * q + 64 * a) & (2 * N - 1);
+                        const int ip = (bp + 1024 * q + 64 * a) & (2 * N - 1);
+                        to[q * R + a] = (uint32_t)fbi[io & (N - 1)];
+                        tp[q * R + a] = (uint32_t)fbi[ip & (N - 1)];
+                    }
+#pragma unroll
+                for (int q = 0; q < 2; q++)
+#pragma unroll
+                    for (int a = 0; a < R; a++) {
+                        const int io = (bo + 1024 * q + 64 * a) & (2 * N - 1);
+                        const int ip = (bp + 1024 * q + 64 * a) & (2 * N - 1);
+                        const uint32_t ro = (io & N) ? 0u - to[q * R + a] : to[q * R + a];
+                        const uint32_t rp = (ip & N) ? 0u - tp[q * R + a] : tp[q * R + a];
+                        to[q * R + a] = ro - acc[q * R + a] + offset;
+                        tp[q * R + a] = rp - pacc[q * R + a] + offset;
+                    }
+                wave_sync();  // the reads are done before the transform reuses the buffer
+                // the truncating gadget decomposition, tgsw-functions.cpp:337-348
+#pragma unroll
+                for (int pp = 1; pp < L; pp++) {
+                    const int sh = 32 - (pp + 1) * Bgbit;
+#pragma unroll
+                    for (int a = 0; a < R; a++) {
+                        dig[pp - 1][0][a] = __builtin_amdgcn_ubfe(to[a], sh, Bgbit) |
+                                            (__builtin_amdgcn_ubfe(to[R + a], sh, Bgbit) << 16);
+                        dig[pp - 1][1][a] = __builtin_amdgcn_ubfe(tp[a], sh, Bgbit) |
+                                            (__builtin_amdgcn_ubfe(tp[R + a], sh, Bgbit) << 16);
+                    }
                 }
-                __syncthreads();
-            }
-            c64 x[R];
-            if (active) {
-                // digit p of the truncating gadget decomposition, tgsw-functions.cpp:337-348
-                const int shift = 32 - (p + 1) * Bgbit;
+                const int sh = 32 - Bgbit;
 #pragma unroll
                 for (int a = 0; a < R; a++) {
-                    const int32_t d0v = (int32_t)__builtin_amdgcn_ubfe(tv[a], shift, Bgbit) - dhalf;
-                    const int32_t d1v = (int32_t)__builtin_amdgcn_ubfe(tv[R + a], shift, Bgbit) - dhalf;
-                    x[a] = c64{(double)d0v, (double)d1v};
-                    fb[a * 64 + lane] = x[a];
+                    x[a] = c64{(double)((int32_t)__builtin_amdgcn_ubfe(to[a], sh, Bgbit) - dhalf),
+                               (double)((int32_t)__builtin_amdgcn_ubfe(to[R + a], sh, Bgbit) - dhalf)};
+                    ov[a] = c64{(double)((int32_t)__builtin_amdgcn_ubfe(tp[a], sh, Bgbit) - dhalf),
+                                (double)((int32_t)__builtin_amdgcn_ubfe(tp[R + a], sh, Bgbit) - dhalf)};
                 }
-            }
-            __syncthreads();
-            if (active) {
-                const c64* o = s_fft[w ^ 1];
-                c64 ov[R];
+            } else {
 #pragma unroll
-                for (int a = 0; a < R; a++) ov[a] = o[a * 64 + lane];
-                if (h == 0) P::pair_fwd<0>(x, ov);
-                else P::pair_fwd<1>(x, ov);
+                for (int a = 0; a < R; a++) {
+                    const uint32_t vo = dig[0][0][a], vp = dig[0][1][a];
+                    x[a] = c64{(double)((int32_t)(vo & 0xffffu) - dhalf), (double)((int32_t)(vo >> 16) - dhalf)};
+                    ov[a] = c64{(double)((int32_t)(vp & 0xffffu) - dhalf), (double)((int32_t)(vp >> 16) - dhalf)};
+                }
+#pragma unroll
+                for (int pp = 0; pp + 1 < L - 1; pp++)
+#pragma unroll
+                    for (int a = 0; a < R; a++) {
+                        dig[pp][0][a] = dig[pp + 1][0][a];
+                        dig[pp][1][a] = dig[pp + 1][1][a];
+                    }
             }
-            __syncthreads();  // the partner has read this buffer
-            if (active) {
-                if (h == 0) P::template fwd1<0, 64>(x, lane, fb, tw1);
-                else P::template fwd1<1, 64>(x, lane, fb, tw1);
-                wave_sync();
-                F::fwd2(x, lane, fb, tw2);
-                wave_sync();
-                F::fwd3(x, lane, fb);
-                wave_sync();
-                F::fwd4(x, lane, fb);
+            if (h == 0) {
+                P::pair_fwd<0>(x, ov);
+                P::template fwd1<0, 64>(x, lane, fb, tw1);
+            } else {
+                P::pair_fwd<1>(x, ov);
+                P::template fwd1<1, 64>(x, lane, fb, tw1);
             }
+            wave_sync();
+            F::fwd2(x, lane, fb, tw2);
+            wave_sync();
+            F::fwd3(x, lane, fb);
+            wave_sync();
+            F::fwd4(x, lane, fb);
             __syncthreads();  // key blocks landed
-            if (active) {
-                const c64* bk = s_bk[c] + h * 512;
+            const c64* bk = s_bk[c] + h * 512;
 #pragma unroll
-                for (int s = 0; s < R; s++) {
+            for (int s = 0; s < R; s++) {
+                if constexpr (FIRST) {
+                    p0[s] = cmul_first(x[s], bk[s * 64 + lane]);
+                    p1[s] = cmul_first(x[s], bk[1024 + s * 64 + lane]);
+                } else {
                     cmac(p0[s], x[s], bk[s * 64 + lane]);
                     cmac(p1[s], x[s], bk[1024 + s * 64 + lane]);
                 }
             }
+        };
+        digit(0, std::true_type{});
+        if constexpr (L == 2) {
+            digit(1, std::false_type{});
+        } else {
+#pragma unroll 1
+            for (int p = 1; p < L; p++) digit(p, std::false_type{});
         }
         // swap partial sums with the other component's wavefront of the same half
-        if (active) {
+        {
             if (c) {
 #pragma unroll
                 for (int s = 0; s < R; s++) fb[s * 64 + lane] = p0[s];
@@ -1044,9 +1098,11 @@ __global__ __launch_bounds__(512, 1) void brp_kernel(BrArgs A, const c64* __rest
                 for (int s = 0; s < R; s++) fb[s * 64 + lane] = p1[s];
             }
         }
-        __syncthreads();
+        __syncthreads();  // partials written; every multiply-accumulate is done with the key blocks
         c64 x[R];
-        if (active) {
+        {
+#pragma unroll
+            for (int k = 0; k < 2 * R; k++) acc_out[k * 64 + lane] = (int32_t)acc[k];
             const c64* other = s_fft[w ^ 2];
             if (c) {
 #pragma unroll
@@ -1057,7 +1113,7 @@ __global__ __launch_bounds__(512, 1) void brp_kernel(BrArgs A, const c64* __rest
             }
         }
         __syncthreads();
-        if (active) {
+        {
             F::inv1(x, lane, fb);
             wave_sync();
             F::inv2(x, lane, fb, tw2);
@@ -1071,18 +1127,23 @@ __global__ __launch_bounds__(512, 1) void brp_kernel(BrArgs A, const c64* __rest
             for (int s = 0; s < R; s++) fb[s * 64 + lane] = x[s];
         }
         __syncthreads();
-        if (active) {
+        {
+            // y_0 / 2 and y_1 / 2 -> u_{j'} = y0 + y1 (quarters 0, 2), u_{j'+512} =
+            // (y0 - y1) e^{-i pi/4} (quarters 1, 3); ACC_c += result
+            // (lwe-bootstrapping-functions-fft.cpp:67-76) for the own and the partner's quarters
             const c64* o = s_fft[w ^ 1];
-            c64 ov[R];
-#pragma unroll
-            for (int a = 0; a < R; a++) ov[a] = o[a * 64 + lane];
-            if (h == 0) P::pair_inv<0>(x, ov);
-            else P::pair_inv<1>(x, ov);
-            // ACC_c += result (lwe-bootstrapping-functions-fft.cpp:67-76)
 #pragma unroll
             for (int a = 0; a < R; a++) {
-                acc[a] += (uint32_t)round_mod32(x[a].re);
-                acc[R + a] += (uint32_t)round_mod32(x[a].im);
+                const c64 ya = x[a], yb = o[a * 64 + lane];
+                const c64 y0 = h == 0 ? ya : yb, y1 = h == 0 ? yb : ya;
+                const c64 u0 = cadd(y0, y1);
+                const c64 dd = csub(y0, y1);
+                const c64 u1 = c64{P::SQ * (dd.re + dd.im), P::SQ * (dd.im - dd.re)};
+                const c64 uo = h == 0 ? u0 : u1, up = h == 0 ? u1 : u0;
+                acc[a] += (uint32_t)round_mod32(uo.re);
+                acc[R + a] += (uint32_t)round_mod32(uo.im);
+                pacc[a] = (uint32_t)acc_in[a * 64 + lane] + (uint32_t)round_mod32(up.re);
+                pacc[R + a] = (uint32_t)acc_in[(R + a) * 64 + lane] + (uint32_t)round_mod32(up.im);
             }
         }
     }
@@ -1091,11 +1152,11 @@ __global__ __launch_bounds__(512, 1) void brp_kernel(BrArgs A, const c64* __rest
     int32_t* e = A.ext + (size_t)job * (N + 1);
     if (c == 0) {
 #pragma unroll
-        for (int hh = 0; hh < 2; hh++)
+        for (int q = 0; q < 2; q++)
 #pragma unroll
             for (int a = 0; a < R; a++) {
-                const int j = 512 * h + 64 * a + lane + 1024 * hh;
-                const uint32_t v = acc[hh * R + a];
+                const int j = 512 * (h + 2 * q) + 64 * a + lane;
+                const uint32_t v = acc[q * R + a];
                 e[j == 0 ? 0 : N - j] = (int32_t)(j == 0 ? v : 0u - v);
             }
     } else if (h == 0 && lane == 0) {
@@ -1810,7 +1871,7 @@ tfhe_amd_device_key* engine_key(const LweBootstrappingKeyFFT* bf0) {
 
 static void free_key(Ctx& C, tfhe_amd_device_key* dk) {
     if (!dk) return;
-    if (C.stream) hipStreamSynchronize(C.stream);
+    if (C.stream) (void)hipStreamSynchronize(C.stream);
     (void)hipFree(dk->bkf);
     (void)hipFree(dk->ksk);
     (void)hipFree(dk->tw);

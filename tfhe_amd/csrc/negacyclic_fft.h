@@ -50,6 +50,8 @@ TA_HD c64 csub(c64 a, c64 b) { return {a.re - b.re, a.im - b.im}; }
 TA_HD c64 cmul(c64 a, c64 b) { return {fma(a.re, b.re, -(a.im * b.im)), fma(a.re, b.im, a.im * b.re)}; }
 // a * conj(b)
 TA_HD c64 cmulc(c64 a, c64 b) { return {fma(a.re, b.re, a.im * b.im), fma(a.im, b.re, -(a.re * b.im))}; }
+// cmac into a zero accumulator, the same roundings: {fma(-a.im, b.im, a.re b.re), fma(a.im, b.re, a.re b.im)}
+TA_HD c64 cmul_first(c64 a, c64 b) { return {fma(-a.im, b.im, a.re * b.re), fma(a.im, b.re, a.re * b.im)}; }
 // acc += a * b
 TA_HD void cmac(c64& acc, c64 a, c64 b) {
     acc.re = fma(a.re, b.re, acc.re);
