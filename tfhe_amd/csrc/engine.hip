@@ -556,19 +556,29 @@ __global__ __launch_bounds__(2 * L * 64, 1) void brl_kernel(BrArgs A, const int3
     __syncthreads();
 
     const int32_t* accc = s_acc[c];
-    for (int i = 0; i < n; i++) {
-        const int abar = s_bara[i];
-        if (abar == 0) continue;  // lwe-bootstrapping-functions-fft.cpp:101 (uniform per workgroup)
-        // key row (i, c*l + p), both output components, into registers
-        c64 k0[R], k1[R];
-        {
-            const c64* q = A.bkf + ((size_t)i * KPL + c * L + p) * 2 * M + lane;
+    // rows whose rotation is zero are skipped (lwe-bootstrapping-functions-fft.cpp:101;
+    // uniform per workgroup)
+    auto next_row = [&](int from) {
+        while (from < n && s_bara[from] == 0) from++;
+        return from;
+    };
+    // key row (i, c*l + p), both output components, into registers; row i + 1's is
+    // requested as soon as row i's products are formed, so its latency runs under
+    // the rest of the row
+    c64 k0[R], k1[R];
+    auto load_key = [&](int row) {
+        const c64* q = A.bkf + ((size_t)row * KPL + c * L + p) * 2 * M + lane;
 #pragma unroll
-            for (int s = 0; s < R; s++) {
-                k0[s] = q[s * 64];
-                k1[s] = q[M + s * 64];
-            }
+        for (int s = 0; s < R; s++) {
+            k0[s] = q[s * 64];
+            k1[s] = q[M + s * 64];
         }
+    };
+    int i = next_row(0);
+    if (i < n) load_key(i);
+    while (i < n) {
+        const int abar = s_bara[i];
+        const int inext = next_row(i + 1);
         // T = (X^abar - 1) * ACC_c + gadget offset, then digit p
         // (tgsw-functions.cpp:337-348)
         uint32_t tv[2 * R];
@@ -604,6 +614,7 @@ __global__ __launch_bounds__(2 * L * 64, 1) void brl_kernel(BrArgs A, const int3
             fb[s * 64 + lane] = cmul(x[s], k0[s]);
             s_par[w][s * 64 + lane] = cmul(x[s], k1[s]);
         }
+        if (inext < n) load_key(inext);
         __syncthreads();  // all products written
         c64 y[R];
         if (p == 0) {
@@ -636,6 +647,7 @@ __global__ __launch_bounds__(2 * L * 64, 1) void brl_kernel(BrArgs A, const int3
             }
         }
         __syncthreads();  // ACC updated for the next row
+        i = inext;
     }
     if (w != 0) return;
     // tLweExtractLweSampleIndex(.., 0, ..), libtfhe/lwe.cpp:41-56
