@@ -107,6 +107,71 @@ struct EmuPair {
     }
 };
 
+
+// N = 1024 on a pair of wavefronts, 256 points each (FftHalf): X[h*64 + lane][slot]
+struct EmuHalf {
+    using H = FftHalf;
+    std::vector<c64> tw;
+    EmuHalf() : tw(HALF_TW_COUNT) { build_twiddles_half(tw.data()); }
+    const c64* t(int h, int lane) const { return tw.data() + (h * 64 + lane) * HALF_TW_PER_LANE; }
+    void forward(const double* p, c64 (*x)[4]) {  // x = X[2*64][4]
+        for (int h = 0; h < 2; h++) {
+            std::vector<c64> lds(H::LDS_C64);
+            c64 (*xh)[4] = x + 64 * h;
+            for (int b = 0; b < 64; b++)
+                for (int a = 0; a < 4; a++) {
+                    const int j = b + 64 * a;
+                    const c64 lo{p[j], p[j + 512]}, hi{p[j + 256], p[j + 768]};
+                    xh[b][a] = h == 0 ? H::pair_fwd<0>(lo, hi) : H::pair_fwd<1>(lo, hi);
+                }
+            for (int l = 0; l < 64; l++) {
+                if (h == 0) H::fwd1<0>(xh[l], l, lds.data(), t(0, l));
+                else H::fwd1<1>(xh[l], l, lds.data(), t(1, l));
+            }
+            for (int l = 0; l < 64; l++) H::fwd2(xh[l], l, lds.data(), t(h, l));
+            for (int l = 0; l < 64; l++) H::fwd2w(xh[l], l, lds.data());
+            for (int l = 0; l < 64; l++) H::fwd3(xh[l], l, lds.data(), t(h, l));
+            for (int l = 0; l < 64; l++) H::fwd3w(xh[l], l, lds.data());
+            for (int l = 0; l < 64; l++) H::fwd4(xh[l], l, lds.data());
+        }
+    }
+    void inverse(c64 (*x)[4], double* p) {
+        for (int h = 0; h < 2; h++) {
+            std::vector<c64> lds(H::LDS_C64);
+            c64 (*xh)[4] = x + 64 * h;
+            // the first step written into two buffers as halves of the spectrum, then
+            // summed by inv2_sum (as the latency kernel sums its products)
+            std::vector<c64> part(H::LDS_C64);
+            for (int l = 0; l < 64; l++) {
+                c64 half1[4], half2[4];
+                for (int k = 0; k < 4; k++) {
+                    half1[k] = c64{0.5 * xh[l][k].re, 0.5 * xh[l][k].im};
+                    half2[k] = half1[k];
+                }
+                H::inv1(half1, l, lds.data());
+                H::inv1(half2, l, part.data());
+            }
+            const c64* src[2] = {lds.data(), part.data()};
+            for (int l = 0; l < 64; l++) H::inv2_sum(xh[l], l, src, 2, t(h, l));
+            for (int l = 0; l < 64; l++) H::inv2w(xh[l], l, lds.data());
+            for (int l = 0; l < 64; l++) H::inv3(xh[l], l, lds.data(), t(h, l));
+            for (int l = 0; l < 64; l++) H::inv3w(xh[l], l, lds.data());
+            for (int l = 0; l < 64; l++) {
+                if (h == 0) H::inv4<0>(xh[l], l, lds.data(), t(0, l));
+                else H::inv4<1>(xh[l], l, lds.data(), t(1, l));
+            }
+        }
+        for (int b = 0; b < 64; b++)
+            for (int a = 0; a < 4; a++) {
+                const c64 y0 = x[b][a], y1 = x[64 + b][a];
+                const c64 lo = H::pair_inv_lo(y0, y1), hi = H::pair_inv_hi(y0, y1);
+                const int j = b + 64 * a;
+                p[j] = lo.re, p[j + 512] = lo.im;
+                p[j + 256] = hi.re, p[j + 768] = hi.im;
+            }
+    }
+};
+
 template <class Em, int M, int SLOTS>
 int run_with(Em& E, int trials, unsigned seed, int dbits);
 
@@ -179,6 +244,10 @@ int main(int argc, char** argv) {
     if (N == -2048) {  // pair-of-wavefronts transform
         EmuPair E;
         return run_with<EmuPair, 1024, 8>(E, trials, seed, dbits);
+    }
+    if (N == -1024) {  // pair of wavefronts, 256 points each
+        EmuHalf E;
+        return run_with<EmuHalf, 512, 4>(E, trials, seed, dbits);
     }
     fprintf(stderr, "N must be 1024 or 2048\n");
     return 2;

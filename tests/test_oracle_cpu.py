@@ -250,7 +250,7 @@ def fft_emu(tmp_path_factory):
     return exe
 
 
-@pytest.mark.parametrize("N,bits", [(1024, 7), (2048, 10), (-2048, 10)])
+@pytest.mark.parametrize("N,bits", [(1024, 7), (2048, 10), (-2048, 10), (-1024, 7)])
 def test_wavefront_transform_emulation(fft_emu, N, bits):
     """The GPU transform's per-lane code, run lane by lane on the CPU, gives the
     exact negacyclic product after rounding (digit x torus polynomials)."""

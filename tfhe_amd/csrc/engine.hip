@@ -42,6 +42,8 @@ struct tfhe_amd_device_key {
     c64* tw;       // TwiddleTable<M>::COUNT
     c64* bkp;      // M = 1024 only: [n][kpl][k+1][2][512], the pair-of-wavefronts transform (FftPair)
     c64* twp;      // M = 1024 only: PAIR_TW_COUNT
+    c64* bkh;      // M = 512 only: [n][kpl][k+1][2][256], the half transform (FftHalf) for the latency kernel
+    c64* twh;      // M = 512 only: HALF_TW_COUNT
     int8_t* kp;    // t = 8, basebit = 2: keyswitch key as signed byte planes [4][k*N][npad][32] (int8 matrix cores)
     size_t bytes;
 };
@@ -488,33 +490,39 @@ constexpr int BRL_NMAX = 1024;
 // largest LWE dimension for which the autotuner compares the N = 1024 kernels
 constexpr int BR4_ANY_NMAX = 768;
 
-// Blind rotation for latency (M = 512): one bootstrap per workgroup of 2l
-// wavefronts, wavefront (c, p) owning digit polynomial p of accumulator
-// component c. Per key row every wavefront rotates and decomposes its
-// component (accumulator in LDS), transforms its one digit polynomial and
-// multiplies it by its key row (c*l + p), held in registers and loaded at the
-// start of the row so the load runs under the transform; the 2l products for
-// each output component are summed by wavefront (c, 0), which runs the
-// inverse transform and updates ACC_c. Per row a bootstrap costs one forward
-// and one inverse transform of latency instead of l + 1.
+// Blind rotation for latency with the transforms split across pairs of
+// wavefronts (M = 512, FftHalf): one bootstrap per workgroup of 4l
+// wavefronts, wavefront w = h + 2c + 4p owning half h of the transform of
+// digit polynomial p of accumulator component c. Per key row every wavefront
+// decomposes its component (the accumulator and its rotation live in LDS), forms
+// its half y_h from the whole digit polynomial, runs its 256-point transform
+// and multiplies the result by its half of key row (c*l + p), held in
+// registers and requested a row ahead. Wavefronts (c, 0, h), one per SIMD,
+// sum the 2l products of output component c for half h, run the inverse
+// 256-point transform, exchange halves with their partner and update their
+// half of ACC_c. The rotated accumulator is formed once per row by the whole
+// workgroup. Four workgroup barriers per row; the forward work is spread
+// evenly over the SIMDs (three half transforms each for l = 3).
 template <int L>
-__global__ __launch_bounds__(2 * L * 64, 1) void brl_kernel(BrArgs A, const int32_t* __restrict__ mux_gate,
+__global__ __launch_bounds__(4 * L * 64, 1) void brl_kernel(BrArgs A, const c64* __restrict__ bkh,
+                                                           const c64* __restrict__ twh,
+                                                           const int32_t* __restrict__ mux_gate,
                                                            const int32_t* __restrict__ n_jobs) {
-    constexpr int M = 512;
-    using F = Fft<M>;
-    constexpr int R = F::R, N = 2 * M, KPL = 2 * L, W = 2 * L;
+    using H = FftHalf;
+    constexpr int R = H::R, N = 1024, KPL = 2 * L, W = 4 * L;
     constexpr int LOG2N2 = 11;
-    __shared__ c64 s_fft[W][F::LDS_C64];  // transform buffer; [0, M) then holds the product for output 0
-    __shared__ c64 s_par[W][M];           // product for output 1
+    __shared__ c64 s_fft[W][H::LDS_C64];  // transform buffer, then the product for output 0
+    __shared__ c64 s_par[W][H::LDS_C64];  // product for output 1
+    __shared__ c64 s_inv[4][H::LDS_C64];  // inverse transforms of wavefronts (c, 0, h)
     __shared__ int32_t s_acc[2][N];
-    __shared__ c64 s_tw1[R * 64];
+    __shared__ uint32_t s_tv[2][N];  // (X^abar - 1) * ACC + gadget offset, both components
     __shared__ int16_t s_bara[BRL_NMAX];
     __shared__ int32_t s_barb;
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int w = tid >> 6;
-    const int c = w / L, p = w % L;
+    const int h = w & 1, c = (w >> 1) & 1, p = w >> 2;
     const int job = blockIdx.x;
     if (job >= *n_jobs) return;  // uniform per workgroup
     const int n = A.n;
@@ -538,40 +546,47 @@ __global__ __launch_bounds__(2 * L * 64, 1) void brl_kernel(BrArgs A, const int3
             else s_barb = (int32_t)bar;
         }
     }
-    for (int q = tid; q < R * 64; q += W * 64) s_tw1[q] = A.tw[(q & 63) * (R + 8) + (q >> 6)];
-    const c64* tw1 = s_tw1 + lane;
-    const c64 tw2 = A.tw[lane * (R + 8) + R + 1];
+    // this lane's twiddles: tw1[0..3], w_64^{g mu}, w_16^{g0 nu}
+    c64 tw[HALF_TW_PER_LANE];
+#pragma unroll
+    for (int k = 0; k < HALF_TW_PER_LANE; k++) tw[k] = twh[(h * 64 + lane) * HALF_TW_PER_LANE + k];
     const uint32_t offset = A.offset;
     const int Bgbit = A.Bgbit;
     const int32_t dhalf = 1 << (Bgbit - 1);
     __syncthreads();
-    {
-        // ACC = (0, X^{2N-barb} * [mu...mu]), lwe-bootstrapping-functions-fft.cpp:146-148
-        const int barb = s_barb;
-        for (int j = tid; j < N; j += W * 64) {
-            s_acc[0][j] = 0;
-            s_acc[1][j] = (((j + barb) & (2 * N - 1)) < N) ? A.mu : -A.mu;
-        }
+    // ACC = (0, X^{2N-barb} * [mu...mu]), lwe-bootstrapping-functions-fft.cpp:146-148.
+    // Wavefront (c, 0, h) owns coefficients lane + 64 a + 256 h (+ 512) of ACC_c in
+    // registers (acc[a], acc[R + a]); s_acc is the copy the rotation reads.
+    const int barb = s_barb;
+    auto acc_init = [&](int cc, int j) { return cc == 0 ? 0 : ((((j + barb) & (2 * N - 1)) < N) ? A.mu : -A.mu); };
+    for (int j = tid; j < N; j += W * 64) {
+        s_acc[0][j] = acc_init(0, j);
+        s_acc[1][j] = acc_init(1, j);
+    }
+    uint32_t acc[2 * R];
+#pragma unroll
+    for (int a = 0; a < R; a++) {
+        const int j = lane + 64 * a + 256 * h;
+        acc[a] = (uint32_t)acc_init(c, j);
+        acc[R + a] = (uint32_t)acc_init(c, j + 512);
     }
     __syncthreads();
 
-    const int32_t* accc = s_acc[c];
     // rows whose rotation is zero are skipped (lwe-bootstrapping-functions-fft.cpp:101;
     // uniform per workgroup)
     auto next_row = [&](int from) {
         while (from < n && s_bara[from] == 0) from++;
         return from;
     };
-    // key row (i, c*l + p), both output components, into registers; row i + 1's is
-    // requested as soon as row i's products are formed, so its latency runs under
-    // the rest of the row
+    // half h of key row (i, c*l + p) for both output components, into registers;
+    // row i + 1's is requested as soon as row i's products are formed
     c64 k0[R], k1[R];
     auto load_key = [&](int row) {
-        const c64* q = A.bkf + ((size_t)row * KPL + c * L + p) * 2 * M + lane;
+        const c64* q = bkh + ((((size_t)row * KPL + c * L + p) * 2) * 2 + h) * 256 + lane;
 #pragma unroll
         for (int s = 0; s < R; s++) {
             k0[s] = q[s * 64];
-            k1[s] = q[M + s * 64];
+            k1[s] = q[512 + s * 64];
         }
     };
     int i = next_row(0);
@@ -579,72 +594,105 @@ __global__ __launch_bounds__(2 * L * 64, 1) void brl_kernel(BrArgs A, const int3
     while (i < n) {
         const int abar = s_bara[i];
         const int inext = next_row(i + 1);
-        // T = (X^abar - 1) * ACC_c + gadget offset, then digit p
-        // (tgsw-functions.cpp:337-348)
-        uint32_t tv[2 * R];
+        // T = (X^abar - 1) * ACC + gadget offset, both components, once for the workgroup
+        {
+            constexpr int PER = (2 * N + W * 64 - 1) / (W * 64);
+            uint32_t rv[PER], av[PER];
 #pragma unroll
-        for (int qq = 0; qq < 2 * R; qq++) {
-            const int j = 64 * (qq % R) + lane + (qq / R) * M;
-            tv[qq] = (uint32_t)accc[(j - abar) & (N - 1)];
-        }
+            for (int u = 0; u < PER; u++) {
+                const int j2 = tid + u * W * 64, cc = (j2 >> 10) & 1, j = j2 & (N - 1);
+                rv[u] = (uint32_t)s_acc[cc][(j - abar) & (N - 1)];
+                av[u] = (uint32_t)s_acc[cc][j];
+            }
 #pragma unroll
-        for (int qq = 0; qq < 2 * R; qq++) {
-            const int j = 64 * (qq % R) + lane + (qq / R) * M;
-            const uint32_t rot = ((j - abar) & N) ? 0u - tv[qq] : tv[qq];
-            tv[qq] = rot - (uint32_t)accc[j] + offset;
+            for (int u = 0; u < PER; u++) {
+                const int j2 = tid + u * W * 64, cc = (j2 >> 10) & 1, j = j2 & (N - 1);
+                if (j2 < 2 * N) s_tv[cc][j] = ((((j - abar) & (2 * N - 1)) & N) ? 0u - rv[u] : rv[u]) - av[u] + offset;
+            }
         }
+        __syncthreads();
+        // digit p of component c (tgsw-functions.cpp:337-348), coefficient
+        // j = lane + 64 a + 256 q in tv[q * R + a]
+        uint32_t tv[4 * R];
+#pragma unroll
+        for (int k = 0; k < 4 * R; k++) tv[k] = s_tv[c][64 * k + lane];
         const int shift = 32 - (p + 1) * Bgbit;
         c64 x[R];
 #pragma unroll
-        for (int s = 0; s < R; s++) {
-            const int32_t d0 = (int32_t)__builtin_amdgcn_ubfe(tv[s], shift, Bgbit) - dhalf;
-            const int32_t d1 = (int32_t)__builtin_amdgcn_ubfe(tv[R + s], shift, Bgbit) - dhalf;
-            x[s] = c64{(double)d0, (double)d1};
-        }
-        F::template fwd1<64>(x, lane, fb, tw1);
-        wave_sync();
-        F::fwd2(x, lane, fb, tw2);
-        wave_sync();
-        F::fwd3(x, lane, fb);
-        wave_sync();
-        F::fwd4(x, lane, fb);
-        wave_sync();
+        for (int a = 0; a < R; a++) {
+            // u_{j'} = d_{j'} + i d_{j'+512}, u_{j'+256} = d_{j'+256} + i d_{j'+768}, j' = lane + 64 a
+            double d[4];
 #pragma unroll
-        for (int s = 0; s < R; s++) {
-            fb[s * 64 + lane] = cmul(x[s], k0[s]);
-            s_par[w][s * 64 + lane] = cmul(x[s], k1[s]);
+            for (int q = 0; q < 4; q++) d[q] = (double)((int32_t)__builtin_amdgcn_ubfe(tv[q * R + a], shift, Bgbit) - dhalf);
+            const c64 lo{d[0], d[2]}, hi{d[1], d[3]};
+            x[a] = h == 0 ? H::pair_fwd<0>(lo, hi) : H::pair_fwd<1>(lo, hi);
+        }
+        if (h == 0) H::fwd1<0>(x, lane, fb, tw);
+        else H::fwd1<1>(x, lane, fb, tw);
+        wave_sync();
+        H::fwd2(x, lane, fb, tw);
+        wave_sync();
+        H::fwd2w(x, lane, fb);
+        wave_sync();
+        H::fwd3(x, lane, fb, tw);
+        wave_sync();
+        H::fwd3w(x, lane, fb);
+        wave_sync();
+        H::fwd4(x, lane, fb);
+        wave_sync();
+        {
+            // the products, after the inverse transform's first (linear) step
+            c64 y0[R], y1[R];
+#pragma unroll
+            for (int s = 0; s < R; s++) {
+                y0[s] = cmul(x[s], k0[s]);
+                y1[s] = cmul(x[s], k1[s]);
+            }
+            H::inv1(y0, lane, fb);
+            H::inv1(y1, lane, s_par[w]);
         }
         if (inext < n) load_key(inext);
         __syncthreads();  // all products written
-        c64 y[R];
         if (p == 0) {
-            // output component c: sum of the 2l products
+            // output component c, half h: the 2l products of the wavefronts (c', p', h)
+            const c64* src[2 * L];
 #pragma unroll
-            for (int s = 0; s < R; s++) y[s] = c64{0.0, 0.0};
-#pragma unroll
-            for (int v = 0; v < W; v++) {
-                const c64* src = c ? s_par[v] : s_fft[v];
-#pragma unroll
-                for (int s = 0; s < R; s++) y[s] = cadd(y[s], src[s * 64 + lane]);
+            for (int v = 0; v < 2 * L; v++) {
+                const int wv = h + 2 * (v & 1) + 4 * (v >> 1);
+                src[v] = c ? s_par[wv] : s_fft[wv];
             }
-        }
-        __syncthreads();  // products read before the inverse transform reuses the buffers
-        if (p == 0) {
-            F::inv1(y, lane, fb);
+            c64 y[R];
+            c64* ib = s_inv[w];
+            H::inv2_sum(y, lane, src, 2 * L, tw);
             wave_sync();
-            F::inv2(y, lane, fb, tw2);
+            H::inv2w(y, lane, ib);
             wave_sync();
-            F::inv3(y, lane, fb);
+            H::inv3(y, lane, ib, tw);
             wave_sync();
-            F::template inv4<64>(y, lane, fb, tw1);
-            // ACC_c += result (lwe-bootstrapping-functions-fft.cpp:67-76)
-            int32_t* acc = s_acc[c];
+            H::inv3w(y, lane, ib);
+            wave_sync();
+            if (h == 0) H::inv4<0>(y, lane, ib, tw);
+            else H::inv4<1>(y, lane, ib, tw);
+            wave_sync();
 #pragma unroll
-            for (int s = 0; s < R; s++) {
-                const int j = 64 * s + lane;
-                acc[j] = (int32_t)((uint32_t)acc[j] + (uint32_t)round_mod32(y[s].re));
-                acc[j + M] = (int32_t)((uint32_t)acc[j + M] + (uint32_t)round_mod32(y[s].im));
+            for (int s = 0; s < R; s++) ib[s * 64 + lane] = y[s];
+            __syncthreads();  // both halves of each component written
+            // ACC_c += result (lwe-bootstrapping-functions-fft.cpp:67-76): half h holds
+            // coefficients j' + 256 h and j' + 256 h + 512, j' = lane + 64 a
+            const c64* o = s_inv[w ^ 1];
+            int32_t* sa = s_acc[c];
+#pragma unroll
+            for (int a = 0; a < R; a++) {
+                const c64 yo = o[a * 64 + lane];
+                const c64 u = h == 0 ? H::pair_inv_lo(y[a], yo) : H::pair_inv_hi(yo, y[a]);
+                const int j = lane + 64 * a + 256 * h;
+                acc[a] += (uint32_t)round_mod32(u.re);
+                acc[R + a] += (uint32_t)round_mod32(u.im);
+                sa[j] = (int32_t)acc[a];
+                sa[j + 512] = (int32_t)acc[R + a];
             }
+        } else {
+            __syncthreads();
         }
         __syncthreads();  // ACC updated for the next row
         i = inext;
@@ -654,6 +702,43 @@ __global__ __launch_bounds__(2 * L * 64, 1) void brl_kernel(BrArgs A, const int3
     int32_t* e = A.ext + (size_t)job * (N + 1);
     for (int j = lane; j < N; j += 64) e[j == 0 ? 0 : N - j] = j == 0 ? s_acc[0][0] : (int32_t)(0u - (uint32_t)s_acc[0][j]);
     if (lane == 0) e[N] = s_acc[1][0];
+}
+
+// Half-transform copy of every bootstrapping-key polynomial (N = 1024): one
+// 128-thread workgroup per polynomial, output [poly][h][slot][lane].
+__global__ __launch_bounds__(128) void bk_convert_half_kernel(const int32_t* __restrict__ bk, c64* __restrict__ bkh,
+                                                              const c64* __restrict__ twh) {
+    using H = FftHalf;
+    constexpr int R = H::R;
+    __shared__ c64 s_fft[2][H::LDS_C64];
+    const int lane = threadIdx.x & 63, h = threadIdx.x >> 6;
+    const size_t poly = blockIdx.x;
+    const int32_t* p = bk + poly * 1024;
+    c64* fb = s_fft[h];
+    c64 tw[HALF_TW_PER_LANE], x[R];
+#pragma unroll
+    for (int k = 0; k < HALF_TW_PER_LANE; k++) tw[k] = twh[(h * 64 + lane) * HALF_TW_PER_LANE + k];
+#pragma unroll
+    for (int a = 0; a < R; a++) {
+        const int j = lane + 64 * a;
+        const c64 lo{(double)p[j], (double)p[j + 512]}, hi{(double)p[j + 256], (double)p[j + 768]};
+        x[a] = h == 0 ? H::pair_fwd<0>(lo, hi) : H::pair_fwd<1>(lo, hi);
+    }
+    if (h == 0) H::fwd1<0>(x, lane, fb, tw);
+    else H::fwd1<1>(x, lane, fb, tw);
+    wave_sync();
+    H::fwd2(x, lane, fb, tw);
+    wave_sync();
+    H::fwd2w(x, lane, fb);
+    wave_sync();
+    H::fwd3(x, lane, fb, tw);
+    wave_sync();
+    H::fwd3w(x, lane, fb);
+    wave_sync();
+    H::fwd4(x, lane, fb);
+    c64* out = bkh + (poly * 2 + h) * 256;
+#pragma unroll
+    for (int s = 0; s < R; s++) out[s * 64 + lane] = x[s];
 }
 
 // largest LWE dimension the two-wavefront kernel's LDS table holds
@@ -1479,7 +1564,7 @@ struct Ctx {
     void* h_pinned = nullptr;
     size_t h_cap = 0;
     int br_mode = 0;
-    int brl_max = 256;        // automatic mode: batches up to this many gates use 2l wavefronts per bootstrap
+    int brl_max = 256;        // automatic mode: batches up to this many gates use the latency kernel (4l wavefronts per bootstrap)
     int br_small_max = 1024;  // automatic mode: batches up to this many gates use two wavefronts per bootstrap
     // optional per-launch timing with HIP events on the launch stream
     bool prof = false;
@@ -1685,7 +1770,7 @@ int gates_device_locked(Ctx& C, tfhe_amd_device_key* dk, int count, const int32_
         HIP_TRY(hipEventRecord(ev.e[0], s));
     }
     // kernel choice: 1 = one wavefront per bootstrap, 2 = four bootstraps per workgroup,
-    // 3 = two wavefronts per bootstrap, 5 = 2l wavefronts per bootstrap; 0 = automatic
+    // 3 = two wavefronts per bootstrap, 5 = 4l wavefronts per bootstrap (half transforms); 0 = automatic
     // (5 for the smallest batches, 3 while the batch leaves CUs idle, 2 for large
     // batches). N = 2048 uses the pair-transform kernel (4) unless mode 1 is forced.
     int mode = C.br_mode;
@@ -1697,10 +1782,11 @@ int gates_device_locked(Ctx& C, tfhe_amd_device_key* dk, int count, const int32_
         else mode = (count <= C.br_small_max && dk->P.n <= BR2_NMAX) ? 3 : 2;
     }
     if (mode == 5 && (dk->M != 512 || dk->P.n > BRL_NMAX)) mode = 3;
+    if (mode == 5 && !dk->bkh) mode = 3;
     if (mode == 5) {
         const int blocks = 2 * count;  // one bootstrap per workgroup, at most 2*count jobs
-        if (dk->P.l == 3) hipLaunchKernelGGL(brl_kernel<3>, dim3(blocks), dim3(384), 0, s, A, mux_gate, n_jobs);
-        else hipLaunchKernelGGL(brl_kernel<2>, dim3(blocks), dim3(256), 0, s, A, mux_gate, n_jobs);
+        if (dk->P.l == 3) hipLaunchKernelGGL(brl_kernel<3>, dim3(blocks), dim3(768), 0, s, A, dk->bkh, dk->twh, mux_gate, n_jobs);
+        else hipLaunchKernelGGL(brl_kernel<2>, dim3(blocks), dim3(512), 0, s, A, dk->bkh, dk->twh, mux_gate, n_jobs);
     } else if (mode == 4) {
         const int blocks = count;  // two bootstraps per workgroup, at most 2*count jobs
         if (dk->P.l == 2) hipLaunchKernelGGL(brp_kernel<2>, dim3(blocks), dim3(512), 0, s, A, dk->bkp, dk->twp, mux_gate, n_jobs);
@@ -1786,6 +1872,8 @@ static int upload_key(Ctx& C, const LweBootstrappingKeyFFT* bf, tfhe_amd_device_
     dk->bytes = bkf_bytes + ksk_bytes + twc * sizeof(c64);
     dk->bkp = nullptr;
     dk->twp = nullptr;
+    dk->bkh = nullptr;
+    dk->twh = nullptr;
     dk->kp = nullptr;
     auto fail = [&](int rc) {
         (void)hipFree(dk->bkf);
@@ -1793,6 +1881,8 @@ static int upload_key(Ctx& C, const LweBootstrappingKeyFFT* bf, tfhe_amd_device_
         (void)hipFree(dk->tw);
         (void)hipFree(dk->bkp);
         (void)hipFree(dk->twp);
+        (void)hipFree(dk->bkh);
+        (void)hipFree(dk->twh);
         (void)hipFree(dk->kp);
         delete dk;
         return rc;
@@ -1807,6 +1897,11 @@ static int upload_key(Ctx& C, const LweBootstrappingKeyFFT* bf, tfhe_amd_device_
         if (hipMalloc(&dk->bkp, bkf_bytes) != hipSuccess || hipMalloc(&dk->twp, PAIR_TW_COUNT * sizeof(c64)) != hipSuccess)
             return fail(set_error("hipMalloc(pair-transform key, %zu bytes) failed", bkf_bytes));
         dk->bytes += bkf_bytes + PAIR_TW_COUNT * sizeof(c64);
+    }
+    if (M == 512 && k == 1 && n <= BRL_NMAX) {
+        if (hipMalloc(&dk->bkh, bkf_bytes) != hipSuccess || hipMalloc(&dk->twh, HALF_TW_COUNT * sizeof(c64)) != hipSuccess)
+            return fail(set_error("hipMalloc(half-transform key, %zu bytes) failed", bkf_bytes));
+        dk->bytes += bkf_bytes + HALF_TW_COUNT * sizeof(c64);
     }
     if (hipMalloc(&dk->bkf, bkf_bytes) != hipSuccess) return fail(set_error("hipMalloc(bootstrapping key, %zu bytes) failed", bkf_bytes));
     if (hipMalloc(&dk->ksk, ksk_bytes) != hipSuccess) return fail(set_error("hipMalloc(keyswitch key, %zu bytes) failed", ksk_bytes));
@@ -1843,12 +1938,22 @@ static int upload_key(Ctx& C, const LweBootstrappingKeyFFT* bf, tfhe_amd_device_
         build_twiddles_pair(twp.data());
         if (e == hipSuccess) e = hipMemcpyAsync(dk->twp, twp.data(), PAIR_TW_COUNT * sizeof(c64), hipMemcpyHostToDevice, C.stream);
     }
+    std::vector<c64> twh;
+    if (dk->twh) {
+        twh.resize(HALF_TW_COUNT);
+        build_twiddles_half(twh.data());
+        if (e == hipSuccess) e = hipMemcpyAsync(dk->twh, twh.data(), HALF_TW_COUNT * sizeof(c64), hipMemcpyHostToDevice, C.stream);
+    }
     if (e == hipSuccess) {
         if (M == 512) hipLaunchKernelGGL(bk_convert_kernel<512>, dim3((unsigned)npoly), dim3(64), 0, C.stream, d_coef, dk->bkf, dk->tw);
         else hipLaunchKernelGGL(bk_convert_kernel<1024>, dim3((unsigned)npoly), dim3(64), 0, C.stream, d_coef, dk->bkf, dk->tw);
         e = hipGetLastError();
         if (e == hipSuccess && dk->bkp) {
             hipLaunchKernelGGL(bk_convert_pair_kernel, dim3((unsigned)npoly), dim3(128), 0, C.stream, d_coef, dk->bkp, dk->twp);
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess && dk->bkh) {
+            hipLaunchKernelGGL(bk_convert_half_kernel, dim3((unsigned)npoly), dim3(128), 0, C.stream, d_coef, dk->bkh, dk->twh);
             e = hipGetLastError();
         }
         if (e == hipSuccess && dk->kp) {
@@ -1889,6 +1994,8 @@ static void free_key(Ctx& C, tfhe_amd_device_key* dk) {
     (void)hipFree(dk->tw);
     (void)hipFree(dk->bkp);
     (void)hipFree(dk->twp);
+    (void)hipFree(dk->bkh);
+    (void)hipFree(dk->twh);
     (void)hipFree(dk->kp);
     delete dk;
 }

@@ -360,6 +360,188 @@ struct FftPair {
     }
 };
 
+// Negacyclic transform for N = 1024 on a PAIR of wavefronts, 256 points each
+// (four slots per lane), for the latency kernel. With u_j = p_j + i p_{j+512}
+// (j < 512) and X_m = sum_j u_j psi^j w_512^{jm} (psi = exp(i pi/1024)),
+// splitting j = j' + 256 t gives
+//   y_h[j'] = u_{j'} + (-1)^h e^{i pi/4} u_{j'+256},            j' < 256,
+//   X_{2K+h} = sum_{j'} y_h[j'] psi^{(1+4h) j'} w_256^{j' K},
+// so wavefront h forms y_h from the whole polynomial (no exchange with its
+// partner) and runs a 256-point DFT. With j' = b + 64a (lane b, slot a) and
+// K = k1 + 4 k2:
+//   stage 1  (in registers)   : twist psi^{(1+4h) 64a}, 4-point DFT over a,
+//                               times tw1[k1] = psi^{(1+4h) b} w_256^{b k1};
+//   then the 64-point DFT over b for each row r = k1, as three radix-4 steps
+//   with b = g + 16 tau, g = g0 + 4 g1, k2 = mu + 4 (nu + 4 omega):
+//   step A, lane (r, g)       : 4-point DFT over tau -> mu, times w_64^{g mu}
+//   step B, lane (r, mu, g0)  : 4-point DFT over g1  -> nu, times w_16^{g0 nu}
+//   step C, lane (r, mu, nu)  : 4-point DFT over g0  -> omega
+// (lane numbers r*16 + g and r*16 + mu*4 + g0 / nu). Slot omega of lane
+// (r, mu, nu) then holds X_{2K+h} with K = r + 4 mu + 16 nu + 64 omega. The
+// inverse runs the steps backwards with negative exponents; the pair's
+// combination u_{j'} = y0 + y1, u_{j'+256} = (y0 - y1) e^{-i pi/4} needs both
+// halves. Each function either computes and writes the LDS buffer or reads it
+// and computes, so the same code runs on the GPU (one wavefront, in order) and
+// in the lane-by-lane CPU emulation of the tests.
+struct FftHalf {
+    static constexpr int R = 4;
+    static constexpr int LDS_C64 = 272;  // 256 elements, one pad every 16
+    static constexpr double SQ = kH;     // e^{i pi/4} = (SQ, SQ)
+    // element e at ix(e) = e + (e >> 4); the per-lane bases below are ix() of
+    // each step's first element, the steps adding compile-time offsets
+    TA_HD static int ix(int e) { return e + (e >> 4); }
+    TA_HD static int base1(int b) { return b + (b >> 4); }                                    // + 68 k1
+    TA_HD static int base2(int lane) { return 68 * (lane >> 4) + (lane & 15); }               // + 17 tau / mu
+    TA_HD static int base3(int lane) { return 68 * (lane >> 4) + 17 * ((lane >> 2) & 3) + (lane & 3); }  // + 4 g1 / nu
+    TA_HD static int base4(int lane) { return 68 * (lane >> 4) + 17 * ((lane >> 2) & 3) + 4 * (lane & 3); }  // + g0
+
+    // natural-order 4-point DFT in place, exponent sign S
+    template <int S>
+    TA_HD static void dft4(c64* x) {
+        const c64 t0 = cadd(x[0], x[2]), t1 = csub(x[0], x[2]);
+        const c64 t2 = cadd(x[1], x[3]), t3 = csub(x[1], x[3]);
+        // i * t3 (S > 0) or -i * t3 (S < 0)
+        const c64 it3 = S > 0 ? c64{-t3.im, t3.re} : c64{t3.im, -t3.re};
+        x[0] = cadd(t0, t2);
+        x[2] = csub(t0, t2);
+        x[1] = cadd(t1, it3);
+        x[3] = csub(t1, it3);
+    }
+    // y_h from u_lo = u_{j'} and u_hi = u_{j'+256}
+    template <int H>
+    TA_HD static c64 pair_fwd(c64 lo, c64 hi) {
+        const c64 r = c64{SQ * (hi.re - hi.im), SQ * (hi.re + hi.im)};  // e^{i pi/4} hi
+        return H == 0 ? cadd(lo, r) : csub(lo, r);
+    }
+    // ---- forward; tw holds tw1[0..3], w_64^{g mu} (mu = 1..3), w_16^{g0 nu} (nu = 1..3) ----
+    template <int H>
+    TA_HD static void fwd1(c64* x, int b, c64* lds, const c64* tw) {
+#pragma unroll
+        for (int a = 1; a < R; a++) x[a] = cmul(x[a], e32(2 * (1 + 4 * H) * a));
+        dft4<1>(x);
+        c64* d = lds + base1(b);
+#pragma unroll
+        for (int k1 = 0; k1 < R; k1++) d[68 * k1] = cmul(x[k1], tw[k1]);
+    }
+    TA_HD static void fwd2(c64* x, int lane, const c64* lds, const c64* tw) {
+        const c64* q = lds + base2(lane);
+#pragma unroll
+        for (int t = 0; t < R; t++) x[t] = q[17 * t];
+        dft4<1>(x);
+#pragma unroll
+        for (int mu = 1; mu < R; mu++) x[mu] = cmul(x[mu], tw[3 + mu]);
+    }
+    TA_HD static void fwd2w(const c64* x, int lane, c64* lds) {
+        c64* d = lds + base2(lane);
+#pragma unroll
+        for (int mu = 0; mu < R; mu++) d[17 * mu] = x[mu];
+    }
+    TA_HD static void fwd3(c64* x, int lane, const c64* lds, const c64* tw) {
+        const c64* q = lds + base3(lane);
+#pragma unroll
+        for (int g1 = 0; g1 < R; g1++) x[g1] = q[4 * g1];
+        dft4<1>(x);
+#pragma unroll
+        for (int nu = 1; nu < R; nu++) x[nu] = cmul(x[nu], tw[6 + nu]);
+    }
+    TA_HD static void fwd3w(const c64* x, int lane, c64* lds) {
+        c64* d = lds + base3(lane);
+#pragma unroll
+        for (int nu = 0; nu < R; nu++) d[4 * nu] = x[nu];
+    }
+    TA_HD static void fwd4(c64* x, int lane, const c64* lds) {
+        const c64* q = lds + base4(lane);
+#pragma unroll
+        for (int g0 = 0; g0 < R; g0++) x[g0] = q[g0];
+        dft4<1>(x);
+    }
+    // ---- inverse (exact inverse of the forward steps, times 256) ----
+    // first step: 4-point DFT over omega, written transposed. It is linear, so a
+    // sum of spectra can also be formed after it (inv2_sum reads several buffers).
+    TA_HD static void inv1(c64* x, int lane, c64* lds) {
+        dft4<-1>(x);
+        c64* d = lds + base4(lane);
+#pragma unroll
+        for (int g0 = 0; g0 < R; g0++) d[g0] = x[g0];
+    }
+    TA_HD static void inv2(c64* x, int lane, const c64* lds, const c64* tw) {
+        const c64* q = lds + base3(lane);
+        x[0] = q[0];
+#pragma unroll
+        for (int nu = 1; nu < R; nu++) x[nu] = cmulc(q[4 * nu], tw[6 + nu]);
+        dft4<-1>(x);
+    }
+    // inv2 on the sum of `count` buffers src[v] (each written by inv1)
+    TA_HD static void inv2_sum(c64* x, int lane, const c64* const* src, int count, const c64* tw) {
+        const int o = base3(lane);
+#pragma unroll
+        for (int nu = 0; nu < R; nu++) x[nu] = src[0][o + 4 * nu];
+        for (int v = 1; v < count; v++)
+#pragma unroll
+            for (int nu = 0; nu < R; nu++) x[nu] = cadd(x[nu], src[v][o + 4 * nu]);
+#pragma unroll
+        for (int nu = 1; nu < R; nu++) x[nu] = cmulc(x[nu], tw[6 + nu]);
+        dft4<-1>(x);
+    }
+    TA_HD static void inv2w(const c64* x, int lane, c64* lds) {
+        c64* d = lds + base3(lane);
+#pragma unroll
+        for (int g1 = 0; g1 < R; g1++) d[4 * g1] = x[g1];
+    }
+    TA_HD static void inv3(c64* x, int lane, const c64* lds, const c64* tw) {
+        const c64* q = lds + base2(lane);
+        x[0] = q[0];
+#pragma unroll
+        for (int mu = 1; mu < R; mu++) x[mu] = cmulc(q[17 * mu], tw[3 + mu]);
+        dft4<-1>(x);
+    }
+    TA_HD static void inv3w(const c64* x, int lane, c64* lds) {
+        c64* d = lds + base2(lane);
+#pragma unroll
+        for (int t = 0; t < R; t++) d[17 * t] = x[t];
+    }
+    // result: y_h[b + 64a] / 2 (the pair's 1/2 and the 1/256 folded in)
+    template <int H>
+    TA_HD static void inv4(c64* x, int b, const c64* lds, const c64* tw) {
+        const c64* q = lds + base1(b);
+#pragma unroll
+        for (int k1 = 0; k1 < R; k1++) x[k1] = cmulc(q[68 * k1], tw[k1]);
+        dft4<-1>(x);
+#pragma unroll
+        for (int a = 0; a < R; a++) {
+            const c64 t = e32(2 * (1 + 4 * H) * a);
+            x[a] = cmul(x[a], c64{t.re / 512.0, -t.im / 512.0});
+        }
+    }
+    // u_{j'} = y0 + y1 and u_{j'+256} = (y0 - y1) e^{-i pi/4} from y0/2 and y1/2
+    TA_HD static c64 pair_inv_lo(c64 y0, c64 y1) { return cadd(y0, y1); }
+    TA_HD static c64 pair_inv_hi(c64 y0, c64 y1) {
+        const c64 d = csub(y0, y1);
+        return c64{SQ * (d.re + d.im), SQ * (d.im - d.re)};
+    }
+};
+
+// Twiddles of the half transform: [h][lane][10]: tw1_h[k1] = psi^{(1+4h) b} w_256^{b k1}
+// (psi = exp(i pi/1024), b = lane), then w_64^{g mu} (g = lane & 15, mu = 1..3) and
+// w_16^{g0 nu} (g0 = lane & 3, nu = 1..3).
+constexpr int HALF_TW_PER_LANE = 10;
+constexpr int HALF_TW_COUNT = 2 * 64 * HALF_TW_PER_LANE;
+inline void build_twiddles_half(c64* out) {
+    const long double pi = 3.141592653589793238462643383279502884L;
+    auto ex = [&](long double ang) { return c64{(double)cosl(ang), (double)sinl(ang)}; };
+    for (int h = 0; h < 2; h++)
+        for (int b = 0; b < 64; b++) {
+            c64* t = out + (h * 64 + b) * HALF_TW_PER_LANE;
+            for (int k1 = 0; k1 < 4; k1++) {
+                // exp(i pi ((1+4h) b + 8 b k1) / 1024)
+                const long e = ((long)(1 + 4 * h) * b + 8L * b * k1) % 2048;
+                t[k1] = ex(pi * (long double)e / 1024.0L);
+            }
+            for (int mu = 1; mu < 4; mu++) t[3 + mu] = ex(2.0L * pi * (long double)(((b & 15) * mu) % 64) / 64.0L);
+            for (int nu = 1; nu < 4; nu++) t[6 + nu] = ex(2.0L * pi * (long double)(((b & 3) * nu) % 16) / 16.0L);
+        }
+}
+
 // Twiddles of the pair transform: [h][lane][16]: tw1_h[k1] = psi^{(1+4h) b} w_512^{b k1}
 // (psi = exp(i pi/2048)), then the 8 stage-2a twiddles w_64^{(b&7) k2a}.
 constexpr int PAIR_TW_COUNT = 2 * 64 * 16;
