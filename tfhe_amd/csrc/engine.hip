@@ -1480,14 +1480,18 @@ __global__ __launch_bounds__(256) void ks_planes_kernel(const int32_t* __restric
 }
 
 // A workgroup = 4 wavefronts; wavefront w owns 16 output columns (group
-// blockIdx.x*4 + w) of GT*16 gates (GT tiles of 16, from blockIdx.y*GT*16).
-// abar is the digit source a_i + prec_offset, [kN][cpad] with cpad a multiple
-// of 128.
+// blockIdx.x*4 + w) of GT*16 gates (GT tiles of 16, from blockIdx.y*GT*16)
+// over input coefficients [blockIdx.z * ips, +ips). abar is the digit source
+// a_i + prec_offset, [kN][cpad] with cpad a multiple of 128. With one slice
+// (gridDim.z == 1) the output rows are written; with several, each slice
+// subtracts its partial sum from rows already set to (0, ..., 0, b) — integer
+// atomics, exact mod 2^32 in any order.
 template <int GT>
 __global__ __launch_bounds__(256, 2) void ks_mma_kernel(const int8_t* __restrict__ kp, const uint32_t* __restrict__ abar,
                                                      int cpad, int nin, int npad, int count,
                                                      const int32_t* __restrict__ binit, int n,
-                                                     int32_t* __restrict__ out, const int32_t* __restrict__ out_idx) {
+                                                     int32_t* __restrict__ out, const int32_t* __restrict__ out_idx,
+                                                     int ips) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int q = lane >> 4, r16 = lane & 15;
     const int col = (blockIdx.x * 4 + w) * 16 + r16;
@@ -1512,15 +1516,16 @@ __global__ __launch_bounds__(256, 2) void ks_mma_kernel(const int8_t* __restrict
 #pragma unroll
         for (int gt = 0; gt < GT; gt++) av8[gt] = ab[(size_t)i * cpad + gt * 16];
     };
-    load(0);
-    for (int i0 = 0; i0 < nin; i0 += 2) {
+    const int ibeg = blockIdx.z * ips, iend = min(nin, ibeg + ips);
+    load(ibeg);
+    for (int i0 = ibeg; i0 < iend; i0 += 2) {
         v4i b_cur[4];
         uint32_t a_cur[GT];
 #pragma unroll
         for (int b = 0; b < 4; b++) b_cur[b] = bv[b];
 #pragma unroll
         for (int gt = 0; gt < GT; gt++) a_cur[gt] = av8[gt];
-        if (i0 + 2 < nin) load(i0 + 2);
+        if (i0 + 2 < iend) load(i0 + 2);
 #pragma unroll
         for (int gt = 0; gt < GT; gt++) {
             const uint32_t x = (a_cur[gt] >> sh) & 255u;
@@ -1541,9 +1546,18 @@ __global__ __launch_bounds__(256, 2) void ks_mma_kernel(const int8_t* __restrict
             if (gate >= count) continue;
             const uint32_t v = (uint32_t)acc[gt][0][rr] + ((uint32_t)acc[gt][1][rr] << 8) +
                                ((uint32_t)acc[gt][2][rr] << 16) + ((uint32_t)acc[gt][3][rr] << 24);
-            const uint32_t o = (col == n ? (uint32_t)binit[gate] : 0u) - v;
-            out[(size_t)(out_idx ? out_idx[gate] : gate) * (n + 1) + col] = (int32_t)o;
+            int32_t* dst = out + (size_t)(out_idx ? out_idx[gate] : gate) * (n + 1) + col;
+            if (gridDim.z == 1) *dst = (int32_t)((col == n ? (uint32_t)binit[gate] : 0u) - v);
+            else atomicSub((unsigned int*)dst, v);
         }
+}
+
+// Output rows (0, ..., 0, b) for the sliced matrix-core keyswitch.
+__global__ __launch_bounds__(256) void ks_out_init_kernel(const int32_t* __restrict__ binit, int n,
+                                                          int32_t* __restrict__ out, const int32_t* __restrict__ out_idx) {
+    const int gate = blockIdx.x;
+    int32_t* row = out + (size_t)(out_idx ? out_idx[gate] : gate) * (n + 1);
+    for (int c = threadIdx.x; c <= n; c += 256) row[c] = c == n ? binit[gate] : 0;
 }
 
 // ---------------------------------------------------------------- runtime --
@@ -1703,8 +1717,15 @@ int run_keyswitch(const tfhe_amd_device_key* dk, const int32_t* ext, const int32
         int32_t* binit = (int32_t*)(scr + align256(sizeof(uint32_t) * (size_t)nin * cpad));
         hipLaunchKernelGGL(ks_prep_kernel, dim3(count), dim3(256), 0, s, ext, second, nin, 1 << 29, prec_offset0, abar,
                            cpad, binit);
-        hipLaunchKernelGGL(ks_mma_kernel<KS_MMA_GT>, dim3(dk->npad / 64, cpad / (16 * KS_MMA_GT)), dim3(256), 0, s,
-                           dk->kp, abar, cpad, nin, dk->npad, count, binit, P.n, out, out_idx);
+        // small batches leave most CUs idle: slice the input coefficients over
+        // up to 32 workgroups per tile (an even number of coefficients each)
+        const int gx = dk->npad / 64, gy = (count + 16 * KS_MMA_GT - 1) / (16 * KS_MMA_GT);
+        int slices = 1;
+        while (gx * gy * slices < 512 && slices < 32 && nin / (2 * slices) >= 16) slices *= 2;
+        const int ips = (nin / slices + 1) & ~1;
+        if (slices > 1) hipLaunchKernelGGL(ks_out_init_kernel, dim3(count), dim3(256), 0, s, binit, P.n, out, out_idx);
+        hipLaunchKernelGGL(ks_mma_kernel<KS_MMA_GT>, dim3(gx, gy, slices), dim3(256), 0, s, dk->kp, abar, cpad, nin,
+                           dk->npad, count, binit, P.n, out, out_idx, ips);
         HIP_TRY(hipGetLastError());
         return 0;
     }
