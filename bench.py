@@ -456,7 +456,7 @@ def main():
         lat, small, tune = T.autotune(sk.cloud)
         autotune = {"latency_kernel_up_to": lat, "two_wave_kernel_up_to": small,
                     "ms": {name: {str(k): round(v, 3) for k, v in t.items()} for name, t in tune.items()}}
-        print(f"[bench] autotune: 2l-wavefront kernel up to {lat} gates, two-wavefront up to {small}",
+        print(f"[bench] autotune: latency kernel (4l wavefronts) up to {lat} gates, two-wavefront up to {small}",
               file=sys.stderr, flush=True)
     sizes = [int(x) for x in args.scaling.split(",") if x.strip()]
     scaling = bench_scaling(T, sk, P, sizes, args.seed) if sizes else None
