@@ -55,6 +55,25 @@ def test_keyswitch_bitexact(p128, count, path):
         assert np.array_equal(got[g], O.keyswitch(keys, ext[g])), f"gate {g}"
 
 
+@pytest.mark.parametrize("count", [1, 64, 65, 257, 1500])
+def test_sliced_keyswitch_matches_integer_path(p128, count):
+    """The matrix-core keyswitch slices its input coefficients over several
+    workgroups (integer atomics into rows preset to (0, b)) while the grid is
+    small, and runs one slice for large batches: every output row equals the
+    integer lanes-over-gates kernel's, across the slicing thresholds."""
+    T, P, sk, keys = p128
+    rng = np.random.default_rng(300 + count)
+    ext = rand_i32(rng, (count, P.N + 1))
+    got = T.keyswitch(sk.cloud, P, ext)
+    T.lib().tfhe_amd_set_keyswitch_path(1)
+    try:
+        ref = T.keyswitch(sk.cloud, P, ext)
+    finally:
+        T.lib().tfhe_amd_set_keyswitch_path(0)
+    assert np.array_equal(got, ref)
+    assert np.array_equal(got[0], O.keyswitch(keys, ext[0]))
+
+
 def _row_ptr(cloud, i):
     base = cloud.contents.bkFFT.contents.bkFFT
     addr = ctypes.addressof(base.contents) + i * ctypes.sizeof(base._type_)
