@@ -2245,7 +2245,7 @@ TFHE_AMD_EXPORT int32_t tfhe_amd_get_launch_config(int32_t* br_mode, int32_t* la
 
 // Times the three N = 1024 blind-rotation kernels (each with its keyswitch) on
 // pseudo-random ciphertexts over a ladder of batch sizes and sets the automatic
-// mode's crossovers: the 2l-wavefront kernel up to the largest size up to
+// mode's crossovers: the latency kernel up to the largest size up to
 // which it is the fastest at every measured point, then the two-wavefront
 // kernel up to the largest size up to which it beats the four-gate kernel at
 // every further point.
