@@ -1,0 +1,114 @@
+"""Edge cases of the batched gate path on the GPU: empty batches through every
+entry point, a batch of MUX gates only (the job count at its maximum of two
+bootstraps per gate) in every N = 1024 kernel, a ragged batch whose last
+workgroup is partly idle, and one very large launch (131,072 gates reading a
+shared pool of 4096 inputs) — each against the plaintext gates, and sampled
+gates bit-exact against the oracle."""
+import ctypes
+
+import numpy as np
+import pytest
+
+import oracle_py as O
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def key(gpu):
+    T = gpu
+    P = T.default_params(110)
+    sk = T.SecretKey(P, seed=21)
+    keys = O.Keys(O.p128(), **sk.export())
+    sk.upload()
+    yield T, P, sk, keys
+    sk.close()
+
+
+def _truth(ops, bits):
+    return np.array([O.TRUTH[O.GATES[o]](x, y, z) for o, x, y, z in zip(ops, *bits)])
+
+
+def test_empty_batches(key):
+    T, P, sk, keys = key
+    L = T.lib()
+    d_pool = T.DeviceArray((4, P.n + 1))
+    d_idx = T.DeviceArray(4)
+    try:
+        assert L.tfhe_amd_gates_device(sk.cloud, 0, d_idx.ptr, d_idx.ptr, d_idx.ptr, d_idx.ptr, d_idx.ptr,
+                                       d_pool.ptr, None) == 0
+        assert L.tfhe_amd_keyswitch_device(sk.cloud, 0, d_pool.ptr, d_pool.ptr, None) == 0
+        assert L.bootsGate_batch(0, None, None, None, None, None, sk.cloud) == 0
+        T.synchronize()
+    finally:
+        d_pool.free()
+        d_idx.free()
+    # a negative count is an error, not a launch
+    assert L.tfhe_amd_gates_device(sk.cloud, -1, None, None, None, None, None, None, None) != 0
+
+
+@pytest.mark.parametrize("mode", [1, 2, 3, 5])
+def test_mux_only_batch_every_kernel(key, mode):
+    T, P, sk, keys = key
+    rng = np.random.default_rng(70 + mode)
+    count = 515  # 1030 bootstraps: not a multiple of any kernel's bootstraps per workgroup
+    ops = np.full(count, 10, np.int32)
+    bits = rng.integers(0, 2, size=(3, count)).astype(np.int32)
+    a, b, c = (sk.encrypt(x) for x in bits)
+    T.set_launch_config(mode)
+    try:
+        got = T.run_gates(sk.cloud, ops, a, b, c)
+    finally:
+        T.set_launch_config(0)
+    assert np.array_equal(sk.decrypt(got), _truth(ops, bits))
+    sel = [0, 257, 514]
+    assert np.array_equal(got[sel], O.gates(keys, ops[sel], a[sel], b[sel], c[sel], threads=8))
+
+
+def test_ragged_batch_in_the_batched_kernel(key):
+    """4097 gates in the four-gates-per-workgroup kernel: the last workgroup has
+    one active wavefront (with MUX gates appended as extra jobs)."""
+    T, P, sk, keys = key
+    rng = np.random.default_rng(80)
+    count = 4097
+    ops = rng.integers(0, 11, size=count).astype(np.int32)
+    ops[-1] = 10
+    bits = rng.integers(0, 2, size=(3, count)).astype(np.int32)
+    a, b, c = (sk.encrypt(x) for x in bits)
+    T.set_launch_config(2)
+    try:
+        got = T.run_gates(sk.cloud, ops, a, b, c)
+    finally:
+        T.set_launch_config(0)
+    assert np.array_equal(sk.decrypt(got), _truth(ops, bits))
+    sel = [0, 2048, 4095, 4096]
+    assert np.array_equal(got[sel], O.gates(keys, ops[sel], a[sel], b[sel], c[sel], threads=8))
+
+
+def test_very_large_launch(key):
+    """131,072 gates in one launch over a pool of 4096 inputs (so every index and
+    workspace size is well past 2^16 gates)."""
+    T, P, sk, keys = key
+    rng = np.random.default_rng(90)
+    n_in, count = 4096, 131072
+    w = P.n + 1
+    in_bits = rng.integers(0, 2, size=n_in).astype(np.int32)
+    cts = sk.encrypt(in_bits)
+    ops = rng.integers(0, 11, size=count).astype(np.int32)
+    i0, i1, i2 = (rng.integers(0, n_in, size=count).astype(np.int32) for _ in range(3))
+    out = (n_in + np.arange(count)).astype(np.int32)
+    pool = np.zeros((n_in + count, w), np.int32)
+    pool[:n_in] = cts
+    arrs = [T.DeviceArray.from_numpy(x) for x in (pool, ops, i0, i1, i2, out)]
+    d_pool, d_ops, d_i0, d_i1, d_i2, d_o = arrs
+    try:
+        T.gates_device(sk.cloud, d_ops, d_i0, d_i1, d_i2, d_o, d_pool)
+        T.synchronize()
+        res = d_pool.numpy()[n_in:]
+    finally:
+        for d in arrs:
+            d.free()
+    exp = _truth(ops, (in_bits[i0], in_bits[i1], in_bits[i2]))
+    assert np.array_equal(sk.decrypt(res), exp)
+    sel = np.array([0, 65535, 65536, 131071])
+    assert np.array_equal(res[sel], O.gates(keys, ops[sel], cts[i0[sel]], cts[i1[sel]], cts[i2[sel]], threads=8))
