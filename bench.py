@@ -38,6 +38,8 @@ def parse():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--adders", type=int, default=512, help="ripple-carry adders in the netlist section (0 = skip)")
     ap.add_argument("--adder-bits", type=int, default=32)
+    ap.add_argument("--client-bits", type=int, default=1 << 20,
+                    help="client-side section: bits encrypted and decrypted on the device (0 = skip)")
     ap.add_argument("--scaling", default="1,16,64,128,256,512,1024,2048,8192",
                     help="batch sizes for the per-kernel batch-size sweep ('' = skip)")
     ap.add_argument("--serve-clients", type=int, default=32, help="serving section: clients, one key each (0 = skip)")
@@ -327,6 +329,44 @@ def bench_adders(T, sk, P, n_adders, bits, reps, seed):
                         "level by level from a captured hipGraph", **out}
 
 
+def bench_client(T, sk, P, count, seed):
+    """Client-side work on the device: batch encryption of `count` bits
+    (Philox noise and masks) and batch decryption, both on HBM-resident data,
+    and the generation of one secret key set on the device."""
+    rng = np.random.default_rng(seed + 11)
+    bits = rng.integers(0, 2, size=count).astype(np.int32)
+    d_bits = T.DeviceArray.from_numpy(bits)
+    d_cts = T.DeviceArray((count, P.n + 1))
+    d_dec = T.DeviceArray(count)
+    try:
+        T.encrypt_device(sk, d_bits, d_cts, seed=seed + 12)  # warm-up
+        T.decrypt_device(sk, d_cts, d_bits=d_dec)
+        T.synchronize()
+        t0 = time.perf_counter()
+        T.encrypt_device(sk, d_bits, d_cts, seed=seed + 13)
+        T.synchronize()
+        t1 = time.perf_counter()
+        T.decrypt_device(sk, d_cts, d_bits=d_dec)
+        T.synchronize()
+        t2 = time.perf_counter()
+        errors = int((d_dec.numpy() != bits).sum())
+    finally:
+        for d in (d_bits, d_cts, d_dec):
+            d.free()
+    t3 = time.perf_counter()
+    k = T.SecretKey.generate_on_device(P, seed=seed + 14)
+    keygen_s = time.perf_counter() - t3
+    k.close()
+    out = {"bits": count,
+           "encrypt_per_s": round(count / (t1 - t0), 1),
+           "decrypt_per_s": round(count / (t2 - t1), 1),
+           "encrypt_ms": round((t1 - t0) * 1e3, 3), "decrypt_ms": round((t2 - t1) * 1e3, 3),
+           "decrypt_errors": errors,
+           "keygen_device_s": round(keygen_s, 3)}
+    print(f"[bench] client side: {out}", file=sys.stderr, flush=True)
+    return out
+
+
 def cpu_baseline(count, seed):
     """Reference spqlios-FMA CPU path (oracle/_ref, built from the reference's
     own sources) timed on this host; falls back to the oracle port."""
@@ -460,6 +500,7 @@ def main():
               file=sys.stderr, flush=True)
     sizes = [int(x) for x in args.scaling.split(",") if x.strip()]
     scaling = bench_scaling(T, sk, P, sizes, args.seed) if sizes else None
+    client = bench_client(T, sk, P, args.client_bits, args.seed) if args.client_bits > 0 else None
     adders = bench_adders(T, sk, P, args.adders, args.adder_bits, 2, args.seed) if args.adders > 0 else None
     serving = bench_serving(T, P, args) if args.serve_clients > 0 else None
     pbig = bench_pbig(T, args) if args.pbig_launches > 0 else None
@@ -492,6 +533,7 @@ def main():
         "autotune": autotune,
         "batch_size_sweep": scaling,
         "netlist_adders": adders,
+        "client_side": client,
         "serving": serving,
         "pbig_mixed": pbig,
         "kernel_ms": {"blind_rotate_avg": round(br_ms / max(launches, 1), 4),
