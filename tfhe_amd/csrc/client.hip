@@ -11,6 +11,8 @@
 // not the reference's ciphertexts for the same seed.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include <cmath>
 #include <map>
 #include <mutex>
@@ -22,21 +24,45 @@
 namespace tfhe_amd {
 namespace {
 
-// One wavefront per ciphertext; rows of w = n + 1 int32 (a[0..n), b).
+// Rows of w = n + 1 int32 (a[0..n), b). Each wavefront keeps its slice of the
+// key in registers (key[lane + 64 k], n <= 1024) and walks the rows four at a
+// time (grid-stride), so 4 x ceil(n/64) row loads are in flight per step.
+constexpr int DECRYPT_KS = 16;
 __global__ void __launch_bounds__(256) decrypt_kernel(const int32_t* __restrict__ cts, int count, int n,
                                                       const int32_t* __restrict__ key, int32_t* __restrict__ bits,
                                                       int32_t* __restrict__ phases) {
     const int lane = threadIdx.x & 63;
-    const int g = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (g >= count) return;
-    const int32_t* a = cts + (size_t)g * (n + 1);
-    uint32_t s = 0;
-    for (int i = lane; i < n; i += 64) s += (uint32_t)a[i] * (uint32_t)key[i];
-    for (int d = 32; d >= 1; d >>= 1) s += (uint32_t)__shfl_xor((int)s, d, 64);
-    if (lane == 0) {
-        const int32_t phase = (int32_t)((uint32_t)a[n] - s);
-        if (bits) bits[g] = phase > 0 ? 1 : 0;
-        if (phases) phases[g] = phase;
+    uint32_t kr[DECRYPT_KS];
+#pragma unroll
+    for (int k = 0; k < DECRYPT_KS; k++) {
+        const int i = lane + 64 * k;
+        kr[k] = i < n ? (uint32_t)key[i] : 0u;
+    }
+    const int stride = gridDim.x * 4 * 4;
+    for (int g0 = (blockIdx.x * 4 + (threadIdx.x >> 6)) * 4; g0 < count; g0 += stride) {
+        uint32_t sum[4];
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            sum[c] = 0;
+            if (g0 + c < count) {
+                const int32_t* a = cts + (size_t)(g0 + c) * (n + 1);
+#pragma unroll
+                for (int k = 0; k < DECRYPT_KS; k++) {
+                    const int i = lane + 64 * k;
+                    if (i < n) sum[c] += (uint32_t)a[i] * kr[k];
+                }
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < 4; c++)
+            for (int d = 32; d >= 1; d >>= 1) sum[c] += (uint32_t)__shfl_xor((int)sum[c], d, 64);
+        if (lane < 4 && g0 + lane < count) {
+            const int g = g0 + lane;
+            const uint32_t s = lane == 0 ? sum[0] : lane == 1 ? sum[1] : lane == 2 ? sum[2] : sum[3];
+            const int32_t phase = (int32_t)((uint32_t)cts[(size_t)g * (n + 1) + n] - s);
+            if (bits) bits[g] = phase > 0 ? 1 : 0;
+            if (phases) phases[g] = phase;
+        }
     }
 }
 
@@ -252,8 +278,10 @@ TFHE_AMD_EXPORT int32_t tfhe_amd_decrypt_device(const TFheGateBootstrappingSecre
     const int32_t* dkey = device_lwe_key(key->lwe_key, n);
     if (!dkey) return -1;
     hipStream_t s = stream ? (hipStream_t)stream : (hipStream_t)engine_stream();
-    hipLaunchKernelGGL(decrypt_kernel, dim3((count + 3) / 4), dim3(256), 0, s, d_cts, count, n, dkey, d_bits,
-                       d_phases);
+    if (n > 64 * DECRYPT_KS) return set_error("decrypt_device: n = %d above %d", n, 64 * DECRYPT_KS);
+    // up to 16 workgroups (64 wavefronts) per CU's worth of rows, at least one
+    const int blocks = std::max(1, std::min((count + 15) / 16, 256 * 16));
+    hipLaunchKernelGGL(decrypt_kernel, dim3(blocks), dim3(256), 0, s, d_cts, count, n, dkey, d_bits, d_phases);
     const hipError_t e = hipGetLastError();
     return e == hipSuccess ? 0 : set_error("decrypt_device: launch failed: %s", hipGetErrorString(e));
 }
