@@ -34,7 +34,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--batch", type=int, default=4096)
     ap.add_argument("--per-gate", type=int, default=64, help="one-gate-per-launch samples")
-    ap.add_argument("--cpu-gates", type=int, default=500, help="reference CPU sample size (0 = skip)")
+    ap.add_argument("--cpu-gates", type=int, default=1500, help="reference CPU sample size, about 15 s of one core (0 = skip)")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--adders", type=int, default=512, help="ripple-carry adders in the netlist section (0 = skip)")
     ap.add_argument("--adder-bits", type=int, default=32)
