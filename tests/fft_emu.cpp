@@ -139,20 +139,8 @@ struct EmuHalf {
         for (int h = 0; h < 2; h++) {
             std::vector<c64> lds(H::LDS_C64);
             c64 (*xh)[4] = x + 64 * h;
-            // the first step written into two buffers as halves of the spectrum, then
-            // summed by inv2_sum (as the latency kernel sums its products)
-            std::vector<c64> part(H::LDS_C64);
-            for (int l = 0; l < 64; l++) {
-                c64 half1[4], half2[4];
-                for (int k = 0; k < 4; k++) {
-                    half1[k] = c64{0.5 * xh[l][k].re, 0.5 * xh[l][k].im};
-                    half2[k] = half1[k];
-                }
-                H::inv1(half1, l, lds.data());
-                H::inv1(half2, l, part.data());
-            }
-            const c64* src[2] = {lds.data(), part.data()};
-            for (int l = 0; l < 64; l++) H::inv2_sum(xh[l], l, src, 2, t(h, l));
+            for (int l = 0; l < 64; l++) H::inv1(xh[l], l, lds.data());
+            for (int l = 0; l < 64; l++) H::inv2(xh[l], l, lds.data(), t(h, l));
             for (int l = 0; l < 64; l++) H::inv2w(xh[l], l, lds.data());
             for (int l = 0; l < 64; l++) H::inv3(xh[l], l, lds.data(), t(h, l));
             for (int l = 0; l < 64; l++) H::inv3w(xh[l], l, lds.data());

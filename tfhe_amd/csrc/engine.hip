@@ -512,7 +512,7 @@ __global__ __launch_bounds__(4 * L * 64, 1) void brl_kernel(BrArgs A, const c64*
     constexpr int R = H::R, N = 1024, KPL = 2 * L, W = 4 * L;
     constexpr int LOG2N2 = 11;
     __shared__ c64 s_fft[W][H::LDS_C64];  // transform buffer, then the product for output 0
-    __shared__ c64 s_par[W][H::LDS_C64];  // product for output 1
+    __shared__ c64 s_par[W][256];         // product for output 1
     __shared__ c64 s_inv[4][H::LDS_C64];  // inverse transforms of wavefronts (c, 0, h)
     __shared__ int32_t s_acc[2][N];
     __shared__ uint32_t s_tv[2][N];  // (X^abar - 1) * ACC + gadget offset, both components
@@ -620,12 +620,14 @@ __global__ __launch_bounds__(4 * L * 64, 1) void brl_kernel(BrArgs A, const c64*
         c64 x[R];
 #pragma unroll
         for (int a = 0; a < R; a++) {
-            // u_{j'} = d_{j'} + i d_{j'+512}, u_{j'+256} = d_{j'+256} + i d_{j'+768}, j' = lane + 64 a
-            double d[4];
+            // u_{j'} = d_{j'} + i d_{j'+512}, u_{j'+256} = d_{j'+256} + i d_{j'+768}, j' = lane + 64 a;
+            // y_h = u_{j'} +- e^{i pi/4} u_{j'+256}: the integer parts of e^{i pi/4} u_{j'+256}
+            // are formed exactly in int32, then one FMA per component
+            int32_t d[4];
 #pragma unroll
-            for (int q = 0; q < 4; q++) d[q] = (double)((int32_t)__builtin_amdgcn_ubfe(tv[q * R + a], shift, Bgbit) - dhalf);
-            const c64 lo{d[0], d[2]}, hi{d[1], d[3]};
-            x[a] = h == 0 ? H::pair_fwd<0>(lo, hi) : H::pair_fwd<1>(lo, hi);
+            for (int q = 0; q < 4; q++) d[q] = (int32_t)__builtin_amdgcn_ubfe(tv[q * R + a], shift, Bgbit) - dhalf;
+            const double sq = h == 0 ? H::SQ : -H::SQ;
+            x[a] = c64{fma(sq, (double)(d[1] - d[3]), (double)d[0]), fma(sq, (double)(d[1] + d[3]), (double)d[2])};
         }
         if (h == 0) H::fwd1<0>(x, lane, fb, tw);
         else H::fwd1<1>(x, lane, fb, tw);
@@ -640,30 +642,29 @@ __global__ __launch_bounds__(4 * L * 64, 1) void brl_kernel(BrArgs A, const c64*
         wave_sync();
         H::fwd4(x, lane, fb);
         wave_sync();
-        {
-            // the products, after the inverse transform's first (linear) step
-            c64 y0[R], y1[R];
 #pragma unroll
-            for (int s = 0; s < R; s++) {
-                y0[s] = cmul(x[s], k0[s]);
-                y1[s] = cmul(x[s], k1[s]);
-            }
-            H::inv1(y0, lane, fb);
-            H::inv1(y1, lane, s_par[w]);
+        for (int s = 0; s < R; s++) {
+            fb[s * 64 + lane] = cmul(x[s], k0[s]);
+            s_par[w][s * 64 + lane] = cmul(x[s], k1[s]);
         }
         if (inext < n) load_key(inext);
         __syncthreads();  // all products written
         if (p == 0) {
             // output component c, half h: the 2l products of the wavefronts (c', p', h)
-            const c64* src[2 * L];
+            c64 y[R];
+#pragma unroll
+            for (int s = 0; s < R; s++) y[s] = c64{0.0, 0.0};
 #pragma unroll
             for (int v = 0; v < 2 * L; v++) {
                 const int wv = h + 2 * (v & 1) + 4 * (v >> 1);
-                src[v] = c ? s_par[wv] : s_fft[wv];
+                const c64* src = c ? s_par[wv] : s_fft[wv];
+#pragma unroll
+                for (int s = 0; s < R; s++) y[s] = cadd(y[s], src[s * 64 + lane]);
             }
-            c64 y[R];
             c64* ib = s_inv[w];
-            H::inv2_sum(y, lane, src, 2 * L, tw);
+            H::inv1(y, lane, ib);
+            wave_sync();
+            H::inv2(y, lane, ib, tw);
             wave_sync();
             H::inv2w(y, lane, ib);
             wave_sync();

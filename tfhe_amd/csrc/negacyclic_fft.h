@@ -456,8 +456,6 @@ struct FftHalf {
         dft4<1>(x);
     }
     // ---- inverse (exact inverse of the forward steps, times 256) ----
-    // first step: 4-point DFT over omega, written transposed. It is linear, so a
-    // sum of spectra can also be formed after it (inv2_sum reads several buffers).
     TA_HD static void inv1(c64* x, int lane, c64* lds) {
         dft4<-1>(x);
         c64* d = lds + base4(lane);
@@ -469,18 +467,6 @@ struct FftHalf {
         x[0] = q[0];
 #pragma unroll
         for (int nu = 1; nu < R; nu++) x[nu] = cmulc(q[4 * nu], tw[6 + nu]);
-        dft4<-1>(x);
-    }
-    // inv2 on the sum of `count` buffers src[v] (each written by inv1)
-    TA_HD static void inv2_sum(c64* x, int lane, const c64* const* src, int count, const c64* tw) {
-        const int o = base3(lane);
-#pragma unroll
-        for (int nu = 0; nu < R; nu++) x[nu] = src[0][o + 4 * nu];
-        for (int v = 1; v < count; v++)
-#pragma unroll
-            for (int nu = 0; nu < R; nu++) x[nu] = cadd(x[nu], src[v][o + 4 * nu]);
-#pragma unroll
-        for (int nu = 1; nu < R; nu++) x[nu] = cmulc(x[nu], tw[6 + nu]);
         dft4<-1>(x);
     }
     TA_HD static void inv2w(const c64* x, int lane, c64* lds) {
