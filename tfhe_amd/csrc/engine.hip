@@ -888,11 +888,16 @@ __global__ __launch_bounds__(256, 2) void br2_kernel(BrArgs A, const int32_t* __
             __syncthreads();  // key blocks landed
             if (active) {
                 const c64* bk = s_bk[c];
+                // eight key values in flight at a time, then their multiply-accumulates
+                c64 kv[R];
 #pragma unroll
-                for (int s = 0; s < R; s++) {
-                    cmac(p0[s], x[s], bk[s * 64 + lane]);
-                    cmac(p1[s], x[s], bk[M + s * 64 + lane]);
-                }
+                for (int s = 0; s < R; s++) kv[s] = bk[s * 64 + lane];
+#pragma unroll
+                for (int s = 0; s < R; s++) cmac(p0[s], x[s], kv[s]);
+#pragma unroll
+                for (int s = 0; s < R; s++) kv[s] = bk[M + s * 64 + lane];
+#pragma unroll
+                for (int s = 0; s < R; s++) cmac(p1[s], x[s], kv[s]);
             }
         }
         // swap partial sums: wavefront c keeps output c, hands over output 1-c
@@ -1168,15 +1173,21 @@ __global__ __launch_bounds__(512, 1) void brp_kernel(BrArgs A, const c64* __rest
             F::fwd4(x, lane, fb);
             __syncthreads();  // key blocks landed
             const c64* bk = s_bk[c] + h * 512;
+            // eight key values in flight at a time, then their multiply-accumulates
+            c64 kv[R];
+#pragma unroll
+            for (int s = 0; s < R; s++) kv[s] = bk[s * 64 + lane];
 #pragma unroll
             for (int s = 0; s < R; s++) {
-                if constexpr (FIRST) {
-                    p0[s] = cmul_first(x[s], bk[s * 64 + lane]);
-                    p1[s] = cmul_first(x[s], bk[1024 + s * 64 + lane]);
-                } else {
-                    cmac(p0[s], x[s], bk[s * 64 + lane]);
-                    cmac(p1[s], x[s], bk[1024 + s * 64 + lane]);
-                }
+                if constexpr (FIRST) p0[s] = cmul_first(x[s], kv[s]);
+                else cmac(p0[s], x[s], kv[s]);
+            }
+#pragma unroll
+            for (int s = 0; s < R; s++) kv[s] = bk[1024 + s * 64 + lane];
+#pragma unroll
+            for (int s = 0; s < R; s++) {
+                if constexpr (FIRST) p1[s] = cmul_first(x[s], kv[s]);
+                else cmac(p1[s], x[s], kv[s]);
             }
         };
         digit(0, std::true_type{});
