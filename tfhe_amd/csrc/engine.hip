@@ -299,13 +299,15 @@ __global__ __launch_bounds__(1024) void mux_scan_kernel(const int32_t* __restric
     if (t == 1023) *n_jobs = count + s_cnt[1023];
 }
 
-// Blind rotation with four gates per workgroup in lockstep (M = 512, i.e.
-// N = 1024). Each wavefront owns one bootstrap; for every digit polynomial of
-// every key row the workgroup loads the 16 KiB of transformed key it needs
-// once into LDS (prefetched one digit ahead into registers) and all four
-// wavefronts read it from there. The accumulator lives in registers in the
-// transform's coefficient layout; the negacyclic rotation is staged through
-// the wavefront's own transform buffer.
+// Blind rotation with one bootstrap per wavefront, four wavefronts per
+// workgroup (M = 512, i.e. N = 1024). The wavefronts share nothing and never
+// wait for each other: the transposes of the transform go through the
+// wavefront's own LDS buffer, and the transformed key and the stage-1
+// twiddles come straight from global memory (cached; the 4096 wavefronts of a
+// batch walk the key rows nearly together), so the LDS carries only the
+// transposes. The accumulator lives in registers in the transform's
+// coefficient layout; the negacyclic rotation is staged through the
+// wavefront's transform buffer.
 template <int L>
 __global__ __launch_bounds__(256, 2) void br4_kernel(BrArgs A, const int32_t* __restrict__ mux_gate,
                                                     const int32_t* __restrict__ n_jobs) {
@@ -314,17 +316,12 @@ __global__ __launch_bounds__(256, 2) void br4_kernel(BrArgs A, const int32_t* __
     constexpr int R = F::R, N = 2 * M, KPL = 2 * L;
     constexpr int LOG2N2 = 11;
     __shared__ c64 s_fft[4][F::LDS_C64];
-    __shared__ c64 s_bk[2 * M];
-    __shared__ c64 s_tw1[R * 64];
     __shared__ int16_t s_bara[4][1024];
 
-    const int tid = threadIdx.x;
-    const int lane = tid & 63;
-    const int w = tid >> 6;
-    const int njobs = *n_jobs;
+    const int lane = threadIdx.x & 63;
+    const int w = threadIdx.x >> 6;
     const int job = blockIdx.x * 4 + w;
-    if (blockIdx.x * 4 >= njobs) return;  // whole block idle (uniform)
-    const bool active = job < njobs;
+    if (job >= *n_jobs) return;  // no barriers below, so idle wavefronts leave at once
     const int n = A.n;
     int16_t* bara = s_bara[w];
     c64* fb = s_fft[w];
@@ -332,7 +329,7 @@ __global__ __launch_bounds__(256, 2) void br4_kernel(BrArgs A, const int32_t* __
 
     // accumulator: acc[c][h*R + s] = ACC_c[64 s + lane + h M]
     uint32_t acc[2][2 * R];
-    if (active) {
+    {
         const int gate = job < A.count ? job : mux_gate[job - A.count];
         const int half = job < A.count ? 0 : 1;
         const int op = A.op[gate];
@@ -361,21 +358,17 @@ __global__ __launch_bounds__(256, 2) void br4_kernel(BrArgs A, const int32_t* __
                 acc[0][h * R + s] = 0;
                 acc[1][h * R + s] = (((j + barb) & (2 * N - 1)) < N) ? (uint32_t)A.mu : 0u - (uint32_t)A.mu;
             }
-    } else {
-        for (int i = lane; i < n; i += 64) bara[i] = 0;
-#pragma unroll
-        for (int q = 0; q < 2 * R; q++) acc[0][q] = acc[1][q] = 0;
     }
 
-    // per-lane stage-1 twiddles psi^b w_M^{b k1}, shared by the four wavefronts: s_tw1[k1][lane]
-    for (int q = tid; q < R * 64; q += 256) s_tw1[q] = A.tw[(q & 63) * (R + 8) + (q >> 6)];
-    const c64* tw1 = s_tw1 + lane;  // tw1[k1 * 64]
+    // stage-1 twiddles psi^b w_M^{b k1}, read per transform from the table's [k1][lane] copy
+    // (8 KiB, cached) through the vector memory path; the stage-2a twiddles are powers of one
+    // per-lane value
+    const c64* tw1 = A.tw + TwiddleTable<M>::TW1_OFF + lane;  // tw1[k1 * 64]
     const c64 tw2 = A.tw[lane * (R + 8) + R + 1];
     const uint32_t offset = A.offset;
     const int Bgbit = A.Bgbit;
     const int32_t dhalf = 1 << (Bgbit - 1);
-
-    __syncthreads();
+    wave_sync();  // bara written by the whole wavefront
 
     for (int i = 0; i < n; i++) {
         const int abar = bara[i];
@@ -386,92 +379,79 @@ __global__ __launch_bounds__(256, 2) void br4_kernel(BrArgs A, const int32_t* __
 #pragma unroll 1
         for (int r = 0; r < KPL; r++) {
             const int c = r >= L ? 1 : 0, p = r - c * L;
-            __syncthreads();  // previous digit's MAC is done with s_bk
-            {
-                // key digit polynomial (i, r): 2*M contiguous c64, copied global -> LDS
-                // without registers; complete (vmcnt) at the barrier before the MAC
-                const c64* q = A.bkf + ((size_t)i * KPL + r) * 2 * M + tid;
-                c64* d = s_bk + (tid & ~63);
+            if (p == 0) {
+                // T = (X^abar - 1) * ACC_c + gadget offset, through the wave's LDS buffer
+                wave_sync();  // the previous transform is done with the buffer
 #pragma unroll
-                for (int u = 0; u < 4; u++)
-                    __builtin_amdgcn_global_load_lds((const void*)(q + u * 256),
-                                                     (__attribute__((address_space(3))) void*)(d + u * 256), 16, 0, 0);
+                for (int h = 0; h < 2; h++)
+#pragma unroll
+                    for (int s = 0; s < R; s++) fbi[64 * s + lane + h * M] = (int32_t)(c ? acc[1][h * R + s] : acc[0][h * R + s]);
+                wave_sync();
+                // all 16 rotated reads in flight, then the differences
+#pragma unroll
+                for (int q = 0; q < 2 * R; q++) {
+                    const int j = 64 * (q % R) + lane + (q / R) * M;
+                    tv[q] = (uint32_t)fbi[(j - abar) & (N - 1)];
+                }
+#pragma unroll
+                for (int q = 0; q < 2 * R; q++) {
+                    const int j = 64 * (q % R) + lane + (q / R) * M;
+                    const uint32_t rot = ((j - abar) & N) ? 0u - tv[q] : tv[q];
+                    tv[q] = rot - (c ? acc[1][q] : acc[0][q]) + offset;
+                }
             }
+            // digit p of the truncating gadget decomposition, tgsw-functions.cpp:337-348
+            const int shift = 32 - (p + 1) * Bgbit;
             c64 x[R];
-            if (active) {
-                if (p == 0) {
-                    // T = (X^abar - 1) * ACC_c + gadget offset, through the wave's LDS buffer
 #pragma unroll
-                    for (int h = 0; h < 2; h++)
-#pragma unroll
-                        for (int s = 0; s < R; s++) fbi[64 * s + lane + h * M] = (int32_t)(c ? acc[1][h * R + s] : acc[0][h * R + s]);
-                    wave_sync();
-                    // all 16 rotated reads in flight, then the differences
-#pragma unroll
-                    for (int q = 0; q < 2 * R; q++) {
-                        const int j = 64 * (q % R) + lane + (q / R) * M;
-                        tv[q] = (uint32_t)fbi[(j - abar) & (N - 1)];
-                    }
-#pragma unroll
-                    for (int q = 0; q < 2 * R; q++) {
-                        const int j = 64 * (q % R) + lane + (q / R) * M;
-                        const uint32_t rot = ((j - abar) & N) ? 0u - tv[q] : tv[q];
-                        tv[q] = rot - (c ? acc[1][q] : acc[0][q]) + offset;
-                    }
-                    wave_sync();
-                }
-                // digit p of the truncating gadget decomposition, tgsw-functions.cpp:337-348
-                const int shift = 32 - (p + 1) * Bgbit;
-#pragma unroll
-                for (int s = 0; s < R; s++) {
-                    const int32_t d0 = (int32_t)__builtin_amdgcn_ubfe(tv[s], shift, Bgbit) - dhalf;
-                    const int32_t d1 = (int32_t)__builtin_amdgcn_ubfe(tv[R + s], shift, Bgbit) - dhalf;
-                    x[s] = c64{(double)d0, (double)d1};
-                }
-                F::template fwd1<64>(x, lane, fb, tw1);
-                wave_sync();
-                F::fwd2(x, lane, fb, tw2);
-                wave_sync();
-                F::fwd3(x, lane, fb);
-                wave_sync();
-                F::fwd4(x, lane, fb);
+            for (int s = 0; s < R; s++) {
+                const int32_t d0 = (int32_t)__builtin_amdgcn_ubfe(tv[s], shift, Bgbit) - dhalf;
+                const int32_t d1 = (int32_t)__builtin_amdgcn_ubfe(tv[R + s], shift, Bgbit) - dhalf;
+                x[s] = c64{(double)d0, (double)d1};
             }
-            __syncthreads();  // s_bk holds digit (i, r) for every wavefront
-            if (active) {
-                // eight key values in flight at a time, then their multiply-accumulates
-                c64 kv[R];
+            wave_sync();  // rotated reads (p = 0) or the previous transform done with the buffer
+            F::template fwd1<64>(x, lane, fb, tw1);
+            wave_sync();
+            F::fwd2(x, lane, fb, tw2);
+            wave_sync();
+            F::fwd3(x, lane, fb);
+            // key digit polynomial (i, r): component 0 at q[s * 64], component 1 at q[M + s * 64];
+            // half of component 0 requested before the last stage, the rest as registers free up
+            const c64* q = A.bkf + ((size_t)i * KPL + r) * 2 * M + lane;
+            c64 kv[R];
 #pragma unroll
-                for (int s = 0; s < R; s++) kv[s] = s_bk[s * 64 + lane];
+            for (int s = 0; s < R / 2; s++) kv[s] = q[s * 64];
+            wave_sync();
+            F::fwd4(x, lane, fb);
 #pragma unroll
-                for (int s = 0; s < R; s++) cmac(r0[s], x[s], kv[s]);
+            for (int s = R / 2; s < R; s++) kv[s] = q[s * 64];
 #pragma unroll
-                for (int s = 0; s < R; s++) kv[s] = s_bk[M + s * 64 + lane];
-#pragma unroll
-                for (int s = 0; s < R; s++) cmac(r1[s], x[s], kv[s]);
+            for (int s = 0; s < R; s++) {
+                cmac(r0[s], x[s], kv[s]);
+                kv[s] = q[M + s * 64];
             }
+#pragma unroll
+            for (int s = 0; s < R; s++) cmac(r1[s], x[s], kv[s]);
         }
-        if (active) {
-            // inverse transforms and ACC += result (lwe-bootstrapping-functions-fft.cpp:67-76)
+        // inverse transforms and ACC += result (lwe-bootstrapping-functions-fft.cpp:67-76)
 #pragma unroll
-            for (int cc = 0; cc < 2; cc++) {
-                c64* x = cc ? r1 : r0;
-                wave_sync();
-                F::inv1(x, lane, fb);
-                wave_sync();
-                F::inv2(x, lane, fb, tw2);
-                wave_sync();
-                F::inv3(x, lane, fb);
-                wave_sync();
-                F::template inv4<64>(x, lane, fb, tw1);
+        for (int cc = 0; cc < 2; cc++) {
+            c64* x = cc ? r1 : r0;
+            wave_sync();
+            F::inv1(x, lane, fb);
+            wave_sync();
+            F::inv2(x, lane, fb, tw2);
+            wave_sync();
+            F::inv3(x, lane, fb);
+            wave_sync();
+            F::template inv4<64>(x, lane, fb, tw1);
 #pragma unroll
-                for (int s = 0; s < R; s++) {
-                    acc[cc][s] += (uint32_t)round_mod32(x[s].re);
-                    acc[cc][R + s] += (uint32_t)round_mod32(x[s].im);
-                }
+            for (int s = 0; s < R; s++) {
+                acc[cc][s] += (uint32_t)round_mod32(x[s].re);
+                acc[cc][R + s] += (uint32_t)round_mod32(x[s].im);
             }
         }
     }
-    if (!active) return;
     // tLweExtractLweSampleIndex(.., 0, ..), libtfhe/lwe.cpp:41-56: a'_0 = a_0, a'_m = -a_{N-m}
     int32_t* e = A.ext + (size_t)job * (N + 1);
 #pragma unroll
