@@ -203,14 +203,6 @@ struct Fft {
         powers8(w, t);
         fwd2_pow(x, lane, lds, t);
     }
-    // the same with the powers read from a table: tw2[k * TS] = w_64^{(lane & 7) k}
-    template <int TS>
-    TA_HD static void fwd2_tab(c64* x, int lane, const c64* lds, const c64* tw2) {
-        c64 t[8];
-#pragma unroll
-        for (int k = 1; k < 8; k++) t[k] = tw2[k * TS];
-        fwd2_pow(x, lane, lds, t);
-    }
     TA_HD static void fwd2_pow(c64* x, int lane, const c64* lds, const c64* t) {
         const int b0 = lane & 7, c = lane >> 3;
 #pragma unroll
@@ -252,13 +244,6 @@ struct Fft {
     TA_HD static void inv2(c64* x, int lane, const c64* lds, c64 w) {
         c64 t[8];
         powers8(w, t);
-        inv2_pow(x, lane, lds, t);
-    }
-    template <int TS>
-    TA_HD static void inv2_tab(c64* x, int lane, const c64* lds, const c64* tw2) {
-        c64 t[8];
-#pragma unroll
-        for (int k = 1; k < 8; k++) t[k] = tw2[k * TS];
         inv2_pow(x, lane, lds, t);
     }
     TA_HD static void inv2_pow(c64* x, int lane, const c64* lds, const c64* t) {
@@ -569,15 +554,14 @@ inline void build_twiddles_pair(c64* out) {
 
 // Per-lane twiddle tables, built on the host in long double and rounded once.
 // Layout: for each lane, R entries of tw1 then 8 entries of tw2; then the R
-// uniform twist entries and R inverse-twist entries; then the tw2 entries again
-// as [k2a][lane] (TW2_OFF) and the tw1 entries as [k1][lane] (TW1_OFF), so that
-// a wavefront reads one column with one coalesced 1 KiB load.
+// uniform twist entries and R inverse-twist entries; then the tw1 entries again
+// as [k1][lane] (TW1_OFF), so that a wavefront reads one k1 column with one
+// coalesced 1 KiB load.
 template <int M>
 struct TwiddleTable {
     static constexpr int R = M / 64;
     static constexpr int PER_LANE = R + 8;
-    static constexpr int TW2_OFF = 64 * PER_LANE + 2 * R;
-    static constexpr int TW1_OFF = TW2_OFF + 8 * 64;
+    static constexpr int TW1_OFF = 64 * PER_LANE + 2 * R;
     static constexpr int COUNT = TW1_OFF + R * 64;
 };
 
@@ -595,10 +579,7 @@ inline void build_twiddles(c64* out) {
             out[TwiddleTable<M>::TW1_OFF + k1 * 64 + b] = t[k1];
         }
         const int b0 = b & 7;
-        for (int k2a = 0; k2a < 8; k2a++) {
-            t[R + k2a] = ex(2.0L * pi * (long double)((b0 * k2a) % 64) / 64.0L);
-            out[TwiddleTable<M>::TW2_OFF + k2a * 64 + b] = t[R + k2a];
-        }
+        for (int k2a = 0; k2a < 8; k2a++) t[R + k2a] = ex(2.0L * pi * (long double)((b0 * k2a) % 64) / 64.0L);
     }
     c64* tw = out + 64 * (R + 8);
     for (int a = 0; a < R; a++) {
