@@ -307,7 +307,8 @@ __global__ __launch_bounds__(1024) void mux_scan_kernel(const int32_t* __restric
 // batch walk the key rows nearly together), so the LDS carries only the
 // transposes. The accumulator lives in registers in the transform's
 // coefficient layout; the negacyclic rotation is staged through the
-// wavefront's transform buffer.
+// wavefront's transform buffer as (ACC_c, -ACC_c, ACC_c), so every rotated
+// read is a plain load.
 template <int L>
 __global__ __launch_bounds__(256, 2) void br4_kernel(BrArgs A, const int32_t* __restrict__ mux_gate,
                                                     const int32_t* __restrict__ n_jobs) {
@@ -315,7 +316,9 @@ __global__ __launch_bounds__(256, 2) void br4_kernel(BrArgs A, const int32_t* __
     using F = Fft<M>;
     constexpr int R = F::R, N = 2 * M, KPL = 2 * L;
     constexpr int LOG2N2 = 11;
-    __shared__ c64 s_fft[4][F::LDS_C64];
+    // per wavefront: the transform buffer, also holding (v, -v, v) for the rotation (3N int32)
+    constexpr int BUF_C64 = (F::LDS_C64 > 3 * N / 4) ? F::LDS_C64 : 3 * N / 4;
+    __shared__ c64 s_fft[4][BUF_C64];
     __shared__ int16_t s_bara[4][1024];
 
     const int lane = threadIdx.x & 63;
@@ -380,25 +383,24 @@ __global__ __launch_bounds__(256, 2) void br4_kernel(BrArgs A, const int32_t* __
         for (int r = 0; r < KPL; r++) {
             const int c = r >= L ? 1 : 0, p = r - c * L;
             if (p == 0) {
-                // T = (X^abar - 1) * ACC_c + gadget offset, through the wave's LDS buffer
+                // T = (X^abar - 1) * ACC_c + gadget offset. The wave's buffer holds
+                // E = (ACC_c, -ACC_c, ACC_c), so (X^abar ACC_c)[j] = E[j + 2N - abar] with
+                // 0 < j + 2N - abar < 3N: one base address, constant offsets, no masks.
                 wave_sync();  // the previous transform is done with the buffer
 #pragma unroll
-                for (int h = 0; h < 2; h++)
-#pragma unroll
-                    for (int s = 0; s < R; s++) fbi[64 * s + lane + h * M] = (int32_t)(c ? acc[1][h * R + s] : acc[0][h * R + s]);
+                for (int q = 0; q < 2 * R; q++) {
+                    const int j = 64 * (q % R) + lane + (q / R) * M;
+                    const uint32_t v = c ? acc[1][q] : acc[0][q];
+                    fbi[j] = (int32_t)v;
+                    fbi[j + N] = (int32_t)(0u - v);
+                    fbi[j + 2 * N] = (int32_t)v;
+                }
                 wave_sync();
-                // all 16 rotated reads in flight, then the differences
+                const int32_t* er = fbi + lane + 2 * N - abar;
 #pragma unroll
-                for (int q = 0; q < 2 * R; q++) {
-                    const int j = 64 * (q % R) + lane + (q / R) * M;
-                    tv[q] = (uint32_t)fbi[(j - abar) & (N - 1)];
-                }
+                for (int q = 0; q < 2 * R; q++) tv[q] = (uint32_t)er[64 * (q % R) + (q / R) * M];
 #pragma unroll
-                for (int q = 0; q < 2 * R; q++) {
-                    const int j = 64 * (q % R) + lane + (q / R) * M;
-                    const uint32_t rot = ((j - abar) & N) ? 0u - tv[q] : tv[q];
-                    tv[q] = rot - (c ? acc[1][q] : acc[0][q]) + offset;
-                }
+                for (int q = 0; q < 2 * R; q++) tv[q] = tv[q] - (c ? acc[1][q] : acc[0][q]) + offset;
             }
             // digit p of the truncating gadget decomposition, tgsw-functions.cpp:337-348
             const int shift = 32 - (p + 1) * Bgbit;
@@ -416,15 +418,14 @@ __global__ __launch_bounds__(256, 2) void br4_kernel(BrArgs A, const int32_t* __
             wave_sync();
             F::fwd3(x, lane, fb);
             // key digit polynomial (i, r): component 0 at q[s * 64], component 1 at q[M + s * 64];
-            // half of component 0 requested before the last stage, the rest as registers free up
+            // component 0 requested before the last stage, component 1 as its products free
+            // the registers
             const c64* q = A.bkf + ((size_t)i * KPL + r) * 2 * M + lane;
             c64 kv[R];
 #pragma unroll
-            for (int s = 0; s < R / 2; s++) kv[s] = q[s * 64];
+            for (int s = 0; s < R; s++) kv[s] = q[s * 64];
             wave_sync();
             F::fwd4(x, lane, fb);
-#pragma unroll
-            for (int s = R / 2; s < R; s++) kv[s] = q[s * 64];
 #pragma unroll
             for (int s = 0; s < R; s++) {
                 cmac(r0[s], x[s], kv[s]);
