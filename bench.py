@@ -109,7 +109,7 @@ def bench_scaling(T, sk, P, sizes, seed):
                                                       d_pool.ptr, None), "gates")
             T.synchronize()
             dt = (time.perf_counter() - t0) / reps
-            row[{2: "four_gate_lockstep", 3: "two_wave_split", 5: "digit_parallel", 0: "auto"}[mode]] = {
+            row[{2: "one_per_wavefront_batched", 3: "two_wave_split", 5: "digit_parallel", 0: "auto"}[mode]] = {
                 "gates_per_s": round(B / dt, 1), "ms": round(dt * 1e3, 3)}
         out[str(B)] = row
         print(f"[bench] batch {B}: {row}", file=sys.stderr, flush=True)
@@ -486,7 +486,7 @@ def main():
             for _ in range(args.steps):
                 step()
             T.synchronize()
-            mode_cmp[{1: "one_wave_per_bootstrap", 2: "four_gate_lockstep"}[mode]] = round(
+            mode_cmp[{1: "one_wave_per_bootstrap", 2: "one_per_wavefront_batched"}[mode]] = round(
                 B * args.steps / (time.perf_counter() - t1), 2)
         T.lib().tfhe_amd_set_launch_config(0, -1, -1)
         print(f"[bench] modes {mode_cmp}", file=sys.stderr, flush=True)

@@ -364,7 +364,8 @@ TFHE_AMD_EXPORT int32_t tfhe_amd_server_get_stats(tfhe_amd_server* srv, tfhe_amd
 TFHE_AMD_EXPORT void tfhe_amd_server_destroy(tfhe_amd_server* srv);
 
 /* Blind-rotation kernel choice. mode: 0 = automatic, 1 = one wavefront per
- * bootstrap, 2 = four bootstraps per workgroup in lockstep, 3 = two
+ * bootstrap, 2 = one wavefront per bootstrap with the key and twiddles read
+ * through the vector memory path (the batched kernel), 3 = two
  * wavefronts per bootstrap, 5 = 4l wavefronts per bootstrap (lowest latency;
  * modes 2, 3, 5 for N = 1024), 4 = the N = 2048 pair-transform kernel (the
  * automatic choice for N = 2048). In automatic mode, for N = 1024, batches of

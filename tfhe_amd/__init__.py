@@ -440,7 +440,7 @@ def decrypt_device(sk, d_cts, d_bits=None, d_phases=None, count=None, stream=Non
 
 
 AUTOTUNE_SIZES = (64, 128, 256, 384, 512, 640, 768, 896, 1024, 1280, 1536)
-AUTOTUNE_MODES = ("digit_parallel", "two_wave_split", "four_gate_lockstep")
+AUTOTUNE_MODES = ("digit_parallel", "two_wave_split", "one_per_wavefront_batched")
 
 
 def autotune(cloud):

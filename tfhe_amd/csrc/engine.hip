@@ -264,7 +264,7 @@ __global__ __launch_bounds__(64, M == 512 ? 2 : 1) void br_kernel(BrArgs A) {
     }
 }
 
-// Job list for the lockstep kernel: jobs [0, count) are the first bootstrap
+// Job list for the blind-rotation kernels: jobs [0, count) are the first bootstrap
 // of every gate; jobs [count, count + n_mux) the second bootstrap of the MUX
 // gates, in gate order. second[g] = job index of gate g's second bootstrap or -1.
 // One block; count is split into contiguous chunks, one per thread.
