@@ -940,54 +940,51 @@ __global__ __launch_bounds__(256, 2) void br2_kernel(BrArgs A, const int32_t* __
 // largest LWE dimension the pair kernel's LDS table holds
 constexpr int BRP_NMAX = 768;
 
-// Blind rotation for N = 2048 with four wavefronts per bootstrap and two
-// bootstraps per 512-thread workgroup. Wavefront (g, c, h): bootstrap g,
-// accumulator component c, half h of the pair-of-wavefronts transform
-// (FftPair); it owns the quarters m = h and h + 2 of ACC_c (coefficients
-// 512 m + 64 a + lane, a < 8). At the start of a key row each wavefront also
-// holds its partner's quarters, so it writes the whole of ACC_c into its own
-// LDS buffer and the rotation, the gadget decomposition and the transform's
-// radix-2 step across the pair (y_h from all four quarters) need no exchange
-// inside the pair. Each wavefront then runs its 512-point transform and
-// multiply-accumulates its bins against key rows (c*l + p), which the
-// workgroup copies into LDS once for both bootstraps; the component partners
-// swap partial sums, each pair inverse-transforms its own component and
-// exchanges the two halves once, and both wavefronts of the pair form the
-// whole update (the partner's old quarters come through the idle key buffer).
-// Seven workgroup barriers per key row for l = 2.
+// Blind rotation for N = 2048 with one bootstrap per 256-thread workgroup of
+// four wavefronts (two workgroups per CU). Wavefront (c, h): accumulator
+// component c, half h of the pair-of-wavefronts transform (FftPair); it owns
+// the quarters m = h and h + 2 of ACC_c (coefficients 512 m + 64 a + lane,
+// a < 8). At the start of a key row each wavefront also holds its partner's
+// quarters, so it writes the whole of ACC_c into its own LDS buffer and the
+// rotation, the gadget decomposition and the transform's radix-2 step across
+// the pair (y_h from all four quarters) need no exchange inside the pair. Each
+// wavefront then runs its 512-point transform and multiply-accumulates its
+// bins against its half of key rows (c*l + p), read straight from global
+// memory (cached); the component partners swap partial sums, each pair
+// inverse-transforms its own component and exchanges the two halves once, and
+// both wavefronts of the pair form the whole update (the partner's old
+// quarters come through s_xacc). Four workgroup barriers per key row; the two
+// workgroups on a CU never wait for each other.
 template <int L>
-__global__ __launch_bounds__(512, 1) void brp_kernel(BrArgs A, const c64* __restrict__ bkp, const c64* __restrict__ twp,
+__global__ __launch_bounds__(256, 2) void brp_kernel(BrArgs A, const c64* __restrict__ bkp, const c64* __restrict__ twp,
                                                     const int32_t* __restrict__ mux_gate,
                                                     const int32_t* __restrict__ n_jobs) {
     using F = Fft<512>;
     using P = FftPair;
     constexpr int R = 8, N = 2048, KPL = 2 * L, LOG2N2 = 12;
     constexpr int NMAX = BRP_NMAX;
-    __shared__ c64 s_fft[8][F::LDS_C64];
-    __shared__ c64 s_bk[2][2048];     // key digit blocks (i, p), (i, L + p): [col][h][slot][lane]
-    __shared__ c64 s_tw1[2][R * 64];  // [h][k1 * 64 + lane]
-    __shared__ int16_t s_bara[2][NMAX];
-    __shared__ int32_t s_barb[2];
+    __shared__ c64 s_fft[4][F::LDS_C64];
+    __shared__ int32_t s_xacc[4][2 * R * 64];  // wavefront w's old quarters of ACC_c, for its partner
+    __shared__ c64 s_tw1[2][R * 64];           // [h][k1 * 64 + lane]
+    __shared__ int16_t s_bara[NMAX];
+    __shared__ int32_t s_barb;
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int w = tid >> 6;
-    const int g = w >> 2, c = (w >> 1) & 1, h = w & 1;
-    const int njobs = *n_jobs;
-    const int job = blockIdx.x * 2 + g;
-    if (blockIdx.x * 2 >= njobs) return;  // whole block idle (uniform)
-    const bool active = job < njobs;
+    const int c = w >> 1, h = w & 1;
+    const int job = blockIdx.x;
+    if (job >= *n_jobs) return;  // whole block idle (uniform)
     const int n = A.n;
-    int16_t* bara = s_bara[g];
+    int16_t* bara = s_bara;
     c64* fb = s_fft[w];
     int32_t* fbi = (int32_t*)fb;
-    // while no key block is in flight: slot of wavefront w for its old quarters of ACC_c
-    int32_t* const acc_out = (int32_t*)&s_bk[0][0] + w * (2 * R * 64);
-    const int32_t* const acc_in = (const int32_t*)&s_bk[0][0] + (w ^ 1) * (2 * R * 64);
+    int32_t* const acc_out = s_xacc[w];
+    const int32_t* const acc_in = s_xacc[w ^ 1];
 
     // gate linear form and modulus switch, one wavefront per bootstrap
     if (c == 1 && h == 0) {
-        if (active) {
+        {
             const int gate = job < A.count ? job : mux_gate[job - A.count];
             const int half = job < A.count ? 0 : 1;
             const int op = A.op[gate];
@@ -1003,14 +1000,11 @@ __global__ __launch_bounds__(512, 1) void brp_kernel(BrArgs A, const c64* __rest
                 // modSwitchFromTorus32(v, 2N), libtfhe/numeric-functions.cpp:59-65
                 const uint32_t bar = (v + (1u << (31 - LOG2N2))) >> (32 - LOG2N2);
                 if (i < n) bara[i] = (int16_t)bar;
-                else s_barb[g] = (int32_t)bar;
+                else s_barb = (int32_t)bar;
             }
-        } else {
-            for (int i = lane; i < n; i += 64) bara[i] = 0;
-            if (lane == 0) s_barb[g] = 0;
         }
     }
-    for (int k = tid; k < 2 * R * 64; k += 512) {
+    for (int k = tid; k < 2 * R * 64; k += 256) {
         const int hh = k >> 9, k1 = (k >> 6) & 7, l = k & 63;
         s_tw1[hh][k1 * 64 + l] = twp[(hh * 64 + l) * 16 + k1];
     }
@@ -1023,13 +1017,12 @@ __global__ __launch_bounds__(512, 1) void brp_kernel(BrArgs A, const c64* __rest
 
     // own quarters acc[q * R + a] = ACC_c[512 (h + 2q) + 64 a + lane]; pacc: the partner's
     // quarters (1 - h + 2q), valid from the end of one key row to the start of the next.
-    // A workgroup's idle second bootstrap (odd batch) runs the same steps on zeros.
     uint32_t acc[2 * R], pacc[2 * R];
 #pragma unroll
     for (int k = 0; k < 2 * R; k++) acc[k] = pacc[k] = 0;
-    if (c == 1 && active) {
+    if (c == 1) {
         // ACC_1 = X^{2N-barb} * [mu...mu], lwe-bootstrapping-functions-fft.cpp:146-148
-        const int barb = s_barb[g];
+        const int barb = s_barb;
 #pragma unroll
         for (int q = 0; q < 2; q++)
 #pragma unroll
@@ -1046,25 +1039,12 @@ __global__ __launch_bounds__(512, 1) void brp_kernel(BrArgs A, const c64* __rest
         // digits 1..L-1 of the own and the partner's quarters, two 16-bit fields per word;
         // the next one to use is always dig[0]
         uint32_t dig[L - 1][2][R];
-        // one gadget digit: key blocks to LDS, decomposition, transform, product.
-        // FIRST (digit 0) also rotates ACC_c and decomposes every digit.
+        // one gadget digit: decomposition, transform, product with the key read from global
+        // memory. FIRST (digit 0) also rotates ACC_c and decomposes every digit.
         auto digit = [&](const int p, auto first) {
             constexpr bool FIRST = decltype(first)::value;
-            __syncthreads();  // key blocks free; the partner has read this buffer
-            {
-                // key digit blocks (i, p) and (i, L + p) -> LDS, 8 x 16 B per thread
-                const c64* q0 = bkp + ((size_t)i * KPL + p) * 2048 + tid;
-                const c64* q1 = bkp + ((size_t)i * KPL + L + p) * 2048 + tid;
-                c64* d0 = s_bk[0] + (tid & ~63);
-                c64* d1 = s_bk[1] + (tid & ~63);
-#pragma unroll
-                for (int u = 0; u < 4; u++) {
-                    __builtin_amdgcn_global_load_lds((const void*)(q0 + u * 512),
-                                                     (__attribute__((address_space(3))) void*)(d0 + u * 512), 16, 0, 0);
-                    __builtin_amdgcn_global_load_lds((const void*)(q1 + u * 512),
-                                                     (__attribute__((address_space(3))) void*)(d1 + u * 512), 16, 0, 0);
-                }
-            }
+            if constexpr (FIRST) __syncthreads();  // the partner has read this wavefront's buffer
+            else wave_sync();                      // the previous digit's transform is done with it
             // x: u_{j'} = d_{j'} + i d_{j'+1024} of the own quarters, ov: of the partner's
             c64 x[R], ov[R];
             if constexpr (FIRST) {
@@ -1150,21 +1130,19 @@ __global__ __launch_bounds__(512, 1) void brp_kernel(BrArgs A, const c64* __rest
             F::fwd2(x, lane, fb, tw2);
             wave_sync();
             F::fwd3(x, lane, fb);
-            wave_sync();
-            F::fwd4(x, lane, fb);
-            __syncthreads();  // key blocks landed
-            const c64* bk = s_bk[c] + h * 512;
-            // eight key values in flight at a time, then their multiply-accumulates
+            // key row (i, c L + p), half h: output component 0 at bk[s * 64], 1 at bk[1024 + s * 64]
+            const c64* bk = bkp + ((size_t)i * KPL + c * L + p) * 2048 + h * 512 + lane;
             c64 kv[R];
 #pragma unroll
-            for (int s = 0; s < R; s++) kv[s] = bk[s * 64 + lane];
+            for (int s = 0; s < R; s++) kv[s] = bk[s * 64];
+            wave_sync();
+            F::fwd4(x, lane, fb);
 #pragma unroll
             for (int s = 0; s < R; s++) {
                 if constexpr (FIRST) p0[s] = cmul_first(x[s], kv[s]);
                 else cmac(p0[s], x[s], kv[s]);
+                kv[s] = bk[1024 + s * 64];
             }
-#pragma unroll
-            for (int s = 0; s < R; s++) kv[s] = bk[1024 + s * 64 + lane];
 #pragma unroll
             for (int s = 0; s < R; s++) {
                 if constexpr (FIRST) p1[s] = cmul_first(x[s], kv[s]);
@@ -1180,6 +1158,7 @@ __global__ __launch_bounds__(512, 1) void brp_kernel(BrArgs A, const c64* __rest
         }
         // swap partial sums with the other component's wavefront of the same half
         {
+            wave_sync();  // the last transform is done with the buffer
             if (c) {
 #pragma unroll
                 for (int s = 0; s < R; s++) fb[s * 64 + lane] = p0[s];
@@ -1188,7 +1167,7 @@ __global__ __launch_bounds__(512, 1) void brp_kernel(BrArgs A, const c64* __rest
                 for (int s = 0; s < R; s++) fb[s * 64 + lane] = p1[s];
             }
         }
-        __syncthreads();  // partials written; every multiply-accumulate is done with the key blocks
+        __syncthreads();  // partials written
         c64 x[R];
         {
 #pragma unroll
@@ -1237,7 +1216,6 @@ __global__ __launch_bounds__(512, 1) void brp_kernel(BrArgs A, const c64* __rest
             }
         }
     }
-    if (!active) return;
     // tLweExtractLweSampleIndex(.., 0, ..), libtfhe/lwe.cpp:41-56
     int32_t* e = A.ext + (size_t)job * (N + 1);
     if (c == 0) {
@@ -1802,9 +1780,9 @@ int gates_device_locked(Ctx& C, tfhe_amd_device_key* dk, int count, const int32_
         if (dk->P.l == 3) hipLaunchKernelGGL(brl_kernel<3>, dim3(blocks), dim3(768), 0, s, A, dk->bkh, dk->twh, mux_gate, n_jobs);
         else hipLaunchKernelGGL(brl_kernel<2>, dim3(blocks), dim3(512), 0, s, A, dk->bkh, dk->twh, mux_gate, n_jobs);
     } else if (mode == 4) {
-        const int blocks = count;  // two bootstraps per workgroup, at most 2*count jobs
-        if (dk->P.l == 2) hipLaunchKernelGGL(brp_kernel<2>, dim3(blocks), dim3(512), 0, s, A, dk->bkp, dk->twp, mux_gate, n_jobs);
-        else hipLaunchKernelGGL(brp_kernel<3>, dim3(blocks), dim3(512), 0, s, A, dk->bkp, dk->twp, mux_gate, n_jobs);
+        const int blocks = 2 * count;  // one bootstrap per workgroup, at most 2*count jobs
+        if (dk->P.l == 2) hipLaunchKernelGGL(brp_kernel<2>, dim3(blocks), dim3(256), 0, s, A, dk->bkp, dk->twp, mux_gate, n_jobs);
+        else hipLaunchKernelGGL(brp_kernel<3>, dim3(blocks), dim3(256), 0, s, A, dk->bkp, dk->twp, mux_gate, n_jobs);
     } else if (mode == 2) {
         const int blocks = (2 * count + 3) / 4;
         if (dk->P.l == 3) hipLaunchKernelGGL(br4_kernel<3>, dim3(blocks), dim3(256), 0, s, A, mux_gate, n_jobs);
