@@ -370,7 +370,9 @@ __global__ __launch_bounds__(256, 2) void br4_kernel(BrArgs A, const int32_t* __
     const c64 tw2 = A.tw[lane * (R + 8) + R + 1];
     const uint32_t offset = A.offset;
     const int Bgbit = A.Bgbit;
-    const int32_t dhalf = 1 << (Bgbit - 1);
+    // the top bit of every digit field: with it flipped, the signed field is the balanced digit
+    uint32_t flip = 0;
+    for (int p = 0; p < L; p++) flip |= 1u << (31 - p * Bgbit);
     wave_sync();  // bara written by the whole wavefront
 
     for (int i = 0; i < n; i++) {
@@ -400,15 +402,16 @@ __global__ __launch_bounds__(256, 2) void br4_kernel(BrArgs A, const int32_t* __
 #pragma unroll
                 for (int q = 0; q < 2 * R; q++) tv[q] = (uint32_t)er[64 * (q % R) + (q / R) * M];
 #pragma unroll
-                for (int q = 0; q < 2 * R; q++) tv[q] = tv[q] - (c ? acc[1][q] : acc[0][q]) + offset;
+                for (int q = 0; q < 2 * R; q++) tv[q] = (tv[q] - (c ? acc[1][q] : acc[0][q]) + offset) ^ flip;
             }
-            // digit p of the truncating gadget decomposition, tgsw-functions.cpp:337-348
+            // digit p of the truncating gadget decomposition, tgsw-functions.cpp:337-348:
+            // ((T + offset) >> shift & (Bg - 1)) - Bg/2 = the signed field of the flipped word
             const int shift = 32 - (p + 1) * Bgbit;
             c64 x[R];
 #pragma unroll
             for (int s = 0; s < R; s++) {
-                const int32_t d0 = (int32_t)__builtin_amdgcn_ubfe(tv[s], shift, Bgbit) - dhalf;
-                const int32_t d1 = (int32_t)__builtin_amdgcn_ubfe(tv[R + s], shift, Bgbit) - dhalf;
+                const int32_t d0 = __builtin_amdgcn_sbfe((int32_t)tv[s], shift, Bgbit);
+                const int32_t d1 = __builtin_amdgcn_sbfe((int32_t)tv[R + s], shift, Bgbit);
                 x[s] = c64{(double)d0, (double)d1};
             }
             wave_sync();  // rotated reads (p = 0) or the previous transform done with the buffer
@@ -496,7 +499,7 @@ __global__ __launch_bounds__(4 * L * 64, 1) void brl_kernel(BrArgs A, const c64*
     __shared__ c64 s_par[W][256];         // product for output 1
     __shared__ c64 s_inv[4][H::LDS_C64];  // inverse transforms of wavefronts (c, 0, h)
     __shared__ int32_t s_acc[2][N];
-    __shared__ uint32_t s_tv[2][N];  // (X^abar - 1) * ACC + gadget offset, both components
+    __shared__ uint32_t s_tv[2][N];  // (X^abar - 1) * ACC + gadget offset, digit tops flipped
     __shared__ int16_t s_bara[BRL_NMAX];
     __shared__ int32_t s_barb;
 
@@ -533,7 +536,9 @@ __global__ __launch_bounds__(4 * L * 64, 1) void brl_kernel(BrArgs A, const c64*
     for (int k = 0; k < HALF_TW_PER_LANE; k++) tw[k] = twh[(h * 64 + lane) * HALF_TW_PER_LANE + k];
     const uint32_t offset = A.offset;
     const int Bgbit = A.Bgbit;
-    const int32_t dhalf = 1 << (Bgbit - 1);
+    // the top bit of every digit field: with it flipped, the signed field is the balanced digit
+    uint32_t flip = 0;
+    for (int pp = 0; pp < L; pp++) flip |= 1u << (31 - pp * Bgbit);
     __syncthreads();
     // ACC = (0, X^{2N-barb} * [mu...mu]), lwe-bootstrapping-functions-fft.cpp:146-148.
     // Wavefront (c, 0, h) owns coefficients lane + 64 a + 256 h (+ 512) of ACC_c in
@@ -588,7 +593,7 @@ __global__ __launch_bounds__(4 * L * 64, 1) void brl_kernel(BrArgs A, const c64*
 #pragma unroll
             for (int u = 0; u < PER; u++) {
                 const int j2 = tid + u * W * 64, cc = (j2 >> 10) & 1, j = j2 & (N - 1);
-                if (j2 < 2 * N) s_tv[cc][j] = ((((j - abar) & (2 * N - 1)) & N) ? 0u - rv[u] : rv[u]) - av[u] + offset;
+                if (j2 < 2 * N) s_tv[cc][j] = (((((j - abar) & (2 * N - 1)) & N) ? 0u - rv[u] : rv[u]) - av[u] + offset) ^ flip;
             }
         }
         __syncthreads();
@@ -606,7 +611,7 @@ __global__ __launch_bounds__(4 * L * 64, 1) void brl_kernel(BrArgs A, const c64*
             // are formed exactly in int32, then one FMA per component
             int32_t d[4];
 #pragma unroll
-            for (int q = 0; q < 4; q++) d[q] = (int32_t)__builtin_amdgcn_ubfe(tv[q * R + a], shift, Bgbit) - dhalf;
+            for (int q = 0; q < 4; q++) d[q] = __builtin_amdgcn_sbfe((int32_t)tv[q * R + a], shift, Bgbit);
             const double sq = h == 0 ? H::SQ : -H::SQ;
             x[a] = c64{fma(sq, (double)(d[1] - d[3]), (double)d[0]), fma(sq, (double)(d[1] + d[3]), (double)d[2])};
         }
@@ -803,7 +808,9 @@ __global__ __launch_bounds__(256, 2) void br2_kernel(BrArgs A, const int32_t* __
     const c64 tw2 = A.tw[lane * (R + 8) + R + 1];
     const uint32_t offset = A.offset;
     const int Bgbit = A.Bgbit;
-    const int32_t dhalf = 1 << (Bgbit - 1);
+    // the top bit of every digit field: with it flipped, the signed field is the balanced digit
+    uint32_t flip = 0;
+    for (int pp = 0; pp < L; pp++) flip |= 1u << (31 - pp * Bgbit);
     __syncthreads();
 
     for (int i = 0; i < n; i++) {
@@ -846,7 +853,7 @@ __global__ __launch_bounds__(256, 2) void br2_kernel(BrArgs A, const int32_t* __
                             const int idx = (j - abar) & (2 * N - 1);
                             const uint32_t v = (uint32_t)fbi[idx & (N - 1)];
                             const uint32_t rot = (idx & N) ? 0u - v : v;
-                            tv[h * R + s] = rot - acc[h * R + s] + offset;
+                            tv[h * R + s] = (rot - acc[h * R + s] + offset) ^ flip;
                         }
                     wave_sync();
                 }
@@ -854,8 +861,8 @@ __global__ __launch_bounds__(256, 2) void br2_kernel(BrArgs A, const int32_t* __
                 const int shift = 32 - (p + 1) * Bgbit;
 #pragma unroll
                 for (int s = 0; s < R; s++) {
-                    const int32_t d0v = (int32_t)__builtin_amdgcn_ubfe(tv[s], shift, Bgbit) - dhalf;
-                    const int32_t d1v = (int32_t)__builtin_amdgcn_ubfe(tv[R + s], shift, Bgbit) - dhalf;
+                    const int32_t d0v = __builtin_amdgcn_sbfe((int32_t)tv[s], shift, Bgbit);
+                    const int32_t d1v = __builtin_amdgcn_sbfe((int32_t)tv[R + s], shift, Bgbit);
                     x[s] = c64{(double)d0v, (double)d1v};
                 }
                 F::template fwd1<64>(x, lane, fb, tw1);
