@@ -1,21 +1,31 @@
 // GPU engine for TFHE gate bootstrapping on MI355X (gfx950).
 //
-// Kernels:
-//   bk_convert_kernel  coefficient-domain bootstrapping key -> FP64 transform
-//                      domain (replaces init_LweBootstrappingKeyFFT,
+// Kernels (blind rotation = gate linear form + modulus switch, test-vector
+// rotation, n external products, sample extraction: tfhe_bootstrap_woKS_FFT /
+// tfhe_blindRotateAndExtract_FFT / tfhe_blindRotate_FFT,
+// libtfhe/lwe-bootstrapping-functions-fft.cpp:67-200):
+//   br_kernel          blind rotation, one bootstrap per 64-lane wavefront, the
+//                      accumulator in LDS (the explicit-accumulator entry points)
+//   br4_kernel         blind rotation for large batches: one bootstrap per
+//                      wavefront, four per workgroup, key read from global memory
+//   br2_kernel         blind rotation, two wavefronts per bootstrap (one per TLWE
+//                      component), key blocks shared through LDS
+//   brl_kernel         blind rotation for one gate: 4l wavefronts per bootstrap on
+//                      half transforms
+//   brp_kernel         blind rotation for N = 2048: four wavefronts per bootstrap,
+//                      1024-point transforms on pairs of wavefronts
+//   bk_convert_*       coefficient-domain bootstrapping key -> FP64 transform
+//                      domain in each kernel's layout (replaces
+//                      init_LweBootstrappingKeyFFT,
 //                      libtfhe/lwe-bootstrapping-functions-fft.cpp:23-52)
-//   br_kernel          one bootstrap per 64-lane wavefront: gate linear form +
-//                      modulus switch, test-vector rotation, n external
-//                      products with the accumulator in LDS, sample extraction
-//                      (tfhe_bootstrap_woKS_FFT / tfhe_blindRotateAndExtract_FFT /
-//                      tfhe_blindRotate_FFT, lwe-bootstrapping-functions-fft.cpp:67-200)
 //   ks_prep_kernel     sums the two halves of MUX gates and transposes the
 //                      extracted masks for the keyswitch
-//   ks_kernel          exact int32 keyswitch, lanes over gates: key rows staged
-//                      in LDS per block so each row read from HBM/L2 serves
-//                      up to 256 gates (lweKeySwitch,
-//                      lwe-keyswitch-functions.cpp:114-129,228-238)
-//   ks_reduce_kernel   adds the partial sums of the input-coefficient splits
+//   ks_planes_kernel,  exact keyswitch as an int8 matrix-core product over byte
+//   ks_mma_kernel      planes of the key (lweKeySwitch,
+//                      lwe-keyswitch-functions.cpp:114-129,228-238), sliced over
+//                      the input coefficients for small batches
+//   ks_kernel,         exact int32 keyswitch with lanes over gates, key rows
+//   ks_reduce_kernel   staged in LDS (the integer path)
 #include <hip/hip_runtime.h>
 
 #include <cstdarg>
