@@ -59,6 +59,8 @@ def parse():
     ap.add_argument("--serve-max-requests", type=int, default=160)
     ap.add_argument("--pbig-launches", type=int, default=64,
                     help="N=2048 mixed-batch section: launches (0 = skip)")
+    ap.add_argument("--host-api", type=int, default=4096,
+                    help="host gate API section: gates per bootsGate_batch call on host LweSample arrays (0 = skip)")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default small-batch crossover")
     ap.add_argument("--compare-modes", action="store_true",
                     help="also time the one-wavefront-per-bootstrap kernel on the same batch")
@@ -375,6 +377,47 @@ def bench_client(T, sk, P, count, seed):
     return out
 
 
+def bench_host_api(T, sk, P, B, seed):
+    """The reference-style host entry points, PCIe included: bootsGate_batch on host
+    LweSample arrays (the library stages the inputs to HBM, runs the batch and copies
+    the outputs back) and bootsNAND, one gate per call."""
+    L = T.lib()
+    rng = np.random.default_rng(seed + 7)
+    ops = rng.integers(0, 6, size=B).astype(np.int32)
+    bits = rng.integers(0, 2, size=(2, B)).astype(np.int32)
+    H = T.HostCiphertexts(P, 3 * B)
+    try:
+        H.set(np.concatenate([sk.encrypt(bits[0]), sk.encrypt(bits[1]), np.zeros((B, P.n + 1), np.int32)]))
+        arr = lambda idx: (T.LSP * B)(*[H.item(i) for i in idx])
+        res, i0, i1 = arr(range(2 * B, 3 * B)), arr(range(B)), arr(range(B, 2 * B))
+        run = lambda: L.bootsGate_batch(B, ops.ctypes.data_as(T.i32p), res, i0, i1, i0, sk.cloud)
+        T.check(run(), "bootsGate_batch")  # warm-up
+        reps = 3
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            T.check(run(), "bootsGate_batch")
+        dt = (time.perf_counter() - t0) / reps
+        lat = []
+        for g in range(16):
+            t1 = time.perf_counter()
+            L.bootsNAND(H.item(2 * B + g), H.item(g), H.item(B + g), sk.cloud)
+            lat.append(time.perf_counter() - t1)
+        out = H.get()[2 * B:]
+    finally:
+        H.close()
+    truth = {0: lambda x, y: 1 - (x & y), 1: lambda x, y: x & y, 2: lambda x, y: x | y, 3: lambda x, y: x ^ y,
+             4: lambda x, y: 1 - (x ^ y), 5: lambda x, y: 1 - (x | y)}
+    exp = np.array([truth[o](x, y) for o, x, y in zip(ops, bits[0], bits[1])])
+    exp[:16] = 1 - (bits[0][:16] & bits[1][:16])  # the first 16 outputs were rewritten by bootsNAND
+    lat_ms = np.array(lat) * 1e3
+    res = {"batch_gates": B, "batch_ms": round(dt * 1e3, 3), "batch_gates_per_s": round(B / dt, 1),
+           "pcie_bytes_per_batch": B * 3 * (P.n + 1) * 4,
+           "boots_nand_p50_ms": round(float(np.percentile(lat_ms, 50)), 4),
+           "decrypt_errors": int((sk.decrypt(out) != exp).sum())}
+    print(f"[bench] host API: {res}", file=sys.stderr, flush=True)
+    return res
+
+
 def cpu_baseline(count, seed):
     """Reference spqlios-FMA CPU path (oracle/_ref, built from the reference's
     own sources) timed on this host; falls back to the oracle port."""
@@ -523,6 +566,7 @@ def main():
     adders = bench_adders(T, sk, P, args.adders, args.adder_bits, 2, args.seed) if args.adders > 0 else None
     serving = bench_serving(T, P, args) if args.serve_clients > 0 else None
     pbig = bench_pbig(T, args) if args.pbig_launches > 0 else None
+    host_api = bench_host_api(T, sk, P, args.host_api, args.seed) if args.host_api > 0 else None
 
     br_avg_s = br_ms / max(launches, 1) / 1e3
     achieved_tflops = FLOP_PER_BOOTSTRAP_P128 * B / br_avg_s / 1e12 if br_avg_s > 0 else None
@@ -568,6 +612,7 @@ def main():
         "client_side": client,
         "serving": serving,
         "pbig_mixed": pbig,
+        "host_api": host_api,
         "kernel_ms": {"blind_rotate_avg": round(br_ms / max(launches, 1), 4),
                       "keyswitch_avg": round(ks_ms / max(launches, 1), 4)},
         "roofline": {"bound": "mfma", "achieved": round(achieved_tflops, 3) if achieved_tflops else None,
