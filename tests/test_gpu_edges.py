@@ -112,3 +112,37 @@ def test_very_large_launch(key):
     assert np.array_equal(sk.decrypt(res), exp)
     sel = np.array([0, 65535, 65536, 131071])
     assert np.array_equal(res[sel], O.gates(keys, ops[sel], cts[i0[sel]], cts[i1[sel]], cts[i2[sel]], threads=8))
+
+
+def test_host_batch_large_and_repeated_outputs(key):
+    """bootsGate_batch on 1100 host LweSample gates (the staging copies split across
+    host threads) equals the device path; with one output named twice the later
+    gate's result stays, as in a serial loop."""
+    T, P, sk, keys = key
+    L = T.lib()
+    rng = np.random.default_rng(95)
+    count = 1100
+    ops = rng.integers(0, 11, size=count).astype(np.int32)
+    bits = rng.integers(0, 2, size=(3, count)).astype(np.int32)
+    a, b, c = (sk.encrypt(x) for x in bits)
+    ref = T.run_gates(sk.cloud, ops, a, b, c)
+    H = T.HostCiphertexts(P, 4 * count)
+    try:
+        H.set(np.concatenate([a, b, c, np.zeros_like(a)]))
+        arr = lambda idx: (T.LSP * count)(*[H.item(i) for i in idx])
+        i0, i1, i2 = arr(range(count)), arr(range(count, 2 * count)), arr(range(2 * count, 3 * count))
+        outs = list(range(3 * count, 4 * count))
+        rc = L.bootsGate_batch(count, ops.ctypes.data_as(T.i32p), arr(outs), i0, i1, i2, sk.cloud)
+        assert rc == 0, L.tfhe_amd_last_error()
+        assert np.array_equal(H.get()[3 * count:], ref)
+        outs[1000] = outs[5]  # gates 5 and 1000 write the same sample
+        H.set(np.concatenate([a, b, c, np.zeros_like(a)]))
+        rc = L.bootsGate_batch(count, ops.ctypes.data_as(T.i32p), arr(outs), i0, i1, i2, sk.cloud)
+        assert rc == 0, L.tfhe_amd_last_error()
+        got = H.get()[3 * count:]
+        assert np.array_equal(got[5], ref[1000])
+        keep = np.ones(count, bool)
+        keep[[5, 1000]] = False
+        assert np.array_equal(got[keep], ref[keep])
+    finally:
+        H.close()

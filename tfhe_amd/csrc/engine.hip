@@ -28,6 +28,7 @@
 //   ks_reduce_kernel   staged in LDS (the integer path)
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
@@ -35,6 +36,7 @@
 #include <type_traits>
 #include <cstring>
 #include <mutex>
+#include <thread>
 #include <unordered_map>
 #include <vector>
 
@@ -2020,6 +2022,25 @@ void engine_forget_host_key(const LweBootstrappingKeyFFT* bkfft) {
 
 // ------------------------------------------------------ host-buffer paths --
 
+// Runs fn(begin, end) over [0, n) on up to 8 host threads for large n (the
+// per-sample copies between LweSample arrays and the pinned staging buffer).
+template <class Fn>
+static void host_parallel(int n, Fn fn) {
+    const int hw = (int)std::thread::hardware_concurrency();
+    const int nt = n < 1024 ? 1 : (hw < 2 ? 1 : (hw < 8 ? hw : 8));
+    if (nt == 1) {
+        fn(0, n);
+        return;
+    }
+    std::vector<std::thread> th;
+    const int per = (n + nt - 1) / nt;
+    for (int t = 0; t < nt; t++) {
+        const int b = t * per, e = b + per < n ? b + per : n;
+        if (b < e) th.emplace_back(fn, b, e);
+    }
+    for (auto& x : th) x.join();
+}
+
 int engine_gates_host(const LweBootstrappingKeyFFT* bf, int32_t count, const int32_t* ops,
                       LweSample* const* out, const LweSample* const* in0, const LweSample* const* in1,
                       const LweSample* const* in2, int32_t mu, bool keyswitch) {
@@ -2067,10 +2088,12 @@ int engine_gates_host(const LweBootstrappingKeyFFT* bf, int32_t count, const int
     char* h = (char*)C.h_pinned;
     memcpy(h, idx.data(), idx.size() * sizeof(int32_t));
     int32_t* hp = (int32_t*)(h + idx_bytes);
-    for (int s = 0; s < nin_slots; s++) {
-        memcpy(hp + (size_t)s * w, uniq[s]->a, sizeof(int32_t) * P.n);
-        hp[(size_t)s * w + P.n] = uniq[s]->b;
-    }
+    host_parallel(nin_slots, [&](int b, int e) {
+        for (int s = b; s < e; s++) {
+            memcpy(hp + (size_t)s * w, uniq[s]->a, sizeof(int32_t) * P.n);
+            hp[(size_t)s * w + P.n] = uniq[s]->b;
+        }
+    });
     // device layout: [idx][pool][ext out] then the engine's own scratch
     const size_t dev_bytes = idx_bytes + pool_bytes + extout_bytes;
     const size_t work = ext_bytes(P, count) + ks_scratch_bytes(P, count);
@@ -2088,12 +2111,24 @@ int engine_gates_host(const LweBootstrappingKeyFFT* bf, int32_t count, const int
     HIP_TRY(hipMemcpyAsync(h + idx_bytes, src, out_bytes, hipMemcpyDeviceToHost, C.stream));
     HIP_TRY(hipStreamSynchronize(C.stream));
     const int32_t* ho = (const int32_t*)(h + idx_bytes);
-    for (int i = 0; i < count; i++) {
-        const int32_t* row = ho + (size_t)i * out_row;
-        LweSample* o = out[i];
-        memcpy(o->a, row, sizeof(int32_t) * (out_row - 1));
-        o->b = row[out_row - 1];
+    auto scatter = [&](int b, int e) {
+        for (int i = b; i < e; i++) {
+            const int32_t* row = ho + (size_t)i * out_row;
+            LweSample* o = out[i];
+            memcpy(o->a, row, sizeof(int32_t) * (out_row - 1));
+            o->b = row[out_row - 1];
+        }
+    };
+    // outputs named twice keep the serial order (the later gate wins), so only
+    // distinct outputs are written from several threads
+    bool distinct = count >= 1024;
+    if (distinct) {
+        std::vector<const LweSample*> sorted(out, out + count);
+        std::sort(sorted.begin(), sorted.end());
+        distinct = std::adjacent_find(sorted.begin(), sorted.end()) == sorted.end();
     }
+    if (distinct) host_parallel(count, scatter);
+    else scatter(0, count);
     return 0;
 }
 
