@@ -146,3 +146,46 @@ def test_host_batch_large_and_repeated_outputs(key):
         assert np.array_equal(got[keep], ref[keep])
     finally:
         H.close()
+
+
+def test_device_api_on_two_streams(key):
+    """Two tfhe_amd_gates_device calls queued back to back on two different HIP
+    streams, with no host synchronisation between them, share the library's
+    workspace: the second waits for the first on the device, so both results
+    equal runs on the library's own stream."""
+    T, P, sk, keys = key
+    hip = ctypes.CDLL("libamdhip64.so")
+    s1, s2 = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipStreamCreate(ctypes.byref(s1)) == 0 and hip.hipStreamCreate(ctypes.byref(s2)) == 0
+    rng = np.random.default_rng(97)
+    count, w = 3000, P.n + 1
+    ops = [rng.integers(0, 11, size=count).astype(np.int32) for _ in range(2)]
+    cts = [sk.encrypt(rng.integers(0, 2, size=3 * count).astype(np.int32)) for _ in range(2)]
+    refs = [T.run_gates(sk.cloud, ops[k], cts[k][:count], cts[k][count:2 * count], cts[k][2 * count:])
+            for k in range(2)]
+    # pool: batch k's inputs at [4k count, 4k count + 3 count), outputs after them
+    pool = np.zeros((8 * count, w), np.int32)
+    for k in range(2):
+        pool[4 * k * count:4 * k * count + 3 * count] = cts[k]
+    base = [np.arange(count, dtype=np.int32) + 4 * k * count for k in range(2)]
+    arrs = [T.DeviceArray.from_numpy(pool)]
+    try:
+        d_pool = arrs[0]
+        calls = []
+        for k in range(2):
+            d = [T.DeviceArray.from_numpy(x) for x in (ops[k], base[k], base[k] + count, base[k] + 2 * count,
+                                                       base[k] + 3 * count)]
+            arrs.extend(d)
+            calls.append(d)
+        T.synchronize()
+        for k, s in enumerate((s1, s2)):
+            T.gates_device(sk.cloud, *calls[k], d_pool, stream=s.value)
+        assert hip.hipStreamSynchronize(s1) == 0 and hip.hipStreamSynchronize(s2) == 0
+        got = d_pool.numpy()
+    finally:
+        for d in arrs:
+            d.free()
+        hip.hipStreamDestroy(s1)
+        hip.hipStreamDestroy(s2)
+    for k in range(2):
+        assert np.array_equal(got[4 * k * count + 3 * count:4 * (k + 1) * count], refs[k]), k

@@ -1570,6 +1570,11 @@ struct Ctx {
     int br_mode = 0;
     int brl_max = 256;        // automatic mode: batches up to this many gates use the latency kernel (4l wavefronts per bootstrap)
     int br_small_max = 1024;  // automatic mode: batches up to this many gates use two wavefronts per bootstrap
+    // the workspace is shared by every call that brings none: the last stream that used
+    // it and an event recorded after that use (see ensure_scratch / scratch_done)
+    hipStream_t scratch_stream = nullptr;
+    hipEvent_t scratch_ev = nullptr;
+    bool scratch_busy = false;
     // optional per-launch timing with HIP events on the launch stream
     bool prof = false;
     struct Ev { hipEvent_t e[3]; };
@@ -1592,18 +1597,39 @@ int ensure_ready(Ctx& C) {
     return 0;
 }
 
-int ensure_scratch(Ctx& C, size_t bytes) {
-    if (C.scratch.cap >= bytes) return 0;
-    if (C.scratch.d) {
-        // work queued on any stream may still use the old buffer
-        HIP_TRY(hipDeviceSynchronize());
-        HIP_TRY(hipFree(C.scratch.d));
-        C.scratch.d = nullptr;
-        C.scratch.cap = 0;
+// Makes the context's workspace at least `bytes` long for work about to be queued
+// on stream s. A call on another stream than the workspace's last user first
+// waits, on the device, for that use to finish (scratch_done recorded it), so
+// calls on different streams never overlap in the workspace; the host does not
+// block. No wait is inserted into a stream under graph capture.
+int ensure_scratch(Ctx& C, size_t bytes, hipStream_t s) {
+    if (C.scratch.cap < bytes) {
+        if (C.scratch.d) {
+            // work queued on any stream may still use the old buffer
+            HIP_TRY(hipDeviceSynchronize());
+            HIP_TRY(hipFree(C.scratch.d));
+            C.scratch.d = nullptr;
+            C.scratch.cap = 0;
+            C.scratch_busy = false;
+        }
+        size_t cap = bytes + bytes / 2 + (1 << 20);
+        HIP_TRY(hipMalloc(&C.scratch.d, cap));
+        C.scratch.cap = cap;
     }
-    size_t cap = bytes + bytes / 2 + (1 << 20);
-    HIP_TRY(hipMalloc(&C.scratch.d, cap));
-    C.scratch.cap = cap;
+    if (C.scratch_busy && C.scratch_stream != s) {
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        HIP_TRY(hipStreamIsCapturing(s, &cs));
+        if (cs == hipStreamCaptureStatusNone) HIP_TRY(hipStreamWaitEvent(s, C.scratch_ev, 0));
+    }
+    return 0;
+}
+
+// Records that work queued on stream s uses the workspace (after its launches).
+int scratch_done(Ctx& C, hipStream_t s) {
+    if (!C.scratch_ev) HIP_TRY(hipEventCreateWithFlags(&C.scratch_ev, hipEventDisableTiming));
+    HIP_TRY(hipEventRecord(C.scratch_ev, s));
+    C.scratch_stream = s;
+    C.scratch_busy = true;
     return 0;
 }
 
@@ -2097,7 +2123,7 @@ int engine_gates_host(const LweBootstrappingKeyFFT* bf, int32_t count, const int
     // device layout: [idx][pool][ext out] then the engine's own scratch
     const size_t dev_bytes = idx_bytes + pool_bytes + extout_bytes;
     const size_t work = ext_bytes(P, count) + ks_scratch_bytes(P, count);
-    if (int rc = ensure_scratch(C, dev_bytes + work)) return rc;
+    if (int rc = ensure_scratch(C, dev_bytes + work, C.stream)) return rc;
     // the engine's working buffers follow the staged region
     char* d = (char*)C.scratch.d;
     HIP_TRY(hipMemcpyAsync(d, h, idx_bytes + pool_in_bytes, hipMemcpyHostToDevice, C.stream));
@@ -2110,6 +2136,7 @@ int engine_gates_host(const LweBootstrappingKeyFFT* bf, int32_t count, const int
     const void* src = keyswitch ? (const void*)(d_pool + (size_t)nin_slots * w) : (const void*)d_extout;
     HIP_TRY(hipMemcpyAsync(h + idx_bytes, src, out_bytes, hipMemcpyDeviceToHost, C.stream));
     HIP_TRY(hipStreamSynchronize(C.stream));
+    C.scratch_busy = false;  // the workspace is idle again
     const int32_t* ho = (const int32_t*)(h + idx_bytes);
     auto scatter = [&](int b, int e) {
         for (int i = b; i < e; i++) {
@@ -2143,7 +2170,7 @@ int engine_blind_rotate_host(tfhe_amd_device_key* dk, int32_t row0, int32_t n, c
     const size_t bara_bytes = align256(sizeof(int32_t) * 1024);
     const size_t ext_b = align256(sizeof(int32_t) * nx);
     const size_t total = 2 * acc_bytes + bara_bytes + ext_b;
-    if (int rc = ensure_scratch(C, total)) return rc;
+    if (int rc = ensure_scratch(C, total, C.stream)) return rc;
     if (int rc = ensure_pinned(C, total)) return rc;
     char* h = (char*)C.h_pinned;
     memcpy(h, acc, sizeof(int32_t) * 2 * N);
@@ -2161,6 +2188,7 @@ int engine_blind_rotate_host(tfhe_amd_device_key* dk, int32_t row0, int32_t n, c
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(h, d + acc_bytes + bara_bytes, acc_bytes + ext_b, hipMemcpyDeviceToHost, C.stream));
     HIP_TRY(hipStreamSynchronize(C.stream));
+    C.scratch_busy = false;
     memcpy(acc, h, sizeof(int32_t) * 2 * N);
     if (ext_out) memcpy(ext_out, h + acc_bytes, sizeof(int32_t) * nx);
     return 0;
@@ -2173,7 +2201,7 @@ int engine_keyswitch_host(tfhe_amd_device_key* dk, const int32_t* in, int32_t* o
     const int nx = P.k * P.N + 1, w = P.n + 1;
     const size_t in_b = align256(sizeof(int32_t) * 2 * nx), out_b = align256(sizeof(int32_t) * w);
     const size_t work = ks_scratch_bytes(P, 1);
-    if (int rc = ensure_scratch(C, in_b + out_b + work)) return rc;
+    if (int rc = ensure_scratch(C, in_b + out_b + work, C.stream)) return rc;
     if (int rc = ensure_pinned(C, in_b + out_b)) return rc;
     char* h = (char*)C.h_pinned;
     memcpy(h, in, sizeof(int32_t) * nx);
@@ -2183,6 +2211,7 @@ int engine_keyswitch_host(tfhe_amd_device_key* dk, const int32_t* in, int32_t* o
         return rc;
     HIP_TRY(hipMemcpyAsync(h, d + in_b, sizeof(int32_t) * w, hipMemcpyDeviceToHost, C.stream));
     HIP_TRY(hipStreamSynchronize(C.stream));
+    C.scratch_busy = false;
     memcpy(out, h, sizeof(int32_t) * w);
     return 0;
 }
@@ -2215,9 +2244,12 @@ TFHE_AMD_EXPORT int32_t tfhe_amd_gates_device(const TFheGateBootstrappingCloudKe
     Ctx& C = ctx();
     std::lock_guard<std::mutex> g(C.mu);
     if (count < 0) return set_error("negative gate count");
-    if (int rc = ensure_scratch(C, ext_bytes(dk->P, count) + ks_scratch_bytes(dk->P, count))) return rc;
-    return gates_device_locked(C, dk, count, d_op, d_in0, d_in1, d_in2, d_out, d_pool, nullptr, 1 << 29,
-                               (char*)C.scratch.d, stream ? (hipStream_t)stream : C.stream);
+    const hipStream_t s = stream ? (hipStream_t)stream : C.stream;
+    if (int rc = ensure_scratch(C, ext_bytes(dk->P, count) + ks_scratch_bytes(dk->P, count), s)) return rc;
+    if (int rc = gates_device_locked(C, dk, count, d_op, d_in0, d_in1, d_in2, d_out, d_pool, nullptr, 1 << 29,
+                                     (char*)C.scratch.d, s))
+        return rc;
+    return scratch_done(C, s);
 }
 
 TFHE_AMD_EXPORT int32_t tfhe_amd_gates_device_woks(const TFheGateBootstrappingCloudKeySet* ck, int32_t count,
@@ -2230,9 +2262,12 @@ TFHE_AMD_EXPORT int32_t tfhe_amd_gates_device_woks(const TFheGateBootstrappingCl
     Ctx& C = ctx();
     std::lock_guard<std::mutex> g(C.mu);
     if (count < 0) return set_error("negative gate count");
-    if (int rc = ensure_scratch(C, ext_bytes(dk->P, count) + ks_scratch_bytes(dk->P, count))) return rc;
-    return gates_device_locked(C, dk, count, d_op, d_in0, d_in1, d_in2, nullptr, const_cast<int32_t*>(d_pool), d_ext, 1 << 29,
-                               (char*)C.scratch.d, stream ? (hipStream_t)stream : C.stream);
+    const hipStream_t s = stream ? (hipStream_t)stream : C.stream;
+    if (int rc = ensure_scratch(C, ext_bytes(dk->P, count) + ks_scratch_bytes(dk->P, count), s)) return rc;
+    if (int rc = gates_device_locked(C, dk, count, d_op, d_in0, d_in1, d_in2, nullptr, const_cast<int32_t*>(d_pool),
+                                     d_ext, 1 << 29, (char*)C.scratch.d, s))
+        return rc;
+    return scratch_done(C, s);
 }
 
 TFHE_AMD_EXPORT int32_t tfhe_amd_keyswitch_device(const TFheGateBootstrappingCloudKeySet* ck, int32_t count,
@@ -2244,9 +2279,10 @@ TFHE_AMD_EXPORT int32_t tfhe_amd_keyswitch_device(const TFheGateBootstrappingClo
     Ctx& C = ctx();
     std::lock_guard<std::mutex> g(C.mu);
     // run_keyswitch reads ext rows [2*count][..]; with op == null only the first count are used
-    if (int rc = ensure_scratch(C, ks_scratch_bytes(dk->P, count))) return rc;
-    return run_keyswitch(dk, d_in, nullptr, count, d_out, nullptr, (char*)C.scratch.d,
-                         stream ? (hipStream_t)stream : C.stream);
+    const hipStream_t s = stream ? (hipStream_t)stream : C.stream;
+    if (int rc = ensure_scratch(C, ks_scratch_bytes(dk->P, count), s)) return rc;
+    if (int rc = run_keyswitch(dk, d_in, nullptr, count, d_out, nullptr, (char*)C.scratch.d, s)) return rc;
+    return scratch_done(C, s);
 }
 
 TFHE_AMD_EXPORT void tfhe_amd_set_launch_config(int32_t br_mode, int32_t latency_max, int32_t small_batch_max) {
@@ -2322,7 +2358,7 @@ TFHE_AMD_EXPORT int32_t tfhe_amd_autotune(const TFheGateBootstrappingCloudKeySet
         hipMemcpyAsync(d_idx, idx.data(), sizeof(int32_t) * idx.size(), hipMemcpyHostToDevice, C.stream) != hipSuccess ||
         hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
         return cleanup(set_error("autotune: set-up failed"));
-    if (int rc = ensure_scratch(C, ext_bytes(dk->P, maxB) + ks_scratch_bytes(dk->P, maxB))) return cleanup(rc);
+    if (int rc = ensure_scratch(C, ext_bytes(dk->P, maxB) + ks_scratch_bytes(dk->P, maxB), C.stream)) return cleanup(rc);
     const int saved = C.br_mode;
     float best[3][NS];
     int rc = 0;
