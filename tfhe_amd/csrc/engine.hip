@@ -2025,7 +2025,8 @@ tfhe_amd_device_key* engine_key(const LweBootstrappingKeyFFT* bf0) {
 
 static void free_key(Ctx& C, tfhe_amd_device_key* dk) {
     if (!dk) return;
-    if (C.stream) (void)hipStreamSynchronize(C.stream);
+    // launches queued on any stream (the caller's, a server's) may still read the key
+    if (C.ready) (void)hipDeviceSynchronize();
     (void)hipFree(dk->bkf);
     (void)hipFree(dk->ksk);
     (void)hipFree(dk->tw);
