@@ -246,14 +246,34 @@ def bench_pbig(T, args):
              lambda a, b, c: b if a else c]
     exp = np.array([truth[o](x, y, z) for o, x, y, z in zip(ops, bits[0], bits[1], bits[2])])
 
-    def run(sizes):
+    def run(sizes, streams=(None,)):
         off = 0
-        for sz in sizes:
+        for k, sz in enumerate(sizes):
             sz = int(sz)
             T.check(T.lib().tfhe_amd_gates_device(
                 sk.cloud, sz, d_ops.ptr + i32 * off, d_idx.ptr + i32 * off, d_idx.ptr + i32 * (2 * G + off),
-                d_idx.ptr + i32 * (4 * G + off), d_idx.ptr + i32 * (6 * G + off), d_pool.ptr, None), "gates")
+                d_idx.ptr + i32 * (4 * G + off), d_idx.ptr + i32 * (6 * G + off), d_pool.ptr,
+                streams[k % len(streams)]), "gates")
             off = (off + sz) % G
+
+    # the same launches spread round-robin over four HIP streams: every stream has a
+    # workspace of its own, so consecutive launches overlap on the device (wall time only)
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    hs = [ctypes.c_void_p() for _ in range(4)]
+    for h in hs:
+        T.check(hip.hipStreamCreate(ctypes.byref(h)), "hipStreamCreate")
+    streams = tuple(h.value for h in hs)
+
+    def timed_streams(sizes):
+        T.synchronize()
+        t0 = time.perf_counter()
+        run(sizes, streams)
+        for h in hs:
+            T.check(hip.hipStreamSynchronize(h), "hipStreamSynchronize")
+        dt = time.perf_counter() - t0
+        gates = int(np.sum(sizes))
+        return {"gates": gates, "gates_per_s": round(gates / dt, 1), "ms_total": round(dt * 1e3, 2), "streams": 4}
 
     def timed(sizes):
         T.synchronize()
@@ -290,6 +310,15 @@ def bench_pbig(T, args):
                                                        "max": int(scaled.max())}),
            "batch_8192": timed([8192]),
            "decrypt_errors": errors}
+    run(scaled, streams)  # warm-up of the streams' workspaces
+    out["drawn_4_streams"] = timed_streams(drawn)
+    out["scaled_4_streams"] = timed_streams(scaled)
+    T.synchronize()
+    for h in hs:
+        hip.hipStreamSynchronize(h)
+    out["decrypt_errors_4_streams"] = int((sk.decrypt(d_pool.numpy()[3 * G:]) != exp).sum())
+    for h in hs:
+        hip.hipStreamDestroy(h)
     print(f"[bench] N=2048 mixed: {out}", file=sys.stderr, flush=True)
     for d in (d_pool, d_ops, d_idx):
         d.free()

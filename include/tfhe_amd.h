@@ -228,11 +228,12 @@ TFHE_AMD_EXPORT size_t tfhe_amd_cloud_key_device_bytes(const TFheGateBootstrappi
  * Gate g reads pool[in0[g]], pool[in1[g]] (and pool[in2[g]] for MUX) and
  * writes pool[out[g]]. All index arrays and the pool are device pointers;
  * stream is a hipStream_t (NULL = the library's stream). Asynchronous.
- * These calls and tfhe_amd_keyswitch_device share one workspace: a call on a
- * different stream than the previous one waits on the device (not the host)
- * for the previous call's work, so calls on different streams run one after
- * the other. For work that overlaps, use the serving loop or the netlist API,
- * which own a workspace per stream. */
+ * Each stream passed to these calls (and to tfhe_amd_keyswitch_device) gets a
+ * device workspace of its own, up to 8 streams, so calls on different streams
+ * overlap on the device; further streams share the library stream's
+ * workspace, and a call whose workspace was last used from another stream
+ * waits for that use on the device (never on the host). A workspace is kept
+ * until the process ends. */
 TFHE_AMD_EXPORT int32_t tfhe_amd_gates_device(const TFheGateBootstrappingCloudKeySet* ck, int32_t count,
                                               const int32_t* d_op, const int32_t* d_in0, const int32_t* d_in1,
                                               const int32_t* d_in2, const int32_t* d_out, int32_t* d_pool,

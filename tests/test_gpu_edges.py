@@ -150,9 +150,9 @@ def test_host_batch_large_and_repeated_outputs(key):
 
 def test_device_api_on_two_streams(key):
     """Two tfhe_amd_gates_device calls queued back to back on two different HIP
-    streams, with no host synchronisation between them, share the library's
-    workspace: the second waits for the first on the device, so both results
-    equal runs on the library's own stream."""
+    streams, with no host synchronisation between them, each in the stream's own
+    workspace (so they overlap on the device): both results equal runs on the
+    library's own stream."""
     T, P, sk, keys = key
     hip = ctypes.CDLL("libamdhip64.so")
     s1, s2 = ctypes.c_void_p(), ctypes.c_void_p()
@@ -189,3 +189,43 @@ def test_device_api_on_two_streams(key):
         hip.hipStreamDestroy(s2)
     for k in range(2):
         assert np.array_equal(got[4 * k * count + 3 * count:4 * (k + 1) * count], refs[k]), k
+
+
+def test_device_api_on_ten_streams(key):
+    """Ten streams, one small batch each, queued back to back: the first eight
+    streams get workspaces of their own, the last two share the library stream's
+    (each call waits on the device for the workspace's previous use). Every result
+    equals the run on the library's own stream."""
+    T, P, sk, keys = key
+    hip = ctypes.CDLL("libamdhip64.so")
+    ns, count, w = 10, 300, P.n + 1
+    hs = [ctypes.c_void_p() for _ in range(ns)]
+    for h in hs:
+        assert hip.hipStreamCreate(ctypes.byref(h)) == 0
+    rng = np.random.default_rng(98)
+    ops = rng.integers(0, 11, size=(ns, count)).astype(np.int32)
+    cts = sk.encrypt(rng.integers(0, 2, size=3 * count).astype(np.int32))
+    refs = [T.run_gates(sk.cloud, ops[k], cts[:count], cts[count:2 * count], cts[2 * count:]) for k in range(ns)]
+    pool = np.zeros((3 * count + ns * count, w), np.int32)
+    pool[:3 * count] = cts
+    idx = np.arange(count, dtype=np.int32)
+    arrs = [T.DeviceArray.from_numpy(pool), T.DeviceArray.from_numpy(idx), T.DeviceArray.from_numpy(idx + count),
+            T.DeviceArray.from_numpy(idx + 2 * count)]
+    try:
+        d_pool, d_i0, d_i1, d_i2 = arrs
+        d_ops = [T.DeviceArray.from_numpy(ops[k]) for k in range(ns)]
+        d_out = [T.DeviceArray.from_numpy(idx + (3 + k) * count) for k in range(ns)]
+        arrs += d_ops + d_out
+        T.synchronize()
+        for k in range(ns):
+            T.gates_device(sk.cloud, d_ops[k], d_i0, d_i1, d_i2, d_out[k], d_pool, stream=hs[k].value)
+        for h in hs:
+            assert hip.hipStreamSynchronize(h) == 0
+        got = d_pool.numpy()
+    finally:
+        for d in arrs:
+            d.free()
+        for h in hs:
+            hip.hipStreamDestroy(h)
+    for k in range(ns):
+        assert np.array_equal(got[(3 + k) * count:(4 + k) * count], refs[k]), k

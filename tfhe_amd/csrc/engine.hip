@@ -1554,27 +1554,32 @@ __global__ __launch_bounds__(256) void ks_out_init_kernel(const int32_t* __restr
 
 namespace {
 
+// A device workspace, the stream that used it last and an event recorded after
+// that use.
 struct Scratch {
     void* d = nullptr;
     size_t cap = 0;
+    hipStream_t last = nullptr;
+    hipEvent_t ev = nullptr;
+    bool busy = false;
 };
+
+// caller streams that get a workspace of their own; further streams share the
+// library stream's (ordered, not overlapped)
+constexpr int MAX_STREAM_WORKSPACES = 8;
 
 struct Ctx {
     std::mutex mu;
     bool ready = false;
     int device = 0;
     hipStream_t stream = nullptr;
-    Scratch scratch;
+    Scratch scratch;                                        // the library stream's workspace
+    std::vector<std::pair<hipStream_t, Scratch>> stream_ws;  // caller streams' own workspaces
     void* h_pinned = nullptr;
     size_t h_cap = 0;
     int br_mode = 0;
     int brl_max = 256;        // automatic mode: batches up to this many gates use the latency kernel (4l wavefronts per bootstrap)
     int br_small_max = 1024;  // automatic mode: batches up to this many gates use two wavefronts per bootstrap
-    // the workspace is shared by every call that brings none: the last stream that used
-    // it and an event recorded after that use (see ensure_scratch / scratch_done)
-    hipStream_t scratch_stream = nullptr;
-    hipEvent_t scratch_ev = nullptr;
-    bool scratch_busy = false;
     // optional per-launch timing with HIP events on the launch stream
     bool prof = false;
     struct Ev { hipEvent_t e[3]; };
@@ -1597,39 +1602,54 @@ int ensure_ready(Ctx& C) {
     return 0;
 }
 
-// Makes the context's workspace at least `bytes` long for work about to be queued
-// on stream s. A call on another stream than the workspace's last user first
-// waits, on the device, for that use to finish (scratch_done recorded it), so
-// calls on different streams never overlap in the workspace; the host does not
-// block. No wait is inserted into a stream under graph capture.
-int ensure_scratch(Ctx& C, size_t bytes, hipStream_t s) {
-    if (C.scratch.cap < bytes) {
-        if (C.scratch.d) {
+// The workspace for work about to be queued on stream s: the library stream's
+// own, or the caller stream's (the first MAX_STREAM_WORKSPACES distinct streams;
+// later ones share the library stream's). It is grown to at least `bytes` (a
+// regrowth waits for the whole device). A workspace whose last use was on
+// another stream (the shared one, or a stream handle reused after its stream
+// was destroyed) is first waited for on the device, so calls never overlap in
+// one workspace, while calls on streams with workspaces of their own overlap
+// freely. The host does not block; no wait goes into a stream under graph
+// capture.
+int ensure_scratch(Ctx& C, size_t bytes, hipStream_t s, Scratch** out) {
+    Scratch* w = &C.scratch;
+    if (s != C.stream) {
+        for (auto& e : C.stream_ws)
+            if (e.first == s) w = &e.second;
+        if (w == &C.scratch && (int)C.stream_ws.size() < MAX_STREAM_WORKSPACES) {
+            C.stream_ws.reserve(MAX_STREAM_WORKSPACES);  // workspaces never move
+            C.stream_ws.emplace_back(s, Scratch{});
+            w = &C.stream_ws.back().second;
+        }
+    }
+    if (w->cap < bytes) {
+        if (w->d) {
             // work queued on any stream may still use the old buffer
             HIP_TRY(hipDeviceSynchronize());
-            HIP_TRY(hipFree(C.scratch.d));
-            C.scratch.d = nullptr;
-            C.scratch.cap = 0;
-            C.scratch_busy = false;
+            HIP_TRY(hipFree(w->d));
+            w->d = nullptr;
+            w->cap = 0;
+            w->busy = false;
         }
         size_t cap = bytes + bytes / 2 + (1 << 20);
-        HIP_TRY(hipMalloc(&C.scratch.d, cap));
-        C.scratch.cap = cap;
+        HIP_TRY(hipMalloc(&w->d, cap));
+        w->cap = cap;
     }
-    if (C.scratch_busy && C.scratch_stream != s) {
+    if (w->busy && w->last != s) {
         hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
         HIP_TRY(hipStreamIsCapturing(s, &cs));
-        if (cs == hipStreamCaptureStatusNone) HIP_TRY(hipStreamWaitEvent(s, C.scratch_ev, 0));
+        if (cs == hipStreamCaptureStatusNone) HIP_TRY(hipStreamWaitEvent(s, w->ev, 0));
     }
+    *out = w;
     return 0;
 }
 
-// Records that work queued on stream s uses the workspace (after its launches).
-int scratch_done(Ctx& C, hipStream_t s) {
-    if (!C.scratch_ev) HIP_TRY(hipEventCreateWithFlags(&C.scratch_ev, hipEventDisableTiming));
-    HIP_TRY(hipEventRecord(C.scratch_ev, s));
-    C.scratch_stream = s;
-    C.scratch_busy = true;
+// Records that work queued on stream s uses workspace w (after its launches).
+int scratch_done(Scratch* w, hipStream_t s) {
+    if (!w->ev) HIP_TRY(hipEventCreateWithFlags(&w->ev, hipEventDisableTiming));
+    HIP_TRY(hipEventRecord(w->ev, s));
+    w->last = s;
+    w->busy = true;
     return 0;
 }
 
@@ -2124,9 +2144,10 @@ int engine_gates_host(const LweBootstrappingKeyFFT* bf, int32_t count, const int
     // device layout: [idx][pool][ext out] then the engine's own scratch
     const size_t dev_bytes = idx_bytes + pool_bytes + extout_bytes;
     const size_t work = ext_bytes(P, count) + ks_scratch_bytes(P, count);
-    if (int rc = ensure_scratch(C, dev_bytes + work, C.stream)) return rc;
+    Scratch* ws = nullptr;
+    if (int rc = ensure_scratch(C, dev_bytes + work, C.stream, &ws)) return rc;
     // the engine's working buffers follow the staged region
-    char* d = (char*)C.scratch.d;
+    char* d = (char*)ws->d;
     HIP_TRY(hipMemcpyAsync(d, h, idx_bytes + pool_in_bytes, hipMemcpyHostToDevice, C.stream));
     const int32_t* d_idx = (const int32_t*)d;
     int32_t* d_pool = (int32_t*)(d + idx_bytes);
@@ -2137,7 +2158,7 @@ int engine_gates_host(const LweBootstrappingKeyFFT* bf, int32_t count, const int
     const void* src = keyswitch ? (const void*)(d_pool + (size_t)nin_slots * w) : (const void*)d_extout;
     HIP_TRY(hipMemcpyAsync(h + idx_bytes, src, out_bytes, hipMemcpyDeviceToHost, C.stream));
     HIP_TRY(hipStreamSynchronize(C.stream));
-    C.scratch_busy = false;  // the workspace is idle again
+    ws->busy = false;  // the workspace is idle again
     const int32_t* ho = (const int32_t*)(h + idx_bytes);
     auto scatter = [&](int b, int e) {
         for (int i = b; i < e; i++) {
@@ -2171,12 +2192,13 @@ int engine_blind_rotate_host(tfhe_amd_device_key* dk, int32_t row0, int32_t n, c
     const size_t bara_bytes = align256(sizeof(int32_t) * 1024);
     const size_t ext_b = align256(sizeof(int32_t) * nx);
     const size_t total = 2 * acc_bytes + bara_bytes + ext_b;
-    if (int rc = ensure_scratch(C, total, C.stream)) return rc;
+    Scratch* ws = nullptr;
+    if (int rc = ensure_scratch(C, total, C.stream, &ws)) return rc;
     if (int rc = ensure_pinned(C, total)) return rc;
     char* h = (char*)C.h_pinned;
     memcpy(h, acc, sizeof(int32_t) * 2 * N);
     if (n) memcpy(h + acc_bytes, bara, sizeof(int32_t) * n);
-    char* d = (char*)C.scratch.d;
+    char* d = (char*)ws->d;
     HIP_TRY(hipMemcpyAsync(d, h, acc_bytes + bara_bytes, hipMemcpyHostToDevice, C.stream));
     BrArgs A = base_args(dk);
     A.n = n;
@@ -2189,7 +2211,7 @@ int engine_blind_rotate_host(tfhe_amd_device_key* dk, int32_t row0, int32_t n, c
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(h, d + acc_bytes + bara_bytes, acc_bytes + ext_b, hipMemcpyDeviceToHost, C.stream));
     HIP_TRY(hipStreamSynchronize(C.stream));
-    C.scratch_busy = false;
+    ws->busy = false;
     memcpy(acc, h, sizeof(int32_t) * 2 * N);
     if (ext_out) memcpy(ext_out, h + acc_bytes, sizeof(int32_t) * nx);
     return 0;
@@ -2202,17 +2224,18 @@ int engine_keyswitch_host(tfhe_amd_device_key* dk, const int32_t* in, int32_t* o
     const int nx = P.k * P.N + 1, w = P.n + 1;
     const size_t in_b = align256(sizeof(int32_t) * 2 * nx), out_b = align256(sizeof(int32_t) * w);
     const size_t work = ks_scratch_bytes(P, 1);
-    if (int rc = ensure_scratch(C, in_b + out_b + work, C.stream)) return rc;
+    Scratch* ws = nullptr;
+    if (int rc = ensure_scratch(C, in_b + out_b + work, C.stream, &ws)) return rc;
     if (int rc = ensure_pinned(C, in_b + out_b)) return rc;
     char* h = (char*)C.h_pinned;
     memcpy(h, in, sizeof(int32_t) * nx);
-    char* d = (char*)C.scratch.d;
+    char* d = (char*)ws->d;
     HIP_TRY(hipMemcpyAsync(d, h, sizeof(int32_t) * nx, hipMemcpyHostToDevice, C.stream));
     if (int rc = run_keyswitch(dk, (const int32_t*)d, nullptr, 1, (int32_t*)(d + in_b), nullptr, d + in_b + out_b, C.stream))
         return rc;
     HIP_TRY(hipMemcpyAsync(h, d + in_b, sizeof(int32_t) * w, hipMemcpyDeviceToHost, C.stream));
     HIP_TRY(hipStreamSynchronize(C.stream));
-    C.scratch_busy = false;
+    ws->busy = false;
     memcpy(out, h, sizeof(int32_t) * w);
     return 0;
 }
@@ -2246,11 +2269,12 @@ TFHE_AMD_EXPORT int32_t tfhe_amd_gates_device(const TFheGateBootstrappingCloudKe
     std::lock_guard<std::mutex> g(C.mu);
     if (count < 0) return set_error("negative gate count");
     const hipStream_t s = stream ? (hipStream_t)stream : C.stream;
-    if (int rc = ensure_scratch(C, ext_bytes(dk->P, count) + ks_scratch_bytes(dk->P, count), s)) return rc;
+    Scratch* ws = nullptr;
+    if (int rc = ensure_scratch(C, ext_bytes(dk->P, count) + ks_scratch_bytes(dk->P, count), s, &ws)) return rc;
     if (int rc = gates_device_locked(C, dk, count, d_op, d_in0, d_in1, d_in2, d_out, d_pool, nullptr, 1 << 29,
-                                     (char*)C.scratch.d, s))
+                                     (char*)ws->d, s))
         return rc;
-    return scratch_done(C, s);
+    return scratch_done(ws, s);
 }
 
 TFHE_AMD_EXPORT int32_t tfhe_amd_gates_device_woks(const TFheGateBootstrappingCloudKeySet* ck, int32_t count,
@@ -2264,11 +2288,12 @@ TFHE_AMD_EXPORT int32_t tfhe_amd_gates_device_woks(const TFheGateBootstrappingCl
     std::lock_guard<std::mutex> g(C.mu);
     if (count < 0) return set_error("negative gate count");
     const hipStream_t s = stream ? (hipStream_t)stream : C.stream;
-    if (int rc = ensure_scratch(C, ext_bytes(dk->P, count) + ks_scratch_bytes(dk->P, count), s)) return rc;
+    Scratch* ws = nullptr;
+    if (int rc = ensure_scratch(C, ext_bytes(dk->P, count) + ks_scratch_bytes(dk->P, count), s, &ws)) return rc;
     if (int rc = gates_device_locked(C, dk, count, d_op, d_in0, d_in1, d_in2, nullptr, const_cast<int32_t*>(d_pool),
-                                     d_ext, 1 << 29, (char*)C.scratch.d, s))
+                                     d_ext, 1 << 29, (char*)ws->d, s))
         return rc;
-    return scratch_done(C, s);
+    return scratch_done(ws, s);
 }
 
 TFHE_AMD_EXPORT int32_t tfhe_amd_keyswitch_device(const TFheGateBootstrappingCloudKeySet* ck, int32_t count,
@@ -2281,9 +2306,10 @@ TFHE_AMD_EXPORT int32_t tfhe_amd_keyswitch_device(const TFheGateBootstrappingClo
     std::lock_guard<std::mutex> g(C.mu);
     // run_keyswitch reads ext rows [2*count][..]; with op == null only the first count are used
     const hipStream_t s = stream ? (hipStream_t)stream : C.stream;
-    if (int rc = ensure_scratch(C, ks_scratch_bytes(dk->P, count), s)) return rc;
-    if (int rc = run_keyswitch(dk, d_in, nullptr, count, d_out, nullptr, (char*)C.scratch.d, s)) return rc;
-    return scratch_done(C, s);
+    Scratch* ws = nullptr;
+    if (int rc = ensure_scratch(C, ks_scratch_bytes(dk->P, count), s, &ws)) return rc;
+    if (int rc = run_keyswitch(dk, d_in, nullptr, count, d_out, nullptr, (char*)ws->d, s)) return rc;
+    return scratch_done(ws, s);
 }
 
 TFHE_AMD_EXPORT void tfhe_amd_set_launch_config(int32_t br_mode, int32_t latency_max, int32_t small_batch_max) {
@@ -2359,7 +2385,9 @@ TFHE_AMD_EXPORT int32_t tfhe_amd_autotune(const TFheGateBootstrappingCloudKeySet
         hipMemcpyAsync(d_idx, idx.data(), sizeof(int32_t) * idx.size(), hipMemcpyHostToDevice, C.stream) != hipSuccess ||
         hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
         return cleanup(set_error("autotune: set-up failed"));
-    if (int rc = ensure_scratch(C, ext_bytes(dk->P, maxB) + ks_scratch_bytes(dk->P, maxB), C.stream)) return cleanup(rc);
+    Scratch* ws = nullptr;
+    if (int rc = ensure_scratch(C, ext_bytes(dk->P, maxB) + ks_scratch_bytes(dk->P, maxB), C.stream, &ws))
+        return cleanup(rc);
     const int saved = C.br_mode;
     float best[3][NS];
     int rc = 0;
@@ -2372,7 +2400,7 @@ TFHE_AMD_EXPORT int32_t tfhe_amd_autotune(const TFheGateBootstrappingCloudKeySet
                 rc = (int)(hipEventRecord(e0, C.stream) != hipSuccess);
                 if (!rc)
                     rc = gates_device_locked(C, dk, B, d_idx, d_idx + maxB, d_idx + 2 * maxB, d_idx + maxB,
-                                             d_idx + 3 * maxB, d_pool, nullptr, 1 << 29, (char*)C.scratch.d,
+                                             d_idx + 3 * maxB, d_pool, nullptr, 1 << 29, (char*)ws->d,
                                              C.stream, false);
                 float t = 0;
                 if (!rc && (hipEventRecord(e1, C.stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
