@@ -233,7 +233,9 @@ TFHE_AMD_EXPORT size_t tfhe_amd_cloud_key_device_bytes(const TFheGateBootstrappi
  * overlap on the device; further streams share the library stream's
  * workspace, and a call whose workspace was last used from another stream
  * waits for that use on the device (never on the host). A workspace is kept
- * until the process ends. */
+ * until the process ends; the first call on a stream (or one with a larger
+ * count than before) allocates it, so make that call before capturing the
+ * stream into a graph. */
 TFHE_AMD_EXPORT int32_t tfhe_amd_gates_device(const TFheGateBootstrappingCloudKeySet* ck, int32_t count,
                                               const int32_t* d_op, const int32_t* d_in0, const int32_t* d_in1,
                                               const int32_t* d_in2, const int32_t* d_out, int32_t* d_pool,
