@@ -391,15 +391,31 @@ struct FftPair {
 // in the lane-by-lane CPU emulation of the tests.
 struct FftHalf {
     static constexpr int R = 4;
-    static constexpr int LDS_C64 = 272;  // 256 elements, one pad every 16
-    static constexpr double SQ = kH;     // e^{i pi/4} = (SQ, SQ)
-    // element e at ix(e) = e + (e >> 4); the per-lane bases below are ix() of
-    // each step's first element, the steps adding compile-time offsets
-    TA_HD static int ix(int e) { return e + (e >> 4); }
-    TA_HD static int base1(int b) { return b + (b >> 4); }                                    // + 68 k1
-    TA_HD static int base2(int lane) { return 68 * (lane >> 4) + (lane & 15); }               // + 17 tau / mu
-    TA_HD static int base3(int lane) { return 68 * (lane >> 4) + 17 * ((lane >> 2) & 3) + (lane & 3); }  // + 4 g1 / nu
-    TA_HD static int base4(int lane) { return 68 * (lane >> 4) + 17 * ((lane >> 2) & 3) + 4 * (lane & 3); }  // + g0
+    static constexpr int LDS_C64 = 256;
+    static constexpr double SQ = kH;  // e^{i pi/4} = (SQ, SQ)
+    // Element e = 64 d3 + 16 d2 + 4 d1 + d0 (base-4 digits) sits at
+    //   64 d3 + 16 d2 + 4 (d1 ^ d2) + (d0 ^ d1).
+    // Every step below holds three of the digits in the lane and one (its slot)
+    // in a register; with this placement any 8 or 16 consecutive lanes of a
+    // step touch distinct 16-byte bank groups, so no LDS access conflicts. The
+    // bases are the slot-0 addresses; a slot adds 64 k1 (step 1) or is XORed in.
+    TA_HD static int base1(int b) {  // lane (d0, d1, d2); slot d3: + 64 k1
+        const int d0 = b & 3, d1 = (b >> 2) & 3, d2 = (b >> 4) & 3;
+        return 16 * d2 + 4 * (d1 ^ d2) + (d0 ^ d1);
+    }
+    TA_HD static int base2(int lane) {  // lane (d0, d1, d3); slot t = d2: (base ^ 4 t) + 16 t
+        const int d0 = lane & 3, d1 = (lane >> 2) & 3, d3 = lane >> 4;
+        return 64 * d3 + 4 * d1 + (d0 ^ d1);
+    }
+    TA_HD static int base3(int lane) {  // lane (d0, d2, d3); slot g1 = d1: base ^ 5 g1
+        const int d0 = lane & 3, d2 = (lane >> 2) & 3, d3 = lane >> 4;
+        return 64 * d3 + 20 * d2 + d0;
+    }
+    TA_HD static int base4(int lane) {  // lane (d1, d2, d3); slot g0 = d0: base ^ g0
+        const int d1 = lane & 3, d2 = (lane >> 2) & 3, d3 = lane >> 4;
+        return 64 * d3 + 16 * d2 + 4 * (d1 ^ d2) + d1;
+    }
+    TA_HD static int at2(int b2, int t) { return (b2 ^ (4 * t)) + 16 * t; }
 
     // natural-order 4-point DFT in place, exponent sign S
     template <int S>
@@ -427,77 +443,77 @@ struct FftHalf {
         dft4<1>(x);
         c64* d = lds + base1(b);
 #pragma unroll
-        for (int k1 = 0; k1 < R; k1++) d[68 * k1] = cmul(x[k1], tw[k1]);
+        for (int k1 = 0; k1 < R; k1++) d[64 * k1] = cmul(x[k1], tw[k1]);
     }
     TA_HD static void fwd2(c64* x, int lane, const c64* lds, const c64* tw) {
-        const c64* q = lds + base2(lane);
+        const int b2 = base2(lane);
 #pragma unroll
-        for (int t = 0; t < R; t++) x[t] = q[17 * t];
+        for (int t = 0; t < R; t++) x[t] = lds[at2(b2, t)];
         dft4<1>(x);
 #pragma unroll
         for (int mu = 1; mu < R; mu++) x[mu] = cmul(x[mu], tw[3 + mu]);
     }
     TA_HD static void fwd2w(const c64* x, int lane, c64* lds) {
-        c64* d = lds + base2(lane);
+        const int b2 = base2(lane);
 #pragma unroll
-        for (int mu = 0; mu < R; mu++) d[17 * mu] = x[mu];
+        for (int mu = 0; mu < R; mu++) lds[at2(b2, mu)] = x[mu];
     }
     TA_HD static void fwd3(c64* x, int lane, const c64* lds, const c64* tw) {
-        const c64* q = lds + base3(lane);
+        const int b3 = base3(lane);
 #pragma unroll
-        for (int g1 = 0; g1 < R; g1++) x[g1] = q[4 * g1];
+        for (int g1 = 0; g1 < R; g1++) x[g1] = lds[b3 ^ (5 * g1)];
         dft4<1>(x);
 #pragma unroll
         for (int nu = 1; nu < R; nu++) x[nu] = cmul(x[nu], tw[6 + nu]);
     }
     TA_HD static void fwd3w(const c64* x, int lane, c64* lds) {
-        c64* d = lds + base3(lane);
+        const int b3 = base3(lane);
 #pragma unroll
-        for (int nu = 0; nu < R; nu++) d[4 * nu] = x[nu];
+        for (int nu = 0; nu < R; nu++) lds[b3 ^ (5 * nu)] = x[nu];
     }
     TA_HD static void fwd4(c64* x, int lane, const c64* lds) {
-        const c64* q = lds + base4(lane);
+        const int b4 = base4(lane);
 #pragma unroll
-        for (int g0 = 0; g0 < R; g0++) x[g0] = q[g0];
+        for (int g0 = 0; g0 < R; g0++) x[g0] = lds[b4 ^ g0];
         dft4<1>(x);
     }
     // ---- inverse (exact inverse of the forward steps, times 256) ----
     TA_HD static void inv1(c64* x, int lane, c64* lds) {
         dft4<-1>(x);
-        c64* d = lds + base4(lane);
+        const int b4 = base4(lane);
 #pragma unroll
-        for (int g0 = 0; g0 < R; g0++) d[g0] = x[g0];
+        for (int g0 = 0; g0 < R; g0++) lds[b4 ^ g0] = x[g0];
     }
     TA_HD static void inv2(c64* x, int lane, const c64* lds, const c64* tw) {
-        const c64* q = lds + base3(lane);
-        x[0] = q[0];
+        const int b3 = base3(lane);
+        x[0] = lds[b3];
 #pragma unroll
-        for (int nu = 1; nu < R; nu++) x[nu] = cmulc(q[4 * nu], tw[6 + nu]);
+        for (int nu = 1; nu < R; nu++) x[nu] = cmulc(lds[b3 ^ (5 * nu)], tw[6 + nu]);
         dft4<-1>(x);
     }
     TA_HD static void inv2w(const c64* x, int lane, c64* lds) {
-        c64* d = lds + base3(lane);
+        const int b3 = base3(lane);
 #pragma unroll
-        for (int g1 = 0; g1 < R; g1++) d[4 * g1] = x[g1];
+        for (int g1 = 0; g1 < R; g1++) lds[b3 ^ (5 * g1)] = x[g1];
     }
     TA_HD static void inv3(c64* x, int lane, const c64* lds, const c64* tw) {
-        const c64* q = lds + base2(lane);
-        x[0] = q[0];
+        const int b2 = base2(lane);
+        x[0] = lds[b2];
 #pragma unroll
-        for (int mu = 1; mu < R; mu++) x[mu] = cmulc(q[17 * mu], tw[3 + mu]);
+        for (int mu = 1; mu < R; mu++) x[mu] = cmulc(lds[at2(b2, mu)], tw[3 + mu]);
         dft4<-1>(x);
     }
     TA_HD static void inv3w(const c64* x, int lane, c64* lds) {
-        c64* d = lds + base2(lane);
+        const int b2 = base2(lane);
 #pragma unroll
-        for (int t = 0; t < R; t++) d[17 * t] = x[t];
+        for (int t = 0; t < R; t++) lds[at2(b2, t)] = x[t];
     }
     // result: y_h[b + 64a] / 2 (the pair's 1/2 and the 1/256 folded in)
     template <int H>
     TA_HD static void inv4(c64* x, int b, const c64* lds, const c64* tw) {
         const c64* q = lds + base1(b);
 #pragma unroll
-        for (int k1 = 0; k1 < R; k1++) x[k1] = cmulc(q[68 * k1], tw[k1]);
+        for (int k1 = 0; k1 < R; k1++) x[k1] = cmulc(q[64 * k1], tw[k1]);
         dft4<-1>(x);
 #pragma unroll
         for (int a = 0; a < R; a++) {
