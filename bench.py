@@ -84,7 +84,7 @@ def profiled_traffic():
 
 def bench_scaling(T, sk, P, sizes, seed):
     """gates/s of each blind-rotation kernel across batch sizes (input for the
-    automatic kernel choice): 2 = four bootstraps per workgroup, 3 = two
+    automatic kernel choice): 2 = one bootstrap per wavefront, 3 = two
     wavefronts per bootstrap, 0 = automatic."""
     rng = np.random.default_rng(seed + 3)
     maxb = max(sizes)

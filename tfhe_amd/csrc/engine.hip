@@ -7,7 +7,7 @@
 //   br_kernel          blind rotation, one bootstrap per 64-lane wavefront, the
 //                      accumulator in LDS (the explicit-accumulator entry points)
 //   br4_kernel         blind rotation for large batches: one bootstrap per
-//                      wavefront, four per workgroup, key read from global memory
+//                      one-wavefront workgroup, key read from global memory
 //   br2_kernel         blind rotation, two wavefronts per bootstrap (one per TLWE
 //                      component), key blocks shared through LDS
 //   brl_kernel         blind rotation for one gate: 4l wavefronts per bootstrap on
@@ -311,10 +311,11 @@ __global__ __launch_bounds__(1024) void mux_scan_kernel(const int32_t* __restric
     if (t == 1023) *n_jobs = count + s_cnt[1023];
 }
 
-// Blind rotation with one bootstrap per wavefront, four wavefronts per
-// workgroup (M = 512, i.e. N = 1024). The wavefronts share nothing and never
-// wait for each other: the transposes of the transform go through the
-// wavefront's own LDS buffer, and the transformed key and the stage-1
+// Blind rotation with one bootstrap per one-wavefront workgroup (M = 512, i.e.
+// N = 1024). Bootstraps share nothing, so each is its own workgroup: the
+// dispatcher refills a CU as soon as any one of them ends, and a batch of 512
+// gates still spreads over all 256 CUs. The transposes of the transform go
+// through the workgroup's 14 KB of LDS, and the transformed key and the stage-1
 // twiddles come straight from global memory (cached; the 4096 wavefronts of a
 // batch walk the key rows nearly together), so the LDS carries only the
 // transposes. The accumulator lives in registers in the transform's
@@ -322,24 +323,21 @@ __global__ __launch_bounds__(1024) void mux_scan_kernel(const int32_t* __restric
 // wavefront's transform buffer as (ACC_c, -ACC_c, ACC_c), so every rotated
 // read is a plain load.
 template <int L>
-__global__ __launch_bounds__(256, 2) void br4_kernel(BrArgs A, const int32_t* __restrict__ mux_gate,
-                                                    const int32_t* __restrict__ n_jobs) {
+__global__ __launch_bounds__(64, 8) void br4_kernel(BrArgs A, const int32_t* __restrict__ mux_gate,
+                                                   const int32_t* __restrict__ n_jobs) {
     constexpr int M = 512;
     using F = Fft<M>;
     constexpr int R = F::R, N = 2 * M, KPL = 2 * L;
     constexpr int LOG2N2 = 11;
     // per wavefront: the transform buffer, also holding (v, -v, v) for the rotation (3N int32)
     constexpr int BUF_C64 = (F::LDS_C64 > 3 * N / 4) ? F::LDS_C64 : 3 * N / 4;
-    __shared__ c64 s_fft[4][BUF_C64];
-    __shared__ int16_t s_bara[4][1024];
+    __shared__ c64 fb[BUF_C64];
+    __shared__ int16_t bara[1024];
 
-    const int lane = threadIdx.x & 63;
-    const int w = threadIdx.x >> 6;
-    const int job = blockIdx.x * 4 + w;
-    if (job >= *n_jobs) return;  // no barriers below, so idle wavefronts leave at once
+    const int lane = threadIdx.x;
+    const int job = blockIdx.x;
+    if (job >= *n_jobs) return;
     const int n = A.n;
-    int16_t* bara = s_bara[w];
-    c64* fb = s_fft[w];
     int32_t* fbi = (int32_t*)fb;
 
     // accumulator: acc[c][h*R + s] = ACC_c[64 s + lane + h M]
@@ -1826,7 +1824,7 @@ int gates_device_locked(Ctx& C, tfhe_amd_device_key* dk, int count, const int32_
         for (auto& e : ev.e) HIP_TRY(hipEventCreate(&e));
         HIP_TRY(hipEventRecord(ev.e[0], s));
     }
-    // kernel choice: 1 = one wavefront per bootstrap, 2 = four bootstraps per workgroup,
+    // kernel choice: 1 = one wavefront per bootstrap (accumulator in LDS), 2 = one per workgroup of one wavefront,
     // 3 = two wavefronts per bootstrap, 5 = 4l wavefronts per bootstrap (half transforms); 0 = automatic
     // (5 for the smallest batches, 3 while the batch leaves CUs idle, 2 for large
     // batches). N = 2048 uses the pair-transform kernel (4) unless mode 1 is forced.
@@ -1849,9 +1847,9 @@ int gates_device_locked(Ctx& C, tfhe_amd_device_key* dk, int count, const int32_
         if (dk->P.l == 2) hipLaunchKernelGGL(brp_kernel<2>, dim3(blocks), dim3(256), 0, s, A, dk->bkp, dk->twp, mux_gate, n_jobs);
         else hipLaunchKernelGGL(brp_kernel<3>, dim3(blocks), dim3(256), 0, s, A, dk->bkp, dk->twp, mux_gate, n_jobs);
     } else if (mode == 2) {
-        const int blocks = (2 * count + 3) / 4;
-        if (dk->P.l == 3) hipLaunchKernelGGL(br4_kernel<3>, dim3(blocks), dim3(256), 0, s, A, mux_gate, n_jobs);
-        else hipLaunchKernelGGL(br4_kernel<2>, dim3(blocks), dim3(256), 0, s, A, mux_gate, n_jobs);
+        const int blocks = 2 * count;  // one bootstrap per workgroup, at most 2*count jobs
+        if (dk->P.l == 3) hipLaunchKernelGGL(br4_kernel<3>, dim3(blocks), dim3(64), 0, s, A, mux_gate, n_jobs);
+        else hipLaunchKernelGGL(br4_kernel<2>, dim3(blocks), dim3(64), 0, s, A, mux_gate, n_jobs);
     } else if (mode == 3) {
         const int blocks = count;  // two bootstraps per workgroup, at most 2*count jobs
         if (dk->P.l == 3) hipLaunchKernelGGL(br2_kernel<3>, dim3(blocks), dim3(256), 0, s, A, mux_gate, n_jobs);
