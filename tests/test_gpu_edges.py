@@ -66,8 +66,9 @@ def test_mux_only_batch_every_kernel(key, mode):
 
 
 def test_ragged_batch_in_the_batched_kernel(key):
-    """4097 gates in the four-gates-per-workgroup kernel: the last workgroup has
-    one active wavefront (with MUX gates appended as extra jobs)."""
+    """4097 gates in the batched one-bootstrap-per-wavefront kernel: one gate past
+    a full 4096-gate launch, with MUX gates appended as extra jobs and the idle
+    tail of the grid leaving at once."""
     T, P, sk, keys = key
     rng = np.random.default_rng(80)
     count = 4097

@@ -291,8 +291,8 @@ def test_c_program_gpu(tmp_path):
 
 @pytest.mark.parametrize("mode", [1, 2, 3, 5])
 def test_all_blind_rotation_kernels_bitexact(p128, mode):
-    """The blind-rotation kernels (1: one wavefront per bootstrap, 2: four
-    bootstraps per workgroup, 3: two wavefronts per bootstrap, 5: 4l
+    """The blind-rotation kernels (1: one wavefront per bootstrap, 2: one
+    bootstrap per one-wavefront workgroup, 3: two wavefronts per bootstrap, 5: 4l
     wavefronts per bootstrap on half transforms) give the oracle's ciphertexts
     bit-exactly, MUX included, for odd batch sizes."""
     T, P, sk, keys = p128
