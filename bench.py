@@ -62,6 +62,9 @@ def parse():
     ap.add_argument("--host-api", type=int, default=4096,
                     help="host gate API section: gates per bootsGate_batch call on host LweSample arrays (0 = skip)")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default small-batch crossover")
+    ap.add_argument("--crossovers", default="",
+                    help="LAT,SMALL: use the latency kernel up to LAT gates and the two-wavefront kernel up to "
+                         "SMALL instead of autotuning")
     ap.add_argument("--compare-modes", action="store_true",
                     help="also time the one-wavefront-per-bootstrap kernel on the same batch")
     ap.add_argument("--copy-calib-mb", type=int, default=0,
@@ -583,7 +586,13 @@ def main():
           flush=True)
 
     autotune = None
-    if not args.no_autotune:
+    if args.crossovers:
+        lat, small = (int(x) for x in args.crossovers.split(","))
+        T.set_launch_config(0, lat, small)
+        autotune = {"latency_kernel_up_to": lat, "two_wave_kernel_up_to": small, "forced": True}
+        print(f"[bench] crossovers forced: latency kernel up to {lat} gates, two-wavefront up to {small}",
+              file=sys.stderr, flush=True)
+    elif not args.no_autotune:
         lat, small, tune = T.autotune(sk.cloud)
         autotune = {"latency_kernel_up_to": lat, "two_wave_kernel_up_to": small,
                     "ms": {name: {str(k): round(v, 3) for k, v in t.items()} for name, t in tune.items()}}
