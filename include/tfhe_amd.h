@@ -368,6 +368,16 @@ TFHE_AMD_EXPORT int64_t tfhe_amd_server_submit(tfhe_amd_server* srv, int32_t cli
 TFHE_AMD_EXPORT int32_t tfhe_amd_server_wait(tfhe_amd_server* srv, int64_t request, int32_t timeout_ms,
                                              double* latency_ms);
 TFHE_AMD_EXPORT int32_t tfhe_amd_server_get_stats(tfhe_amd_server* srv, tfhe_amd_server_stats* stats);
+/* Packing (no reference counterpart; for loaded servers): when at least two
+ * clients have gates ready and together at least min_gates, one launch carries
+ * the next level of each of them (up to 32 clients and max_batch gates), every
+ * client's gates with its own key in the batched blind-rotation kernel, on a
+ * stream of its own; while packing is on, a client whose newest launch was
+ * packed waits for the next packed launch. 0 (the default) turns it off.
+ * Outputs are bit-identical either way. N = 1024 keys only. */
+TFHE_AMD_EXPORT int32_t tfhe_amd_server_set_packing(tfhe_amd_server* srv, int32_t min_gates);
+/* Packed launches made so far. */
+TFHE_AMD_EXPORT int64_t tfhe_amd_server_packed_launches(tfhe_amd_server* srv);
 /* Finishes every submitted request, then frees the server. */
 TFHE_AMD_EXPORT void tfhe_amd_server_destroy(tfhe_amd_server* srv);
 

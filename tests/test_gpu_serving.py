@@ -79,6 +79,34 @@ def test_same_alone_and_in_the_mix(keys):
     assert np.array_equal(out[last], alone)
 
 
+def test_packed_launches_bit_identical(keys):
+    """With packing on, several clients' levels share launches (each client's
+    gates with its own key as a segment of the batched blind-rotation kernel,
+    each segment's keyswitch with its own key), and every output is bit-identical
+    to the unpacked server's and decrypts to the plaintext evaluation."""
+    T, P, sks = keys
+    rng = np.random.default_rng(11)
+    reqs = []
+    for i in range(18):
+        c = i % 3
+        reqs.append((c,) + _request(sks[c], rng, int(rng.integers(1, 10)), int(rng.integers(1, 300))))
+    outs = {}
+    for pack in (0, 1):
+        S, cids = _server(T, P, sks, n_streams=4, depth=2, max_batch=4096, wire_budget_bytes=256 << 20,
+                          pack_min_gates=pack)
+        try:
+            ids = [S.submit(cids[c], C, np.arange(len(b)), ct, lw) for c, C, b, ct, lw in reqs]
+            outs[pack] = [S.wait(r, 120000)[0] for r in ids]
+            st = S.stats()
+        finally:
+            S.close()
+        assert st["failed"] == 0 and st["completed"] == len(reqs)
+        assert (st["packed_launches"] > 0) if pack else (st["packed_launches"] == 0)
+    for (c, C, b, ct, lw), o0, o1 in zip(reqs, outs[0], outs[1]):
+        assert np.array_equal(o0, o1)
+        assert np.array_equal(sks[c].decrypt(o1), C.evaluate(b)[lw])
+
+
 def test_merging_and_overlap(keys):
     """Many small requests of one client submitted at once run in fewer launches
     than levels submitted, and the largest launch holds several requests."""

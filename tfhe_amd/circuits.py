@@ -165,6 +165,8 @@ _SERVER_SIGS = {
     "tfhe_amd_server_wait": (i32, [vp, ctypes.c_int64, i32, ctypes.POINTER(ctypes.c_double)]),
     "tfhe_amd_server_get_stats": (i32, [vp, ctypes.POINTER(ServerStats)]),
     "tfhe_amd_server_destroy": (None, [vp]),
+    "tfhe_amd_server_set_packing": (i32, [vp, i32]),
+    "tfhe_amd_server_packed_launches": (ctypes.c_int64, [vp]),
 }
 
 
@@ -172,7 +174,7 @@ class Server:
     """Continuous-batching request loop (tfhe_amd_server_*). submit() returns a
     request id; wait() returns (output ciphertexts [n_out][n+1], latency ms)."""
 
-    def __init__(self, n_streams=8, depth=2, max_batch=4096, wire_budget_bytes=1 << 30):
+    def __init__(self, n_streams=8, depth=2, max_batch=4096, wire_budget_bytes=1 << 30, pack_min_gates=0):
         L = lib()
         for name, (res, args) in _SERVER_SIGS.items():
             f = getattr(L, name)
@@ -180,6 +182,8 @@ class Server:
         self.L = L
         self.ptr = L.tfhe_amd_server_create(n_streams, depth, max_batch, wire_budget_bytes)
         if not self.ptr:
+            raise RuntimeError(L.tfhe_amd_last_error().decode())
+        if pack_min_gates and L.tfhe_amd_server_set_packing(self.ptr, pack_min_gates) != 0:
             raise RuntimeError(L.tfhe_amd_last_error().decode())
         self._live = {}
         self.width = None
@@ -218,7 +222,9 @@ class Server:
     def stats(self):
         st = ServerStats()
         check(self.L.tfhe_amd_server_get_stats(self.ptr, ctypes.byref(st)), "server_get_stats")
-        return {name: getattr(st, name) for name, _ in ServerStats._fields_}
+        out = {name: getattr(st, name) for name, _ in ServerStats._fields_}
+        out["packed_launches"] = int(self.L.tfhe_amd_server_packed_launches(self.ptr))
+        return out
 
     def close(self):
         if self.ptr:

@@ -53,6 +53,15 @@ void* engine_stream();
 int engine_launch_batch_locked(tfhe_amd_device_key* dk, int count, const int32_t* d_op, const int32_t* d_in0,
                                const int32_t* d_in1, const int32_t* d_in2, const int32_t* d_out, int32_t* d_pool,
                                char* scr, void* stream, bool capturing);
+// Several clients' batches (segments) in one blind-rotation launch, each segment
+// with its own key (same parameters, N = 1024), then each segment's keyswitch.
+// d_idx holds [5][total] (op, in0, in1, in2, out) with segment g at
+// [sum of earlier counts, + counts[g]). At most 32 segments; call with
+// engine_mutex() held.
+bool engine_can_segment(const tfhe_amd_device_key* dk);
+size_t engine_segments_scratch_bytes(tfhe_amd_device_key* const* dks, const int32_t* counts, int nseg);
+int engine_launch_segments_locked(tfhe_amd_device_key* const* dks, const int32_t* counts, int nseg,
+                                  const int32_t* d_idx, int32_t* d_pool, char* scr, void* stream);
 int engine_param_n(const tfhe_amd_device_key* dk);
 // True when two device keys have the same parameters (ciphertext width, scratch sizes).
 bool engine_same_params(const tfhe_amd_device_key* a, const tfhe_amd_device_key* b);

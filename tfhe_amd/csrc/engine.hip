@@ -322,17 +322,19 @@ __global__ __launch_bounds__(1024) void mux_scan_kernel(const int32_t* __restric
 // coefficient layout; the negacyclic rotation is staged through the
 // wavefront's transform buffer as (ACC_c, -ACC_c, ACC_c), so every rotated
 // read is a plain load.
+struct Br4Lds {
+    // the transform buffer, also holding (v, -v, v) for the rotation (3N int32)
+    static constexpr int BUF_C64 = (Fft<512>::LDS_C64 > 3 * 1024 / 4) ? Fft<512>::LDS_C64 : 3 * 1024 / 4;
+};
+
 template <int L>
-__global__ __launch_bounds__(64, 8) void br4_kernel(BrArgs A, const int32_t* __restrict__ mux_gate,
-                                                   const int32_t* __restrict__ n_jobs) {
+__device__ __forceinline__ void br4_body(BrArgs A, const int32_t* __restrict__ mux_gate,
+                                         const int32_t* __restrict__ n_jobs, c64* __restrict__ fb,
+                                         int16_t* __restrict__ bara) {
     constexpr int M = 512;
     using F = Fft<M>;
     constexpr int R = F::R, N = 2 * M, KPL = 2 * L;
     constexpr int LOG2N2 = 11;
-    // per wavefront: the transform buffer, also holding (v, -v, v) for the rotation (3N int32)
-    constexpr int BUF_C64 = (F::LDS_C64 > 3 * N / 4) ? F::LDS_C64 : 3 * N / 4;
-    __shared__ c64 fb[BUF_C64];
-    __shared__ int16_t bara[1024];
 
     const int lane = threadIdx.x;
     const int job = blockIdx.x;
@@ -477,6 +479,47 @@ __global__ __launch_bounds__(64, 8) void br4_kernel(BrArgs A, const int32_t* __r
             e[j == 0 ? 0 : N - j] = (int32_t)(j == 0 ? v : 0u - v);
         }
     if (lane == 0) e[N] = (int32_t)acc[1][0];
+}
+
+template <int L>
+__global__ __launch_bounds__(64, 8) void br4_kernel(BrArgs A, const int32_t* __restrict__ mux_gate,
+                                                   const int32_t* __restrict__ n_jobs) {
+    __shared__ c64 fb[Br4Lds::BUF_C64];
+    __shared__ int16_t bara[1024];
+    br4_body<L>(A, mux_gate, n_jobs, fb, bara);
+}
+
+// One launch over several clients' batches (segments), each with its own
+// bootstrapping key and job list: workgroup (x, y) runs job x of segment y.
+// The segments share the parameters, the twiddles and the ciphertext pool;
+// segment y's gates are entries [off, off + count) of the launch's index arrays.
+constexpr int SEG_MAX = 32;
+struct BrSeg {
+    const c64* bkf;
+    int32_t* ext;
+    const int32_t* mux_gate;
+    const int32_t* n_jobs;
+    int32_t off, count;
+};
+struct BrSegTable {
+    BrArgs base;
+    BrSeg seg[SEG_MAX];
+};
+
+template <int L>
+__global__ __launch_bounds__(64, 8) void br4_seg_kernel(BrSegTable T) {
+    const BrSeg& g = T.seg[blockIdx.y];
+    BrArgs A = T.base;
+    A.bkf = g.bkf;
+    A.ext = g.ext;
+    A.count = g.count;
+    A.op += g.off;
+    A.in0 += g.off;
+    A.in1 += g.off;
+    A.in2 += g.off;
+    __shared__ c64 fb[Br4Lds::BUF_C64];
+    __shared__ int16_t bara[1024];
+    br4_body<L>(A, g.mux_gate, g.n_jobs, fb, bara);
 }
 
 // largest LWE dimension the latency kernel's LDS table holds
@@ -1893,6 +1936,68 @@ int engine_launch_batch_locked(tfhe_amd_device_key* dk, int count, const int32_t
                                char* scr, void* stream, bool capturing) {
     return gates_device_locked(ctx(), dk, count, d_op, d_in0, d_in1, d_in2, d_out, d_pool, nullptr, 1 << 29, scr,
                                (hipStream_t)stream, !capturing);
+}
+
+bool engine_can_segment(const tfhe_amd_device_key* dk) {
+    return dk && dk->M == 512 && dk->bkf && (dk->P.l == 2 || dk->P.l == 3);
+}
+
+size_t engine_segments_scratch_bytes(tfhe_amd_device_key* const* dks, const int32_t* counts, int nseg) {
+    // the segments' extracted samples and job lists, then one keyswitch area reused by
+    // every segment in turn (its size is not monotone in the count, so take the largest)
+    size_t b = 0, ks = 0;
+    for (int g = 0; g < nseg; g++) {
+        b += ext_bytes(dks[g]->P, counts[g]);
+        ks = std::max(ks, ks_scratch_bytes(dks[g]->P, counts[g]));
+    }
+    return b + ks;
+}
+
+int engine_launch_segments_locked(tfhe_amd_device_key* const* dks, const int32_t* counts, int nseg,
+                                  const int32_t* d_idx, int32_t* d_pool, char* scr, void* stream) {
+    if (nseg <= 0 || nseg > SEG_MAX) return set_error("segments: 1..%d segments per launch", SEG_MAX);
+    int total = 0, maxc = 0;
+    for (int g = 0; g < nseg; g++) {
+        if (!engine_can_segment(dks[g]) || !engine_same_params(dks[0], dks[g]))
+            return set_error("segments: every key must be an N = 1024 key with the same parameters");
+        if (counts[g] <= 0) return set_error("segments: empty segment");
+        total += counts[g];
+        maxc = std::max(maxc, (int)counts[g]);
+    }
+    const HostParams& P = dks[0]->P;
+    hipStream_t s = (hipStream_t)stream;
+    // index arrays [5][total]: op, in0, in1, in2, out; segment g is [off_g, off_g + count_g)
+    BrSegTable T;
+    memset(&T, 0, sizeof T);
+    T.base = base_args(dks[0]);
+    T.base.op = d_idx;
+    T.base.in0 = d_idx + total;
+    T.base.in1 = d_idx + 2 * (size_t)total;
+    T.base.in2 = d_idx + 3 * (size_t)total;
+    T.base.pool = d_pool;
+    int32_t* second[SEG_MAX];
+    char* p = scr;
+    for (int g = 0, off = 0; g < nseg; off += counts[g], g++) {
+        const int c = counts[g];
+        int32_t* ext = (int32_t*)p;
+        char* jp = p + ext_rows_bytes(P, c);
+        int32_t* mux_gate = (int32_t*)jp;
+        second[g] = (int32_t*)(jp + align256(sizeof(int32_t) * (size_t)c));
+        int32_t* n_jobs = (int32_t*)(jp + 2 * align256(sizeof(int32_t) * (size_t)c));
+        p += ext_bytes(P, c);
+        hipLaunchKernelGGL(mux_scan_kernel, dim3(1), dim3(1024), 0, s, d_idx + off, c, mux_gate, second[g], n_jobs);
+        T.seg[g] = BrSeg{dks[g]->bkf, ext, mux_gate, n_jobs, off, c};
+    }
+    // at most 2*count jobs per segment; workgroups past a segment's job count leave at once
+    if (P.l == 3) hipLaunchKernelGGL(br4_seg_kernel<3>, dim3(2 * maxc, nseg), dim3(64), 0, s, T);
+    else hipLaunchKernelGGL(br4_seg_kernel<2>, dim3(2 * maxc, nseg), dim3(64), 0, s, T);
+    HIP_TRY(hipGetLastError());
+    // each segment's keyswitch with its own key, one after another in the shared scratch
+    for (int g = 0; g < nseg; g++)
+        if (int rc = run_keyswitch(dks[g], T.seg[g].ext, second[g], counts[g], d_pool, d_idx + 4 * (size_t)total + T.seg[g].off,
+                                   p, s))
+            return rc;
+    return 0;
 }
 
 int engine_param_n(const tfhe_amd_device_key* dk) { return dk->P.n; }

@@ -71,10 +71,18 @@ struct Request {
     Clock::time_point t_submit, t_done;
 };
 
+// A launch's completion event, shared by every client whose gates it carries.
+struct EvBox {
+    hipEvent_t e = nullptr;
+    ~EvBox() {
+        if (e) (void)hipEventDestroy(e);
+    }
+};
+
 struct Launch {
-    hipEvent_t ev = nullptr;
+    std::shared_ptr<EvBox> ev;
     std::vector<Request*> reqs;
-    int ring = 0;
+    int ring = 0;  // the client's index-ring slot; -1 for a packed launch
 };
 
 struct Client {
@@ -85,7 +93,68 @@ struct Client {
     int32_t* h_idx = nullptr;      // pinned [depth][5*max_batch]
     int32_t* d_idx = nullptr;      // device [depth][5*max_batch]
     int next_ring = 0;
+    int last_stream = 0;           // stream of the newest in-flight launch (-1: the packing stream)
 };
+
+// Gates of one client's requests in one launch: request r's sorted gates [from, to).
+struct Part {
+    Request* r;
+    int32_t from, to;
+};
+
+// The next level (or the rest of it) of each admitted request of the client, in
+// admission order, up to `limit` gates. Changes nothing.
+int gather_parts(const Client& C, int limit, std::vector<Part>& parts) {
+    int count = 0;
+    for (Request* r : C.active) {
+        if (r->pos >= r->G) continue;
+        const int32_t end = r->level_end[r->level];
+        const int take = std::min(end - r->pos, limit - count);
+        if (take <= 0) break;
+        parts.push_back({r, r->pos, r->pos + take});
+        count += take;
+    }
+    return count;
+}
+
+// Index rows of the parts into h[5][stride] (op, in0, in1, in2, out) from column
+// `at` on, wires as absolute arena slots.
+void write_parts(const std::vector<Part>& parts, int32_t* h, int stride, int at) {
+    int k = at;
+    for (const Part& p : parts) {
+        const Request* r = p.r;
+        const int32_t G = r->G;
+        const int32_t s0 = (int32_t)r->slot0;
+        for (int32_t g = p.from; g < p.to; g++, k++) {
+            h[k] = r->gates[g];
+            h[stride + k] = s0 + r->gates[G + g];
+            h[2 * stride + k] = s0 + r->gates[2 * G + g];
+            h[3 * stride + k] = s0 + r->gates[3 * G + g];
+            h[4 * stride + k] = s0 + r->gates[4 * G + g];
+        }
+    }
+}
+
+// Marks the parts launched under L.
+void advance_parts(const std::vector<Part>& parts, Launch& L) {
+    for (const Part& p : parts) {
+        Request* r = p.r;
+        r->pos = p.to;
+        if (r->pos == r->level_end[r->level]) r->level++;
+        r->inflight++;
+        L.reqs.push_back(r);
+    }
+}
+
+int record_event(hipStream_t s, std::shared_ptr<EvBox>& ev) {
+    ev = std::make_shared<EvBox>();
+    if (hipEventCreateWithFlags(&ev->e, hipEventDisableTiming) != hipSuccess || hipEventRecord(ev->e, s) != hipSuccess)
+        return set_error("server: event record failed");
+    return 0;
+}
+
+// A segment table holds at most this many clients per packed launch.
+constexpr int PACK_SEGMENTS = 32;
 
 // First-fit allocator over the slots of the wire arena.
 struct SlotAllocator {
@@ -150,13 +219,14 @@ struct tfhe_amd_server {
     size_t key_bytes = 0;
 
     std::mutex mu;  // guards everything below
+    int pack_min = 0;  // packing threshold in gates (0: each launch carries one client)
     std::condition_variable cv_work, cv_done;
     std::deque<Request*> incoming;
     std::unordered_map<int64_t, std::unique_ptr<Request>> all;
     int64_t next_id = 0;
     bool stop = false;
     std::string fatal;
-    int64_t n_submitted = 0, n_completed = 0, n_failed = 0, n_gates = 0, n_launches = 0, max_launch = 0;
+    int64_t n_submitted = 0, n_completed = 0, n_failed = 0, n_gates = 0, n_launches = 0, max_launch = 0, n_packed = 0;
     int32_t n_queued = 0, n_active = 0;
     size_t wire_in_use = 0, wire_peak = 0;
     std::thread worker;
@@ -164,10 +234,19 @@ struct tfhe_amd_server {
     // --- scheduler-thread state (touched only by the worker) ---
     std::deque<Request*> pending;
     std::vector<Client*> worker_clients;  // snapshot of `clients` for the current pass
+    // packed launches: their own stream, index ring and workspace
+    hipStream_t pack_stream = nullptr;
+    int32_t* h_pack = nullptr;  // pinned [depth][5*max_batch]
+    int32_t* d_pack = nullptr;  // device [depth][5*max_batch]
+    int pack_ring = 0;
+    std::deque<std::shared_ptr<EvBox>> pack_inflight;
+    char* pack_scr = nullptr;
+    size_t pack_scr_cap = 0;
 
     void run();
     int admit_pending(bool* progress);
-    int launch_client(int c, bool* progress);
+    int launch_client(int c, bool packing, bool* progress);
+    int launch_packed(int min_gates, bool* progress);
     int poll_client(int c, bool* progress);
     int finish(Request* r);
     int ensure_bounce(size_t bytes);
@@ -219,41 +298,17 @@ int tfhe_amd_server::admit_pending(bool* progress) {
     return 0;
 }
 
-int tfhe_amd_server::launch_client(int c, bool* progress) {
+int tfhe_amd_server::launch_client(int c, bool packing, bool* progress) {
     Client& C = *worker_clients[c];
     if ((int)C.inflight.size() >= depth) return 0;
+    // while packing, a client whose newest launch is a packed one waits for the next packed launch
+    if (packing && !C.inflight.empty() && C.last_stream < 0) return 0;
+    std::vector<Part> parts;
+    const int count = gather_parts(C, max_batch, parts);
+    if (!count) return 0;
     const int ring = C.next_ring;
     int32_t* h = C.h_idx + (size_t)ring * 5 * max_batch;
-    // gather the next level (or the rest of it) of each request, in admission order
-    struct Part {
-        Request* r;
-        int32_t from, to;
-    };
-    std::vector<Part> parts;
-    int count = 0;
-    for (Request* r : C.active) {
-        if (r->pos >= r->G) continue;
-        const int32_t end = r->level_end[r->level];
-        const int take = std::min(end - r->pos, max_batch - count);
-        if (take <= 0) break;
-        parts.push_back({r, r->pos, r->pos + take});
-        count += take;
-    }
-    if (!count) return 0;
-    // compact index arrays [5][count], wires as absolute arena slots
-    int k = 0;
-    for (const Part& p : parts) {
-        const Request* r = p.r;
-        const int32_t G = r->G;
-        const int32_t s0 = (int32_t)r->slot0;
-        for (int32_t g = p.from; g < p.to; g++, k++) {
-            h[k] = r->gates[g];
-            h[count + k] = s0 + r->gates[G + g];
-            h[2 * count + k] = s0 + r->gates[2 * G + g];
-            h[3 * count + k] = s0 + r->gates[3 * G + g];
-            h[4 * count + k] = s0 + r->gates[4 * G + g];
-        }
-    }
+    write_parts(parts, h, count, 0);
     const int si = C.stream;
     hipStream_t s = streams[si];
     const size_t need = engine_batch_scratch_bytes(C.dk, max_batch);
@@ -265,6 +320,9 @@ int tfhe_amd_server::launch_client(int c, bool* progress) {
         if (hipMalloc(&scratch[si], need) != hipSuccess) return set_error("server: scratch allocation failed");
         scratch_cap[si] = need;
     }
+    // the client's previous level ran in a packed launch: order after it on the device
+    if (!C.inflight.empty() && C.last_stream != si && hipStreamWaitEvent(s, C.inflight.back().ev->e, 0) != hipSuccess)
+        return set_error("server: stream wait failed");
     int32_t* d = C.d_idx + (size_t)ring * 5 * max_batch;
     if (hipMemcpyAsync(d, h, sizeof(int32_t) * 5 * (size_t)count, hipMemcpyHostToDevice, s) != hipSuccess)
         return set_error("server: index upload failed");
@@ -276,22 +334,113 @@ int tfhe_amd_server::launch_client(int c, bool* progress) {
     }
     Launch L;
     L.ring = ring;
-    if (hipEventCreateWithFlags(&L.ev, hipEventDisableTiming) != hipSuccess || hipEventRecord(L.ev, s) != hipSuccess)
-        return set_error("server: event record failed");
-    for (const Part& p : parts) {
-        Request* r = p.r;
-        r->pos = p.to;
-        if (r->pos == r->level_end[r->level]) r->level++;
-        r->inflight++;
-        L.reqs.push_back(r);
-    }
+    if (int rc = record_event(s, L.ev)) return rc;
+    advance_parts(parts, L);
     C.inflight.push_back(std::move(L));
+    C.last_stream = si;
     C.next_ring = (ring + 1) % depth;
     {
         std::lock_guard<std::mutex> g(mu);
         n_launches++;
         n_gates += count;
         max_launch = std::max<int64_t>(max_launch, count);
+    }
+    *progress = true;
+    return 0;
+}
+
+// One launch carrying the next level of several clients (each with its own key,
+// as a segment of the batched blind-rotation kernel), made when at least two
+// clients have gates ready and together at least `min_gates`. Packed launches
+// run in order on their own stream, at most `depth` in flight.
+int tfhe_amd_server::launch_packed(int min_gates, bool* progress) {
+    while (!pack_inflight.empty()) {
+        const hipError_t e = hipEventQuery(pack_inflight.front()->e);
+        if (e == hipErrorNotReady) break;
+        if (e != hipSuccess) return set_error("server: packed launch failed: %s", hipGetErrorString(e));
+        pack_inflight.pop_front();
+    }
+    if ((int)pack_inflight.size() >= depth) return 0;
+    std::vector<int> cand;
+    std::vector<std::vector<Part>> cparts;
+    int total = 0;
+    const tfhe_amd_device_key* ref = nullptr;
+    for (size_t c = 0; c < worker_clients.size() && (int)cand.size() < PACK_SEGMENTS && total < max_batch; c++) {
+        const Client& C = *worker_clients[c];
+        if ((int)C.inflight.size() >= depth || !engine_can_segment(C.dk)) continue;
+        if (ref && !engine_same_params(ref, C.dk)) continue;
+        std::vector<Part> parts;
+        const int cnt = gather_parts(C, max_batch - total, parts);
+        if (!cnt) continue;
+        if (!ref) ref = C.dk;
+        cand.push_back((int)c);
+        cparts.push_back(std::move(parts));
+        total += cnt;
+    }
+    if (cand.size() < 2 || total < min_gates) return 0;
+    if (!pack_stream) {
+        const size_t idx_bytes = sizeof(int32_t) * 5 * (size_t)max_batch * depth;
+        if (hipStreamCreateWithFlags(&pack_stream, hipStreamNonBlocking) != hipSuccess ||
+            hipHostMalloc((void**)&h_pack, idx_bytes, hipHostMallocDefault) != hipSuccess ||
+            hipMalloc(&d_pack, idx_bytes) != hipSuccess)
+            return set_error("server: packing stream or index buffers failed");
+    }
+    const int nseg = (int)cand.size();
+    int32_t* h = h_pack + (size_t)pack_ring * 5 * max_batch;
+    std::vector<tfhe_amd_device_key*> dks(nseg);
+    std::vector<int32_t> counts(nseg);
+    for (int k = 0, at = 0; k < nseg; k++) {
+        int cnt = 0;
+        for (const Part& p : cparts[k]) cnt += p.to - p.from;
+        write_parts(cparts[k], h, total, at);
+        dks[k] = worker_clients[cand[k]]->dk;
+        counts[k] = cnt;
+        at += cnt;
+    }
+    const size_t need = engine_segments_scratch_bytes(dks.data(), counts.data(), nseg);
+    if (pack_scr_cap < need) {
+        if (hipStreamSynchronize(pack_stream) != hipSuccess) return set_error("server: stream synchronize failed");
+        (void)hipFree(pack_scr);
+        pack_scr = nullptr;
+        pack_scr_cap = 0;
+        const size_t cap = need + need / 4;
+        if (hipMalloc(&pack_scr, cap) != hipSuccess) return set_error("server: packing workspace allocation failed");
+        pack_scr_cap = cap;
+    }
+    // each client's previous level may still run on its own stream
+    for (int k = 0; k < nseg; k++) {
+        const Client& C = *worker_clients[cand[k]];
+        if (!C.inflight.empty() && C.last_stream >= 0 &&
+            hipStreamWaitEvent(pack_stream, C.inflight.back().ev->e, 0) != hipSuccess)
+            return set_error("server: stream wait failed");
+    }
+    int32_t* d = d_pack + (size_t)pack_ring * 5 * max_batch;
+    if (hipMemcpyAsync(d, h, sizeof(int32_t) * 5 * (size_t)total, hipMemcpyHostToDevice, pack_stream) != hipSuccess)
+        return set_error("server: index upload failed");
+    {
+        std::lock_guard<std::mutex> g(engine_mutex());
+        if (int rc = engine_launch_segments_locked(dks.data(), counts.data(), nseg, d, arena, pack_scr, pack_stream))
+            return rc;
+    }
+    std::shared_ptr<EvBox> ev;
+    if (int rc = record_event(pack_stream, ev)) return rc;
+    pack_inflight.push_back(ev);
+    pack_ring = (pack_ring + 1) % depth;
+    for (int k = 0; k < nseg; k++) {
+        Client& C = *worker_clients[cand[k]];
+        Launch L;
+        L.ring = -1;
+        L.ev = ev;
+        advance_parts(cparts[k], L);
+        C.inflight.push_back(std::move(L));
+        C.last_stream = -1;
+    }
+    {
+        std::lock_guard<std::mutex> g(mu);
+        n_launches++;
+        n_packed++;
+        n_gates += total;
+        max_launch = std::max<int64_t>(max_launch, total);
     }
     *progress = true;
     return 0;
@@ -327,10 +476,9 @@ int tfhe_amd_server::poll_client(int c, bool* progress) {
     Client& C = *worker_clients[c];
     while (!C.inflight.empty()) {
         Launch& L = C.inflight.front();
-        const hipError_t e = hipEventQuery(L.ev);
+        const hipError_t e = hipEventQuery(L.ev->e);
         if (e == hipErrorNotReady) return 0;
         if (e != hipSuccess) return set_error("server: launch failed: %s", hipGetErrorString(e));
-        (void)hipEventDestroy(L.ev);
         std::vector<Request*> reqs = std::move(L.reqs);
         C.inflight.pop_front();
         for (Request* r : reqs)
@@ -365,6 +513,7 @@ void tfhe_amd_server::run() {
     }
     for (;;) {
         size_t ncl;
+        int pack_now;
         {
             // clients are appended under the lock; the worker only touches the
             // first ncl of them in this pass (their lists are worker-owned)
@@ -374,6 +523,7 @@ void tfhe_amd_server::run() {
                 incoming.pop_front();
             }
             ncl = clients.size();
+            pack_now = pack_min;
             bool busy = !pending.empty();
             for (size_t c = 0; c < ncl; c++) busy = busy || !clients[c]->active.empty() || !clients[c]->inflight.empty();
             if (stop && !busy) break;
@@ -384,16 +534,18 @@ void tfhe_amd_server::run() {
         int rc = 0;
         for (size_t c = 0; c < ncl && !rc; c++) rc = poll_client((int)c, &progress);
         if (!rc) rc = admit_pending(&progress);
-        for (size_t c = 0; c < ncl && !rc; c++) rc = launch_client((int)c, &progress);
+        if (!rc && pack_now > 0) rc = launch_packed(pack_now, &progress);
+        for (size_t c = 0; c < ncl && !rc; c++) rc = launch_client((int)c, pack_now > 0, &progress);
         if (rc) {
             fail_all(tfhe_amd_last_error());
             // drain what is queued on the device before the buffers go away
             for (auto s : streams) (void)hipStreamSynchronize(s);
+            if (pack_stream) (void)hipStreamSynchronize(pack_stream);
             for (Client* cl : worker_clients) {
-                for (auto& L : cl->inflight) (void)hipEventDestroy(L.ev);
                 cl->inflight.clear();
                 cl->active.clear();
             }
+            pack_inflight.clear();
             pending.clear();
             std::unique_lock<std::mutex> lk(mu);
             cv_work.wait(lk, [&] { return stop; });
@@ -593,6 +745,20 @@ TFHE_AMD_EXPORT int32_t tfhe_amd_server_get_stats(tfhe_amd_server* S, tfhe_amd_s
     return 0;
 }
 
+TFHE_AMD_EXPORT int32_t tfhe_amd_server_set_packing(tfhe_amd_server* S, int32_t min_gates) {
+    if (!S || min_gates < 0) return set_error("server_set_packing: bad arguments");
+    std::lock_guard<std::mutex> g(S->mu);
+    S->pack_min = min_gates;
+    S->cv_work.notify_all();
+    return 0;
+}
+
+TFHE_AMD_EXPORT int64_t tfhe_amd_server_packed_launches(tfhe_amd_server* S) {
+    if (!S) return set_error("server_packed_launches: bad arguments");
+    std::lock_guard<std::mutex> g(S->mu);
+    return S->n_packed;
+}
+
 TFHE_AMD_EXPORT void tfhe_amd_server_destroy(tfhe_amd_server* S) {
     if (!S) return;
     {
@@ -612,11 +778,19 @@ TFHE_AMD_EXPORT void tfhe_amd_server_destroy(tfhe_amd_server* S) {
         S->cv_done.notify_all();
     }
     for (auto s : S->streams) (void)hipStreamSynchronize(s);
+    if (S->pack_stream) (void)hipStreamSynchronize(S->pack_stream);
     for (auto& cl : S->clients) {
         (void)hipHostFree(cl->h_idx);
         (void)hipFree(cl->d_idx);
     }
     for (char* p : S->scratch) (void)hipFree(p);
+    if (S->pack_stream) {
+        S->pack_inflight.clear();
+        (void)hipHostFree(S->h_pack);
+        (void)hipFree(S->d_pack);
+        (void)hipFree(S->pack_scr);
+        (void)hipStreamDestroy(S->pack_stream);
+    }
     for (auto s : S->streams) (void)hipStreamDestroy(s);
     (void)hipStreamDestroy(S->io);
     (void)hipFree(S->arena);
