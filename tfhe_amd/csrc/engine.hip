@@ -8,6 +8,8 @@
 //                      accumulator in LDS (the explicit-accumulator entry points)
 //   br4_kernel         blind rotation for large batches: one bootstrap per
 //                      one-wavefront workgroup, key read from global memory
+//   br4_seg_kernel     the same over several clients' batches in one launch,
+//                      each segment with its own key (serving)
 //   br2_kernel         blind rotation, two wavefronts per bootstrap (one per TLWE
 //                      component), key blocks shared through LDS
 //   brl_kernel         blind rotation for one gate: 4l wavefronts per bootstrap on
