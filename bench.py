@@ -56,6 +56,7 @@ def parse():
     ap.add_argument("--serve-min-requests", type=int, default=4)
     ap.add_argument("--serve-streams", type=int, default=16, help="serving section: streams the clients share")
     ap.add_argument("--serve-depth", type=int, default=2, help="serving section: launches in flight per client")
+    ap.add_argument("--serve-max-batch", type=int, default=4096, help="serving section: gates per launch at most")
     ap.add_argument("--serve-pack", type=int, default=1024,
                     help="serving section: pack several clients' levels into one launch from this many ready "
                          "gates on (0 = off)")
@@ -150,15 +151,15 @@ def bench_serving(T, P, args):
             print(f"[bench] serving: {i + 1} client keys generated on the device "
                   f"({time.perf_counter() - t0:.1f} s)", file=sys.stderr, flush=True)
     keygen_s = time.perf_counter() - t0
-    S = Server(n_streams=args.serve_streams, depth=args.serve_depth, max_batch=4096, wire_budget_bytes=4 << 30,
-               pack_min_gates=args.serve_pack)
+    S = Server(n_streams=args.serve_streams, depth=args.serve_depth, max_batch=args.serve_max_batch,
+               wire_budget_bytes=4 << 30, pack_min_gates=args.serve_pack)
     cids = [S.add_client(sk.cloud, P.n) for sk in keys]
     mean_gates = 32.5 * 256.5
     out = {"workload": f"{n_cl} clients (own P128 key each, generated on the device, resident), Poisson arrivals, "
                        "random layered circuits "
                        "depth U[1,64] x width U[1,512], 11 gate types uniform", "clients": n_cl,
            "streams": args.serve_streams, "launches_in_flight_per_client": args.serve_depth,
-           "pack_min_gates": args.serve_pack,
+           "pack_min_gates": args.serve_pack, "max_batch": args.serve_max_batch,
            "key_bytes_resident": S.stats()["key_bytes"], "keygen_and_upload_s": round(keygen_s, 2),
            "rates": {}}
     rng = np.random.default_rng(args.seed + 5)
