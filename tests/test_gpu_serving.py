@@ -1,8 +1,9 @@
 """Serving loop on the GPU (tfhe_amd_server_*): continuous batching of circuit
 requests from several clients with their own keys. A request's outputs are
-bit-identical alone, in a mix of concurrent requests, and through the netlist
-scheduler; every output decrypts to the plaintext evaluation; admission keeps
-the wire arena within its budget."""
+bit-identical alone, in a mix of concurrent requests, through the netlist
+scheduler and with several clients packed into one launch; every output
+decrypts to the plaintext evaluation; admission keeps the wire arena within its
+budget."""
 import numpy as np
 import pytest
 
