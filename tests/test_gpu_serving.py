@@ -108,6 +108,37 @@ def test_packed_launches_bit_identical(keys):
         assert np.array_equal(sks[c].decrypt(o1), C.evaluate(b)[lw])
 
 
+def test_packing_more_clients_than_segments(keys):
+    """34 clients with their own device-generated keys submit together: packed
+    launches fill the 32-entry segment table and the rest of the ready clients
+    launch on their own streams or in the next packed launch; every output is
+    bit-identical to the unpacked server's and decrypts to the plaintext."""
+    T, P, _ = keys
+    sks = [T.SecretKey.generate_on_device(P, seed=500 + i) for i in range(34)]
+    try:
+        rng = np.random.default_rng(12)
+        reqs = [(i,) + _request(sks[i], rng, int(rng.integers(2, 6)), int(rng.integers(20, 120))) for i in range(34)]
+        outs, packed = {}, {}
+        for pack in (0, 1):
+            S, cids = _server(T, P, sks, n_streams=4, depth=2, max_batch=4096, wire_budget_bytes=256 << 20,
+                              pack_min_gates=pack)
+            try:
+                ids = [S.submit(cids[c], C, np.arange(len(b)), ct, lw) for c, C, b, ct, lw in reqs]
+                outs[pack] = [S.wait(r, 120000)[0] for r in ids]
+                st = S.stats()
+            finally:
+                S.close()
+            assert st["failed"] == 0 and st["completed"] == len(reqs)
+            packed[pack] = st["packed_launches"]
+        assert packed[0] == 0 and packed[1] > 0
+        for (c, C, b, ct, lw), o0, o1 in zip(reqs, outs[0], outs[1]):
+            assert np.array_equal(o0, o1)
+            assert np.array_equal(sks[c].decrypt(o1), C.evaluate(b)[lw])
+    finally:
+        for sk in sks:
+            sk.close()
+
+
 def test_merging_and_overlap(keys):
     """Many small requests of one client submitted at once run in fewer launches
     than levels submitted, and the largest launch holds several requests."""
