@@ -78,6 +78,20 @@ def test_batch_rejects_bad_gate_code_without_touching_the_gpu():
     sk.close()
 
 
+def test_server_calls_reject_a_missing_server():
+    """The serving entry points check their handle before anything else: a null
+    server is an error with a reason, not a device call."""
+    L = T.lib()
+    L.tfhe_amd_server_set_packing.argtypes = [ctypes.c_void_p, ctypes.c_int32]
+    L.tfhe_amd_server_set_packing.restype = ctypes.c_int32
+    L.tfhe_amd_server_packed_launches.argtypes = [ctypes.c_void_p]
+    L.tfhe_amd_server_packed_launches.restype = ctypes.c_int64
+    assert L.tfhe_amd_server_set_packing(None, 1024) == -1
+    assert b"set_packing" in L.tfhe_amd_last_error()
+    assert L.tfhe_amd_server_packed_launches(None) == -1
+    assert b"packed_launches" in L.tfhe_amd_last_error()
+
+
 def test_modswitch_and_dtot32_match_oracle():
     L = T.lib()
     rng = np.random.default_rng(2)
