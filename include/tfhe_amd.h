@@ -219,7 +219,10 @@ TFHE_AMD_EXPORT const TFheGateBootstrappingCloudKeySet* tfhe_amd_cloud_of(const 
 /* ---- device-resident API (keys and ciphertexts stay in HBM) ------------- */
 /* Uploads (or returns the already uploaded) device key of a cloud key set. */
 TFHE_AMD_EXPORT int32_t tfhe_amd_cloud_key_upload(const TFheGateBootstrappingCloudKeySet* ck);
-/* Frees the device copy; the host key stays valid and is re-uploaded on next use. */
+/* Frees the device copy; the host key stays valid and is re-uploaded on next use
+ * (a netlist captured into a graph on the old copy is re-captured on its next
+ * run). A copy that a server registered (tfhe_amd_server_add_client) stays
+ * allocated for that server until tfhe_amd_server_destroy. */
 TFHE_AMD_EXPORT void tfhe_amd_cloud_key_release(const TFheGateBootstrappingCloudKeySet* ck);
 /* Bytes of HBM the device key of ck occupies (0 if not uploaded). */
 TFHE_AMD_EXPORT size_t tfhe_amd_cloud_key_device_bytes(const TFheGateBootstrappingCloudKeySet* ck);
@@ -312,9 +315,15 @@ TFHE_AMD_EXPORT void delete_LweKey(LweKey* key);
  * be NULL. encrypt: bootsSymEncrypt (tfhe_gate_bootstrapping.cpp:154-159) of
  * bits[g] with noise stdev alpha_min, randomness from Philox4x32-10 keyed by
  * seed with ciphertext index first_index + g as counter (reproducible from
- * (seed, index); not the reference generator's stream). The LWE key is copied
- * to the device on first use and freed with the key set. Asynchronous on
- * stream (NULL = the library's stream). */
+ * (seed, index); not the reference generator's stream). A (seed, index) pair
+ * must never be used twice under one key: two ciphertexts with the same mask
+ * and noise reveal whether their bits differ. first_index =
+ * TFHE_AMD_NEXT_INDEX takes the key's next unused indices (the library keeps,
+ * per key, one past the highest index any call has used, explicit ones
+ * included), which is what callers without their own index bookkeeping
+ * should pass. The LWE key is copied to the device on first use and freed
+ * with the key set. Asynchronous on stream (NULL = the library's stream). */
+#define TFHE_AMD_NEXT_INDEX UINT64_MAX
 TFHE_AMD_EXPORT int32_t tfhe_amd_encrypt_device(const TFheGateBootstrappingSecretKeySet* key, int32_t count,
                                                 const int32_t* d_bits, int32_t* d_cts, uint64_t seed,
                                                 uint64_t first_index, void* stream);
@@ -350,7 +359,9 @@ typedef struct {
 } tfhe_amd_server_stats;
 TFHE_AMD_EXPORT tfhe_amd_server* tfhe_amd_server_create(int32_t n_streams, int32_t depth, int32_t max_batch,
                                                         size_t wire_budget_bytes);
-/* Registers a client's cloud key (uploaded now if needed); returns the client id. */
+/* Registers a client's cloud key (uploaded now if needed); returns the client id.
+ * The server holds the device copy until it is destroyed, so the key set may be
+ * released or deleted meanwhile without affecting the server. */
 TFHE_AMD_EXPORT int32_t tfhe_amd_server_add_client(tfhe_amd_server* srv, const TFheGateBootstrappingCloudKeySet* ck);
 /* Submits a netlist (same conventions as tfhe_amd_netlist_create) whose
  * undriven wires input_wires[0..n_inputs) hold input_cts[n_inputs][n+1]

@@ -52,7 +52,7 @@ def test_encrypt_device_statistics_and_reproducibility(key):
     bits = rng.integers(0, 2, count).astype(np.int32)
     d_bits = T.DeviceArray.from_numpy(bits)
     d_cts = T.DeviceArray((count, P.n + 1))
-    T.encrypt_device(sk, d_bits, d_cts, seed=0x1234_5678_9ABC)
+    T.encrypt_device(sk, d_bits, d_cts, seed=0x1234_5678_9ABC, first_index=0)
     T.synchronize()
     cts = d_cts.numpy()
     # decrypts to the bits; noise e = phase - mu has stdev alpha = 2^-15, mean 0
@@ -78,6 +78,36 @@ def test_encrypt_device_statistics_and_reproducibility(key):
     assert not np.array_equal(d_cts.numpy()[:1000, :P.n], cts[:1000, :P.n])
     for x in (d_bits, d_cts, d_tail, d_tb):
         x.free()
+
+
+def test_encrypt_device_default_index_never_repeats(key):
+    """Calls that leave the index to the library never reuse a (seed, index)
+    pair under one key: two default calls with the same seed give different
+    masks, and a default call after explicit indices starts past all of them
+    (equal masks would reveal whether two plaintext bits differ)."""
+    T, P, sk = key
+    w = P.n + 1
+    bits = np.ones(64, np.int32)
+    d_bits = T.DeviceArray.from_numpy(bits)
+    d1, d2, d3, d4 = (T.DeviceArray((64, w)) for _ in range(4))
+    T.encrypt_device(sk, d_bits, d1, seed=777)
+    T.encrypt_device(sk, d_bits, d2, seed=777)
+    T.encrypt_device(sk, d_bits, d3, seed=777, first_index=10 ** 9)
+    T.encrypt_device(sk, d_bits, d4, seed=777)
+    T.synchronize()
+    c1, c2, c3, c4 = (d.numpy() for d in (d1, d2, d3, d4))
+    for x, y in ((c1, c2), (c1, c4), (c2, c4), (c3, c4)):
+        assert not np.any(np.all(x[:, :P.n][:, None, :] == y[:, :P.n][None, :, :], axis=2))
+    # the default call after the explicit one continued at 10^9 + 64: re-encrypting
+    # there explicitly reproduces it
+    d5 = T.DeviceArray((64, w))
+    T.encrypt_device(sk, d_bits, d5, seed=777, first_index=10 ** 9 + 64)
+    T.synchronize()
+    assert np.array_equal(d5.numpy(), c4)
+    for b in (d1, d2, d3, d4, d5):
+        assert np.array_equal(sk.decrypt(b.numpy()), bits)
+        b.free()
+    d_bits.free()
 
 
 def _truth_all(op, a, b, c):

@@ -60,3 +60,46 @@ def test_levelize_errors():
     C.op[0] = 17
     with pytest.raises(RuntimeError, match="bad gate code"):
         levelize(C)
+
+
+def test_mux_without_third_input_is_rejected():
+    """A MUX gate in a netlist whose in2 array is NULL is an error (not MUX(a, b, a))."""
+    import ctypes
+    from tfhe_amd import i32p, lib
+    from tfhe_amd.circuits import _bind
+    L = _bind()
+    op = np.array([10], np.int32)
+    in0, in1, out = (np.array([v], np.int32) for v in (0, 1, 3))
+    p = lambda a: a.ctypes.data_as(i32p)
+    lv = np.zeros(1, np.int32)
+    n = L.tfhe_amd_netlist_levelize(4, 1, p(op), p(in0), p(in1), None, p(out), 0, p(lv))
+    assert n < 0
+    assert "MUX needs a third input" in lib().tfhe_amd_last_error().decode()
+    # the same netlist with in2 given levelises
+    in2 = np.array([2], np.int32)
+    assert L.tfhe_amd_netlist_levelize(4, 1, p(op), p(in0), p(in1), p(in2), p(out), 0, p(lv)) == 1
+
+
+class _FakeDevice:
+    """Stands in for a DeviceArray in argument checks that fail before any device call."""
+
+    def __init__(self, shape):
+        self.shape = tuple(shape)
+        self.ptr = None
+
+
+def test_python_wrappers_check_buffer_shapes():
+    import tfhe_amd as T
+    P = T.default_params(110)
+    sk = T.SecretKey(P, seed=3)
+    try:
+        with pytest.raises(ValueError, match="rows of n\\+1"):
+            T.encrypt_device(sk, _FakeDevice((10,)), _FakeDevice((9, P.n + 1)), seed=1)
+        with pytest.raises(ValueError, match="exceeds"):
+            T.encrypt_device(sk, _FakeDevice((10,)), _FakeDevice((20, P.n + 1)), seed=1, count=11)
+        with pytest.raises(ValueError, match="outside d_cts"):
+            T.decrypt_device(sk, _FakeDevice((10, P.n + 1)), _FakeDevice((20,)), count=5, offset_rows=6)
+        with pytest.raises(ValueError, match="d_bits holds fewer"):
+            T.decrypt_device(sk, _FakeDevice((10, P.n + 1)), _FakeDevice((4,)))
+    finally:
+        sk.close()

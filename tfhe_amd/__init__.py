@@ -424,17 +424,37 @@ def gates_device(cloud, ops, in0, in1, in2, out, pool, stream=None):
           "gates_device")
 
 
-def encrypt_device(sk, d_bits, d_cts, seed, first_index=0, count=None, stream=None):
-    """Device encryption of count bits (DeviceArray) into ciphertext rows d_cts."""
+NEXT_INDEX = (1 << 64) - 1  # TFHE_AMD_NEXT_INDEX: the key's next unused ciphertext indices
+
+
+def encrypt_device(sk, d_bits, d_cts, seed, first_index=None, count=None, stream=None):
+    """Device encryption of count bits (DeviceArray) into ciphertext rows d_cts.
+
+    first_index=None takes the key's next unused ciphertext indices, so calls
+    never repeat a (seed, index) pair; pass an explicit index only to reproduce
+    earlier ciphertexts."""
     count = d_bits.shape[0] if count is None else count
-    check(lib().tfhe_amd_encrypt_device(sk.ptr, count, d_bits.ptr, d_cts.ptr, seed, first_index, stream),
+    w = sk.params.n + 1
+    if count < 0 or count > int(np.prod(d_bits.shape)):
+        raise ValueError(f"encrypt_device: count {count} exceeds the {int(np.prod(d_bits.shape))} bits given")
+    if count * w > int(np.prod(d_cts.shape)):
+        raise ValueError(f"encrypt_device: d_cts {d_cts.shape} holds fewer than {count} rows of n+1 = {w} words")
+    idx = NEXT_INDEX if first_index is None else int(first_index)
+    check(lib().tfhe_amd_encrypt_device(sk.ptr, count, d_bits.ptr, d_cts.ptr, seed, idx, stream),
           "encrypt_device")
 
 
 def decrypt_device(sk, d_cts, d_bits=None, d_phases=None, count=None, stream=None, offset_rows=0):
     """Device decryption of count ciphertext rows starting at row offset_rows of d_cts."""
-    count = d_cts.shape[0] - offset_rows if count is None else count
-    src = d_cts.ptr + 4 * offset_rows * (sk.params.n + 1)
+    w = sk.params.n + 1
+    rows = int(np.prod(d_cts.shape)) // w
+    count = rows - offset_rows if count is None else count
+    if count < 0 or offset_rows < 0 or offset_rows + count > rows:
+        raise ValueError(f"decrypt_device: rows [{offset_rows}, {offset_rows + count}) outside d_cts ({rows} rows)")
+    for d, name in ((d_bits, "d_bits"), (d_phases, "d_phases")):
+        if d is not None and count > int(np.prod(d.shape)):
+            raise ValueError(f"decrypt_device: {name} holds fewer than {count} entries")
+    src = d_cts.ptr + 4 * offset_rows * w
     check(lib().tfhe_amd_decrypt_device(sk.ptr, count, src, d_bits.ptr if d_bits else None,
                                         d_phases.ptr if d_phases else None, stream), "decrypt_device")
 

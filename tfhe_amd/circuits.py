@@ -197,9 +197,13 @@ class Server:
 
     def submit(self, client, circuit, input_wires, input_cts, output_wires):
         op, in0, in1, in2, out = circuit.arrays()
-        iw = np.ascontiguousarray(input_wires, np.int32)
+        iw = np.ascontiguousarray(input_wires, np.int32).reshape(-1)
         ic = np.ascontiguousarray(input_cts, np.int32)
-        ow = np.ascontiguousarray(output_wires, np.int32)
+        ow = np.ascontiguousarray(output_wires, np.int32).reshape(-1)
+        if self.width is None:
+            raise ValueError("submit: no client registered")
+        if ic.shape != (len(iw), self.width):
+            raise ValueError(f"submit: input_cts has shape {ic.shape}, expected ({len(iw)}, {self.width})")
         dst = np.zeros((len(ow), self.width), np.int32)
         p = lambda a: a.ctypes.data_as(i32p)
         rid = self.L.tfhe_amd_server_submit(self.ptr, client, circuit.n_wires, len(op), p(op), p(in0), p(in1),

@@ -208,9 +208,16 @@ __global__ void __launch_bounds__(1024) keygen_bk_kernel(int N, int k, int l, in
     }
 }
 
+// Per LWE secret key: its device copy and the first ciphertext index that no
+// encryption under this key has used yet.
+struct DevKey {
+    int32_t* d = nullptr;
+    uint64_t next_index = 0;
+};
+
 struct KeyCache {
     std::mutex mu;
-    std::map<const LweKey*, int32_t*> dev;
+    std::map<const LweKey*, DevKey> dev;
 };
 
 KeyCache& key_cache() {
@@ -218,20 +225,33 @@ KeyCache& key_cache() {
     return c;
 }
 
-const int32_t* device_lwe_key(const LweKey* k, int n) {
+// Device copy of the key. With `index` non-null it also reserves `count`
+// ciphertext indices: *index = TFHE_AMD_NEXT_INDEX takes the next unused ones;
+// an explicit *index is kept, and the key's next unused index moves past it.
+const int32_t* device_lwe_key(const LweKey* k, int n, uint64_t* index = nullptr, uint64_t count = 0) {
     KeyCache& C = key_cache();
     std::lock_guard<std::mutex> g(C.mu);
     auto it = C.dev.find(k);
-    if (it != C.dev.end()) return it->second;
-    int32_t* d = nullptr;
-    if (hipMalloc(&d, sizeof(int32_t) * n) != hipSuccess ||
-        hipMemcpy(d, k->key, sizeof(int32_t) * n, hipMemcpyHostToDevice) != hipSuccess) {
-        (void)hipFree(d);
-        set_error("device copy of the LWE secret key failed");
-        return nullptr;
+    if (it == C.dev.end()) {
+        int32_t* d = nullptr;
+        if (hipMalloc(&d, sizeof(int32_t) * n) != hipSuccess ||
+            hipMemcpy(d, k->key, sizeof(int32_t) * n, hipMemcpyHostToDevice) != hipSuccess) {
+            (void)hipFree(d);
+            set_error("device copy of the LWE secret key failed");
+            return nullptr;
+        }
+        it = C.dev.emplace(k, DevKey{d, 0}).first;
     }
-    C.dev[k] = d;
-    return d;
+    DevKey& e = it->second;
+    if (index) {
+        if (*index == TFHE_AMD_NEXT_INDEX) *index = e.next_index;
+        if (*index > TFHE_AMD_NEXT_INDEX - 1 - count) {
+            set_error("encrypt_device: ciphertext index range overflows");
+            return nullptr;
+        }
+        e.next_index = std::max(e.next_index, *index + count);
+    }
+    return e.d;
 }
 
 }  // namespace
@@ -242,7 +262,7 @@ void engine_forget_secret_key(const LweKey* k) {
     auto it = C.dev.find(k);
     if (it == C.dev.end()) return;
     (void)hipDeviceSynchronize();
-    (void)hipFree(it->second);
+    (void)hipFree(it->second.d);
     C.dev.erase(it);
 }
 
@@ -257,7 +277,7 @@ TFHE_AMD_EXPORT int32_t tfhe_amd_encrypt_device(const TFheGateBootstrappingSecre
         return set_error("encrypt_device: bad arguments");
     if (!count) return 0;
     const int n = key->lwe_key->params->n;
-    const int32_t* dkey = device_lwe_key(key->lwe_key, n);
+    const int32_t* dkey = device_lwe_key(key->lwe_key, n, &first_index, (uint64_t)count);
     if (!dkey) return -1;
     const int32_t mu = (int32_t)(1u << 29);  // modSwitchToTorus32(1, 8)
     const double alpha = key->params->in_out_params->alpha_min;

@@ -40,7 +40,14 @@ int engine_blind_rotate_host(tfhe_amd_device_key* dk, int32_t row0, int32_t n, c
                              int32_t* ext_out);
 int engine_keyswitch_host(tfhe_amd_device_key* dk, const int32_t* in, int32_t* out);
 
+// Drops the host key's device copy. A copy pinned by a server stays allocated
+// (and in use by that server) until its last unpin.
 void engine_forget_host_key(const LweBootstrappingKeyFFT* bkfft);
+// Identity of one upload (a re-uploaded key gets a new one), and pins that keep a
+// device key alive after its host key releases it or is deleted.
+uint64_t engine_key_gen(const tfhe_amd_device_key* dk);
+tfhe_amd_device_key* engine_key_pinned(const LweBootstrappingKeyFFT* bf);
+void engine_key_unpin(tfhe_amd_device_key* dk);
 // Frees the device copy of an LWE secret key (client-side device encryption).
 void engine_forget_secret_key(const LweKey* k);
 

@@ -60,6 +60,9 @@ struct tfhe_amd_device_key {
     c64* twh;      // M = 512 only: HALF_TW_COUNT
     int8_t* kp;    // t = 8, basebit = 2: keyswitch key as signed byte planes [4][k*N][npad][32] (int8 matrix cores)
     size_t bytes;
+    uint64_t gen;  // unique per upload: a re-uploaded key never compares equal to a freed one
+    int pins;      // holders besides the host key (servers); the memory is freed when the last one lets go
+    bool orphaned; // the host key has forgotten it (release or delete) while it was pinned
 };
 
 namespace tfhe_amd {
@@ -1771,25 +1774,39 @@ KsPlan ks_plan(const HostParams& P, int count) {
     return k;
 }
 
-size_t ks_scratch_bytes(const HostParams& P, int count) {
+// Keyswitch scratch of one path (mma: the matrix-core kernels; else the
+// lanes-over-gates ones), or of either (the larger) when the path may change.
+size_t ks_scratch_bytes_path(const HostParams& P, int count, bool mma) {
+    if (mma) {
+        const size_t cpad_m = (size_t)(count + 127) / 128 * 128;
+        return align256(sizeof(uint32_t) * (size_t)P.k * P.N * cpad_m) + align256(sizeof(int32_t) * cpad_m);
+    }
     const KsPlan k = ks_plan(P, count);
     const int npad = ((P.n + 1 + 63) / 64) * 64;
-    const size_t lanes = align256(sizeof(uint32_t) * (size_t)P.k * P.N * k.cpad) +
-                         align256(sizeof(int32_t) * (size_t)k.cpad) +
-                         align256(sizeof(int32_t) * (size_t)k.splits * k.cpad * npad);
-    const size_t cpad_m = (size_t)(count + 127) / 128 * 128;
-    const size_t mma = align256(sizeof(uint32_t) * (size_t)P.k * P.N * cpad_m) + align256(sizeof(int32_t) * cpad_m);
-    return lanes > mma ? lanes : mma;
+    return align256(sizeof(uint32_t) * (size_t)P.k * P.N * k.cpad) + align256(sizeof(int32_t) * (size_t)k.cpad) +
+           align256(sizeof(int32_t) * (size_t)k.splits * k.cpad * npad);
+}
+
+size_t ks_scratch_bytes(const HostParams& P, int count) {
+    return std::max(ks_scratch_bytes_path(P, count, true), ks_scratch_bytes_path(P, count, false));
+}
+
+// The keyswitch path a call takes now: 0 = matrix cores, 1 = lanes over gates.
+int ks_path_now(const tfhe_amd_device_key* dk) {
+    return (dk->kp && g_ks_path.load(std::memory_order_relaxed) == 0) ? 0 : 1;
 }
 
 // Keyswitch of ext rows (sum of halves for MUX) into out rows, using the
 // scratch region starting at `scr` (ks_scratch_bytes(count) bytes).
+// path: -1 = the current setting (ks_path_now), else the path to take (0 only
+// for keys with byte planes).
 int run_keyswitch(const tfhe_amd_device_key* dk, const int32_t* ext, const int32_t* second, int count, int32_t* out,
-                  const int32_t* out_idx, char* scr, hipStream_t s) {
+                  const int32_t* out_idx, char* scr, hipStream_t s, int path = -1) {
     const HostParams& P = dk->P;
     const int nin = P.k * P.N;
     const uint32_t prec_offset0 = 1u << (32 - (1 + P.ks_basebit * P.ks_t));
-    if (dk->kp && g_ks_path.load(std::memory_order_relaxed) == 0) {
+    if (path < 0) path = ks_path_now(dk);
+    if (path == 0 && dk->kp) {
         // matrix-core path: prep (abar [kN][cpad], b) then one product kernel
         const int cpad = (count + 127) / 128 * 128;
         uint32_t* abar = (uint32_t*)scr;
@@ -1842,7 +1859,8 @@ size_t ext_bytes(const HostParams& P, int count) {
 // samples instead of running the keyswitch.
 int gates_device_locked(Ctx& C, tfhe_amd_device_key* dk, int count, const int32_t* d_op, const int32_t* d_in0,
                         const int32_t* d_in1, const int32_t* d_in2, const int32_t* d_out, int32_t* d_pool,
-                        int32_t* d_ext_out, int32_t mu, char* scr, hipStream_t s, bool allow_prof = true) {
+                        int32_t* d_ext_out, int32_t mu, char* scr, hipStream_t s, bool allow_prof = true,
+                        int ks_path = -1) {
     if (count <= 0) return count == 0 ? 0 : set_error("negative gate count");
     const HostParams& P = dk->P;
     const size_t eb = ext_bytes(P, count);
@@ -1909,7 +1927,7 @@ int gates_device_locked(Ctx& C, tfhe_amd_device_key* dk, int count, const int32_
         hipLaunchKernelGGL(sum_halves_kernel, dim3(count), dim3(256), 0, s, ext, second, P.k * P.N, mu, d_ext_out);
         HIP_TRY(hipGetLastError());
     } else {
-        rc = run_keyswitch(dk, ext, second, count, d_pool, d_out, scr + eb, s);
+        rc = run_keyswitch(dk, ext, second, count, d_pool, d_out, scr + eb, s, ks_path);
     }
     if (prof) {
         HIP_TRY(hipEventRecord(ev.e[2], s));
@@ -2021,7 +2039,11 @@ static int upload_key(Ctx& C, const LweBootstrappingKeyFFT* bf, tfhe_amd_device_
     const int base = 1 << P.ks_basebit, t = P.ks_t, nin = k * N;
     const int npad = ((n + 1 + 63) / 64) * 64;
     auto* dk = new tfhe_amd_device_key();
+    static uint64_t next_gen = 0;
     dk->device = C.device;
+    dk->gen = ++next_gen;  // under C.mu
+    dk->pins = 0;
+    dk->orphaned = false;
     dk->P = P;
     dk->M = M;
     dk->kpl = kpl;
@@ -2132,7 +2154,13 @@ static int upload_key(Ctx& C, const LweBootstrappingKeyFFT* bf, tfhe_amd_device_
     return 0;
 }
 
-tfhe_amd_device_key* engine_key(const LweBootstrappingKeyFFT* bf0) {
+static tfhe_amd_device_key* engine_key_impl(const LweBootstrappingKeyFFT* bf0, bool pin);
+
+tfhe_amd_device_key* engine_key(const LweBootstrappingKeyFFT* bf0) { return engine_key_impl(bf0, false); }
+
+tfhe_amd_device_key* engine_key_pinned(const LweBootstrappingKeyFFT* bf0) { return engine_key_impl(bf0, true); }
+
+static tfhe_amd_device_key* engine_key_impl(const LweBootstrappingKeyFFT* bf0, bool pin) {
     if (!bf0) {
         set_error("null bootstrapping key");
         return nullptr;
@@ -2141,11 +2169,13 @@ tfhe_amd_device_key* engine_key(const LweBootstrappingKeyFFT* bf0) {
     std::lock_guard<std::mutex> g(C.mu);
     if (ensure_ready(C)) return nullptr;
     LweBootstrappingKeyFFT* bf = const_cast<LweBootstrappingKeyFFT*>(bf0);
-    if (bf->dkey) return bf->dkey;
-    tfhe_amd_device_key* dk = nullptr;
-    if (upload_key(C, bf, &dk)) return nullptr;
-    bf->dkey = dk;
-    return dk;
+    if (!bf->dkey) {
+        tfhe_amd_device_key* dk = nullptr;
+        if (upload_key(C, bf, &dk)) return nullptr;
+        bf->dkey = dk;
+    }
+    if (pin) bf->dkey->pins++;  // under the same lock as the lookup: no release in between
+    return bf->dkey;
 }
 
 static void free_key(Ctx& C, tfhe_amd_device_key* dk) {
@@ -2168,8 +2198,18 @@ void engine_forget_host_key(const LweBootstrappingKeyFFT* bkfft) {
     Ctx& C = ctx();
     std::lock_guard<std::mutex> g(C.mu);
     LweBootstrappingKeyFFT* bf = const_cast<LweBootstrappingKeyFFT*>(bkfft);
-    free_key(C, bf->dkey);
-    bf->dkey = nullptr;
+    tfhe_amd_device_key* dk = bf->dkey;
+    bf->dkey = nullptr;  // the next use through this host key uploads a fresh copy
+    if (dk->pins > 0) dk->orphaned = true;  // a server still runs on it; the last unpin frees it
+    else free_key(C, dk);
+}
+
+uint64_t engine_key_gen(const tfhe_amd_device_key* dk) { return dk->gen; }
+
+void engine_key_unpin(tfhe_amd_device_key* dk) {
+    Ctx& C = ctx();
+    std::lock_guard<std::mutex> g(C.mu);
+    if (--dk->pins == 0 && dk->orphaned) free_key(C, dk);
 }
 
 // ------------------------------------------------------ host-buffer paths --

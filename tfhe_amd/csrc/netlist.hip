@@ -26,6 +26,7 @@ struct tfhe_amd_netlist {
     hipGraphExec_t exec = nullptr;
     int32_t* graph_pool = nullptr;
     void* graph_stream = nullptr;
+    uint64_t graph_key = 0;  // upload id of the device key the graph's kernels read
 };
 
 namespace tfhe_amd {
@@ -38,6 +39,7 @@ int build_levels(tfhe_amd_netlist* nl, const int32_t* op, const int32_t* in0, co
     std::vector<int32_t> producer(W, -1);
     for (int g = 0; g < G; g++) {
         if (op[g] < TFHE_AMD_NAND || op[g] > TFHE_AMD_MUX) return set_error("netlist gate %d: bad gate code %d", g, op[g]);
+        if (op[g] == TFHE_AMD_MUX && !in2) return set_error("netlist gate %d: MUX needs a third input (in2 is NULL)", g);
         const int nin = op[g] == TFHE_AMD_MUX ? 3 : 2;
         const int32_t ins[3] = {in0[g], in1[g], in2 ? in2[g] : 0};
         for (int j = 0; j < nin; j++)
@@ -115,6 +117,7 @@ void drop_graph(tfhe_amd_netlist* nl) {
     nl->exec = nullptr;
     nl->graph = nullptr;
     nl->graph_pool = nullptr;
+    nl->graph_key = 0;
 }
 
 }  // namespace
@@ -209,7 +212,9 @@ TFHE_AMD_EXPORT int32_t tfhe_amd_netlist_run(tfhe_amd_netlist* nl, int32_t* d_po
     void* s = stream ? stream : engine_stream();
     if (!s) return -1;
     if (!use_graph) return launch_all(nl, dk, d_pool, s, false);
-    if (!nl->exec || nl->graph_pool != d_pool || nl->graph_stream != s) {
+    // the captured kernels hold the device key's addresses: a key released and
+    // uploaded again since the capture (a new upload id) needs a new graph
+    if (!nl->exec || nl->graph_pool != d_pool || nl->graph_stream != s || nl->graph_key != engine_key_gen(dk)) {
         drop_graph(nl);
         hipStream_t hs = (hipStream_t)s;
         if (hipStreamBeginCapture(hs, hipStreamCaptureModeThreadLocal) != hipSuccess)
@@ -229,6 +234,7 @@ TFHE_AMD_EXPORT int32_t tfhe_amd_netlist_run(tfhe_amd_netlist* nl, int32_t* d_po
         }
         nl->graph_pool = d_pool;
         nl->graph_stream = s;
+        nl->graph_key = engine_key_gen(dk);
     }
     if (hipGraphLaunch(nl->exec, (hipStream_t)s) != hipSuccess) return set_error("netlist_run: graph launch failed");
     return 0;

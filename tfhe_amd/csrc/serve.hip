@@ -601,11 +601,15 @@ TFHE_AMD_EXPORT tfhe_amd_server* tfhe_amd_server_create(int32_t n_streams, int32
 
 TFHE_AMD_EXPORT int32_t tfhe_amd_server_add_client(tfhe_amd_server* S, const TFheGateBootstrappingCloudKeySet* ck) {
     if (!S || !ck) return set_error("server_add_client: bad arguments");
-    tfhe_amd_device_key* dk = engine_key(ck->bkFFT);
+    // pinned: releasing or deleting the client's key set later leaves this device
+    // copy allocated until the server is destroyed
+    tfhe_amd_device_key* dk = engine_key_pinned(ck->bkFFT);
     if (!dk) return -1;
     std::lock_guard<std::mutex> g(S->mu);
-    if (S->first_dk && !engine_same_params(S->first_dk, dk))
+    if (S->first_dk && !engine_same_params(S->first_dk, dk)) {
+        engine_key_unpin(dk);
         return set_error("server_add_client: all clients of a server use one parameter set");
+    }
     auto cl = std::make_unique<Client>();
     cl->dk = dk;
     cl->stream = (int)S->clients.size() % S->n_streams;
@@ -613,6 +617,7 @@ TFHE_AMD_EXPORT int32_t tfhe_amd_server_add_client(tfhe_amd_server* S, const TFh
     if (hipHostMalloc((void**)&cl->h_idx, idx_bytes, hipHostMallocDefault) != hipSuccess ||
         hipMalloc(&cl->d_idx, idx_bytes) != hipSuccess) {
         if (cl->h_idx) (void)hipHostFree(cl->h_idx);
+        engine_key_unpin(dk);
         return set_error("server_add_client: staging allocation failed");
     }
     if (!S->first_dk) {
@@ -782,6 +787,7 @@ TFHE_AMD_EXPORT void tfhe_amd_server_destroy(tfhe_amd_server* S) {
     for (auto& cl : S->clients) {
         (void)hipHostFree(cl->h_idx);
         (void)hipFree(cl->d_idx);
+        engine_key_unpin(cl->dk);
     }
     for (char* p : S->scratch) (void)hipFree(p);
     if (S->pack_stream) {
