@@ -243,6 +243,20 @@ TFHE_AMD_EXPORT int32_t tfhe_amd_gates_device(const TFheGateBootstrappingCloudKe
                                               const int32_t* d_op, const int32_t* d_in0, const int32_t* d_in1,
                                               const int32_t* d_in2, const int32_t* d_out, int32_t* d_pool,
                                               void* stream);
+/* Same gates and outputs as tfhe_amd_gates_device, for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops).
+ * No counterpart in the reference. Calls no hipMalloc / hipFree: its
+ * workspace is a static 32 MiB device array per GPU, held until the process
+ * ends, which limits a call to about 2,700 gates with the 110-bit parameters
+ * (larger counts return -1 with the limit in tfhe_amd_last_error). The cloud
+ * key must already be on the device (tfhe_amd_cloud_key_upload, or any
+ * earlier call with it); otherwise -1. All calls on one GPU share the array:
+ * calls that overlap on the device (on different streams, or from several
+ * threads without ordering) race and give wrong results. */
+TFHE_AMD_EXPORT int32_t tfhe_amd_gates_device_noalloc(const TFheGateBootstrappingCloudKeySet* ck, int32_t count,
+                                                      const int32_t* d_op, const int32_t* d_in0, const int32_t* d_in1,
+                                                      const int32_t* d_in2, const int32_t* d_out, int32_t* d_pool,
+                                                      void* stream);
 /* Same, bootstrapping only (no keyswitch): writes extracted samples int32
  * [count][k*N+1] to d_ext. For MUX gates both halves are summed with +1/8. */
 TFHE_AMD_EXPORT int32_t tfhe_amd_gates_device_woks(const TFheGateBootstrappingCloudKeySet* ck, int32_t count,

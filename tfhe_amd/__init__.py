@@ -133,6 +133,7 @@ _SIGS = {
     "tfhe_amd_cloud_key_device_bytes": (ctypes.c_size_t, [CKP]),
     "tfhe_amd_gates_device": (i32, [CKP, i32, vp, vp, vp, vp, vp, vp, vp]),
     "tfhe_amd_gates_device_woks": (i32, [CKP, i32, vp, vp, vp, vp, vp, vp, vp]),
+    "tfhe_amd_gates_device_noalloc": (i32, [CKP, i32, vp, vp, vp, vp, vp, vp, vp]),
     "tfhe_amd_keyswitch_device": (i32, [CKP, i32, vp, vp, vp]),
     "export_tfheGateBootstrappingParameterSet_toFile": (None, [vp, PP]),
     "new_tfheGateBootstrappingParameterSet_fromFile": (PP, [vp]),

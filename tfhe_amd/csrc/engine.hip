@@ -1598,6 +1598,14 @@ __global__ __launch_bounds__(256) void ks_out_init_kernel(const int32_t* __restr
 
 // ---------------------------------------------------------------- runtime --
 
+// Workspace of the entry point that allocates nothing
+// (tfhe_amd_gates_device_noalloc): a static device array, one copy per GPU,
+// held until the process ends. Host code never passes `scratch` itself to a
+// kernel (in host code it is only a stand-in for the symbol): the address of
+// the current device's copy comes from hipGetSymbolAddress at set-up.
+constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)32 << 20;
+static __device__ __align__(256) char scratch[NOALLOC_SCRATCH_BYTES];
+
 namespace {
 
 // A device workspace, the stream that used it last and an event recorded after
@@ -1626,6 +1634,7 @@ struct Ctx {
     int br_mode = 0;
     int brl_max = 256;        // automatic mode: batches up to this many gates use the latency kernel (4l wavefronts per bootstrap)
     int br_small_max = 1024;  // automatic mode: batches up to this many gates use two wavefronts per bootstrap
+    char* noalloc_scr = nullptr;  // this device's copy of `scratch` (hipGetSymbolAddress at set-up)
     // optional per-launch timing with HIP events on the launch stream
     bool prof = false;
     struct Ev { hipEvent_t e[3]; };
@@ -1644,6 +1653,9 @@ int ensure_ready(Ctx& C) {
     if (ndev < 1) return set_error("no GPU device visible");
     HIP_TRY(hipGetDevice(&C.device));
     HIP_TRY(hipStreamCreateWithFlags(&C.stream, hipStreamNonBlocking));
+    // the device address of this GPU's copy of the static workspace (the device is current)
+    HIP_TRY(hipGetSymbolAddress((void**)&C.noalloc_scr, HIP_SYMBOL(scratch)));
+    if (!C.noalloc_scr) return set_error("hipGetSymbolAddress(scratch) returned a null address");
     C.ready = true;
     return 0;
 }
@@ -2420,6 +2432,35 @@ TFHE_AMD_EXPORT int32_t tfhe_amd_gates_device(const TFheGateBootstrappingCloudKe
                                      (char*)ws->d, s))
         return rc;
     return scratch_done(ws, s);
+}
+
+TFHE_AMD_EXPORT int32_t tfhe_amd_gates_device_noalloc(const TFheGateBootstrappingCloudKeySet* ck, int32_t count,
+                                                      const int32_t* d_op, const int32_t* d_in0, const int32_t* d_in1,
+                                                      const int32_t* d_in2, const int32_t* d_out, int32_t* d_pool,
+                                                      void* stream) {
+    if (!ck || !ck->bkFFT) return set_error("gates_device_noalloc: null cloud key");
+    if (count < 0) return set_error("gates_device_noalloc: negative gate count");
+    Ctx& C = ctx();
+    std::lock_guard<std::mutex> g(C.mu);
+    // an upload allocates: the key must already be on the device (read under the
+    // lock a release takes)
+    tfhe_amd_device_key* dk = ck->bkFFT->dkey;
+    if (!C.ready || !dk)
+        return set_error("gates_device_noalloc: the cloud key is not on the device (call tfhe_amd_cloud_key_upload first)");
+    if (count == 0) return 0;
+    // the keyswitch path is fixed for this call, so the scratch it needs is known
+    const int path = ks_path_now(dk);
+    const size_t need = ext_bytes(dk->P, count) + ks_scratch_bytes_path(dk->P, count, path == 0);
+    if (need > NOALLOC_SCRATCH_BYTES) {
+        int most = count;
+        while (most > 0 && ext_bytes(dk->P, most) + ks_scratch_bytes_path(dk->P, most, path == 0) > NOALLOC_SCRATCH_BYTES)
+            most = most * 7 / 8;
+        return set_error("gates_device_noalloc: %d gates need %zu bytes of workspace, above the static %zu bytes "
+                         "(about %d gates fit with this key)", count, need, NOALLOC_SCRATCH_BYTES, most);
+    }
+    const hipStream_t s = stream ? (hipStream_t)stream : C.stream;
+    return gates_device_locked(C, dk, count, d_op, d_in0, d_in1, d_in2, d_out, d_pool, nullptr, 1 << 29, C.noalloc_scr,
+                               s, false, path);
 }
 
 TFHE_AMD_EXPORT int32_t tfhe_amd_gates_device_woks(const TFheGateBootstrappingCloudKeySet* ck, int32_t count,
