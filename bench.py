@@ -69,8 +69,8 @@ def parse():
     ap.add_argument("--crossovers", default="",
                     help="LAT,SMALL: use the latency kernel up to LAT gates and the two-wavefront kernel up to "
                          "SMALL instead of autotuning")
-    ap.add_argument("--compare-modes", action="store_true",
-                    help="also time the one-wavefront-per-bootstrap kernel on the same batch")
+    ap.add_argument("--no-compare-modes", action="store_true",
+                    help="skip timing the batched kernel on the radix-8 transform beside the default one")
     ap.add_argument("--copy-calib-mb", type=int, default=0,
                     help="also run one plain copy kernel of this many MiB (HBM counter calibration)")
     return ap.parse_args()
@@ -558,20 +558,21 @@ def main():
           flush=True)
 
     mode_cmp = None
-    if args.compare_modes:
-        mode_cmp = {}
-        for mode in (1, 2):
-            T.lib().tfhe_amd_set_launch_config(mode, -1, -1)
-            step()
-            T.synchronize()
-            t1 = time.perf_counter()
-            for _ in range(args.steps):
+    if not args.no_compare_modes:
+        # the batched kernel on both transforms, alternating, same batch (gates/s)
+        mode_cmp = {"radix8_transform": [], "fused_butterfly_transform": []}
+        for _ in range(2):
+            for mode, name in ((7, "radix8_transform"), (2, "fused_butterfly_transform")):
+                T.lib().tfhe_amd_set_launch_config(mode, -1, -1)
                 step()
-            T.synchronize()
-            mode_cmp[{1: "one_wave_per_bootstrap", 2: "one_per_wavefront_batched"}[mode]] = round(
-                B * args.steps / (time.perf_counter() - t1), 2)
+                T.synchronize()
+                t1 = time.perf_counter()
+                for _ in range(args.steps):
+                    step()
+                T.synchronize()
+                mode_cmp[name].append(round(B * args.steps / (time.perf_counter() - t1), 1))
         T.lib().tfhe_amd_set_launch_config(0, -1, -1)
-        print(f"[bench] modes {mode_cmp}", file=sys.stderr, flush=True)
+        print(f"[bench] batched kernel, both transforms: {mode_cmp}", file=sys.stderr, flush=True)
 
     # one gate per launch (same kernels, count = 1, synchronised each time)
     lat = []

@@ -408,10 +408,12 @@ TFHE_AMD_EXPORT void tfhe_amd_server_destroy(tfhe_amd_server* srv);
 
 /* Blind-rotation kernel choice. mode: 0 = automatic, 1 = one wavefront per
  * bootstrap, 2 = one wavefront per bootstrap with the key and twiddles read
- * through the vector memory path (the batched kernel), 3 = two
- * wavefronts per bootstrap, 5 = 4l wavefronts per bootstrap (lowest latency;
- * modes 2, 3, 5 for N = 1024), 4 = the N = 2048 pair-transform kernel (the
- * automatic choice for N = 2048). In automatic mode, for N = 1024, batches of
+ * through the vector memory path (the batched kernel, on the fused-butterfly
+ * transform), 7 = the same kernel on the radix-8 transform (kept for
+ * comparison), 3 = two wavefronts per bootstrap, 5 = 4l wavefronts per
+ * bootstrap (lowest latency; modes 2, 3, 5, 7 for N = 1024), 4 = the N = 2048
+ * pair-transform kernel (the automatic choice for N = 2048). All modes give
+ * bit-identical results. In automatic mode, for N = 1024, batches of
  * at most latency_max gates use mode 5, then up to small_batch_max mode 3,
  * larger ones mode 2 (a negative value keeps the current one). */
 TFHE_AMD_EXPORT void tfhe_amd_set_launch_config(int32_t br_mode, int32_t latency_max, int32_t small_batch_max);

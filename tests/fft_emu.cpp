@@ -160,6 +160,56 @@ struct EmuHalf {
     }
 };
 
+// N = 1024 with fused butterflies (FftCT): X[lane][slot]
+struct EmuCT {
+    using F = FftCT;
+    std::vector<c64> tw;
+    EmuCT() : tw(FftCT::COUNT) { build_twiddles_ct(tw.data()); }
+    // the per-lane table entries a lane reads: k-th entry of table `off`
+    void lane_tab(int off, int lane, ct64* w, int n) const {
+        for (int k = 0; k < n; k++) w[k] = ct64{tw[off + 64 * k + lane].re, tw[off + 64 * k + lane].im};
+    }
+    void forward(const double* p, c64 (*x)[8]) {
+        std::vector<c64> lds(F::LDS_C64);
+        for (int b = 0; b < 64; b++)
+            for (int a = 0; a < 8; a++) x[b][a] = c64{p[64 * a + b], p[64 * a + b + 512]};
+        for (int l = 0; l < 64; l++) F::fwd1(x[l], l, lds.data());
+        for (int l = 0; l < 64; l++) {
+            ct64 w[4];
+            lane_tab(FftCT::FB, l, w, 4);
+            F::fwd2(x[l], l, lds.data(), w);
+        }
+        for (int l = 0; l < 64; l++) F::fwd3(x[l], l, lds.data());
+        for (int l = 0; l < 64; l++) {
+            ct64 w[4];
+            lane_tab(FftCT::FC, l, w, 4);
+            F::fwd4(x[l], l, lds.data(), w);
+        }
+    }
+    void inverse(c64 (*x)[8], double* p) {
+        std::vector<c64> lds(F::LDS_C64);
+        for (int l = 0; l < 64; l++) F::inv1(x[l], l, lds.data());
+        for (int l = 0; l < 64; l++) {
+            ct64 w[4];
+            lane_tab(FftCT::IB, l, w, 4);
+            F::inv2(x[l], l, lds.data(), w);
+        }
+        for (int l = 0; l < 64; l++) F::inv3(x[l], l, lds.data());
+        for (int l = 0; l < 64; l++) {
+            ct64 w[4];
+            c64 ut[8];
+            lane_tab(FftCT::IA, l, w, 4);
+            for (int a = 0; a < 8; a++) ut[a] = tw[FftCT::UT + 64 * a + l];
+            F::inv4(x[l], l, lds.data(), w, ut);
+        }
+        for (int b = 0; b < 64; b++)
+            for (int a = 0; a < 8; a++) {
+                p[64 * a + b] = x[b][a].re;
+                p[64 * a + b + 512] = x[b][a].im;
+            }
+    }
+};
+
 template <class Em, int M, int SLOTS>
 int run_with(Em& E, int trials, unsigned seed, int dbits);
 
@@ -232,6 +282,10 @@ int main(int argc, char** argv) {
     if (N == -2048) {  // pair-of-wavefronts transform
         EmuPair E;
         return run_with<EmuPair, 1024, 8>(E, trials, seed, dbits);
+    }
+    if (N == 1) {  // N = 1024, fused butterflies
+        EmuCT E;
+        return run_with<EmuCT, 512, 8>(E, trials, seed, dbits);
     }
     if (N == -1024) {  // pair of wavefronts, 256 points each
         EmuHalf E;

@@ -47,7 +47,7 @@ def test_empty_batches(key):
     assert L.tfhe_amd_gates_device(sk.cloud, -1, None, None, None, None, None, None, None) != 0
 
 
-@pytest.mark.parametrize("mode", [1, 2, 3, 5])
+@pytest.mark.parametrize("mode", [1, 2, 3, 5, 7])
 def test_mux_only_batch_every_kernel(key, mode):
     T, P, sk, keys = key
     rng = np.random.default_rng(70 + mode)

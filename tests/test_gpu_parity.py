@@ -289,10 +289,11 @@ def test_c_program_gpu(tmp_path):
     assert r.returncode == 0, r.stdout + r.stderr
 
 
-@pytest.mark.parametrize("mode", [1, 2, 3, 5])
+@pytest.mark.parametrize("mode", [1, 2, 3, 5, 7])
 def test_all_blind_rotation_kernels_bitexact(p128, mode):
     """The blind-rotation kernels (1: one wavefront per bootstrap, 2: one
-    bootstrap per one-wavefront workgroup, 3: two wavefronts per bootstrap, 5: 4l
+    bootstrap per one-wavefront workgroup on the fused-butterfly transform, 7: the
+    same on the radix-8 transform, 3: two wavefronts per bootstrap, 5: 4l
     wavefronts per bootstrap on half transforms) give the oracle's ciphertexts
     bit-exactly, MUX included, for odd batch sizes."""
     T, P, sk, keys = p128

@@ -137,7 +137,7 @@ def p80(gpu):
     sk.close()
 
 
-@pytest.mark.parametrize("mode", [2, 3, 5])
+@pytest.mark.parametrize("mode", [2, 3, 5, 7])
 def test_80bit_preset_every_kernel(p80, mode):
     T, P, sk, keys = p80
     assert (P.n, P.N, P.l, P.Bgbit) == (500, 1024, 2, 10)

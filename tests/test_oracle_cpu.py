@@ -250,10 +250,11 @@ def fft_emu(tmp_path_factory):
     return exe
 
 
-@pytest.mark.parametrize("N,bits", [(1024, 7), (2048, 10), (-2048, 10), (-1024, 7)])
+@pytest.mark.parametrize("N,bits", [(1024, 7), (2048, 10), (-2048, 10), (-1024, 7), (1, 7), (1, 10)])
 def test_wavefront_transform_emulation(fft_emu, N, bits):
     """The GPU transform's per-lane code, run lane by lane on the CPU, gives the
-    exact negacyclic product after rounding (digit x torus polynomials)."""
+    exact negacyclic product after rounding (digit x torus polynomials). N = 1 is
+    the N = 1024 transform with fused butterflies (FftCT)."""
     out = subprocess.run([fft_emu, str(N), "6", "7", str(bits)], capture_output=True, text=True, check=True).stdout
     f = out.split()
     assert f[0] == "maxerr" and float(f[1]) < 0.1

@@ -59,6 +59,8 @@ struct tfhe_amd_device_key {
     c64* bkh;      // M = 512 only: [n][kpl][k+1][2][256], the half transform (FftHalf) for the latency kernel
     c64* twh;      // M = 512 only: HALF_TW_COUNT
     int8_t* kp;    // t = 8, basebit = 2: keyswitch key as signed byte planes [4][k*N][npad][32] (int8 matrix cores)
+    c64* bkc;      // M = 512 only: [n][kpl][k+1][M] in the fused-butterfly transform's order (FftCT)
+    c64* twc;      // M = 512 only: FftCT::COUNT per-lane tables
     size_t bytes;
     uint64_t gen;  // unique per upload: a re-uploaded key never compares equal to a freed one
     int pins;      // holders besides the host key (servers); the memory is freed when the last one lets go
@@ -91,6 +93,7 @@ struct BrArgs {
     int32_t* ext;            // extracted samples [2*count][kN+1]
     const c64* bkf;          // key rows, already offset to the first row used
     const c64* tw;
+    const c64* twc;  // FftCT tables (kernels on the fused-butterfly transform)
     int32_t n;
     int32_t Bgbit;
     uint32_t offset;  // gadget offset, libtfhe/tgsw.cpp:21-27
@@ -103,6 +106,14 @@ struct BrArgs {
 __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
+}
+
+// The same pointer, opaque to the optimiser (an empty asm that "modifies" it):
+// loads through it are not hoisted out of loops.
+template <class T>
+__device__ __forceinline__ const T* opaque(const T* p) {
+    asm volatile("" : "+s"(p));
+    return p;
 }
 
 // round to nearest, result mod 2^32 (valid for |v| < 2^51)
@@ -332,12 +343,14 @@ struct Br4Lds {
     static constexpr int BUF_C64 = (Fft<512>::LDS_C64 > 3 * 1024 / 4) ? Fft<512>::LDS_C64 : 3 * 1024 / 4;
 };
 
-template <int L>
+// CT: the fused-butterfly transform (FftCT, key bkc, tables twc) instead of Fft<512>.
+template <int L, bool CT = false>
 __device__ __forceinline__ void br4_body(BrArgs A, const int32_t* __restrict__ mux_gate,
                                          const int32_t* __restrict__ n_jobs, c64* __restrict__ fb,
                                          int16_t* __restrict__ bara) {
     constexpr int M = 512;
     using F = Fft<M>;
+    using G = FftCT;
     constexpr int R = F::R, N = 2 * M, KPL = 2 * L;
     constexpr int LOG2N2 = 11;
 
@@ -384,7 +397,21 @@ __device__ __forceinline__ void br4_body(BrArgs A, const int32_t* __restrict__ m
     // (8 KiB, cached) through the vector memory path; the stage-2a twiddles are powers of one
     // per-lane value
     const c64* tw1 = A.tw + TwiddleTable<M>::TW1_OFF + lane;  // tw1[k1 * 64]
-    const c64 tw2 = A.tw[lane * (R + 8) + R + 1];
+    const c64 tw2 = CT ? c64{0.0, 0.0} : A.tw[lane * (R + 8) + R + 1];
+    // FftCT: this lane's column of every table, entry k at [k * 64]; the base
+    // pointer is passed through an empty asm at every use so the compiler
+    // reloads the (cached) tables where they are needed instead of hoisting
+    // ~96 registers of them out of the row loop
+    const ct64* twc_base = (const ct64*)A.twc;
+    // the forward tables stay in registers for the whole kernel (32 VGPRs)
+    ct64 fwb[4], fwc[4];
+    if constexpr (CT) {
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            fwb[k] = twc_base[G::FB + 64 * k + lane];
+            fwc[k] = twc_base[G::FC + 64 * k + lane];
+        }
+    }
     const uint32_t offset = A.offset;
     const int Bgbit = A.Bgbit;
     // the top bit of every digit field: with it flipped, the signed field is the balanced digit
@@ -432,20 +459,32 @@ __device__ __forceinline__ void br4_body(BrArgs A, const int32_t* __restrict__ m
                 x[s] = c64{(double)d0, (double)d1};
             }
             wave_sync();  // rotated reads (p = 0) or the previous transform done with the buffer
-            F::template fwd1<64>(x, lane, fb, tw1);
-            wave_sync();
-            F::fwd2(x, lane, fb, tw2);
-            wave_sync();
-            F::fwd3(x, lane, fb);
-            // key digit polynomial (i, r): component 0 at q[s * 64], component 1 at q[M + s * 64];
-            // component 0 requested before the last stage, component 1 as its products free
-            // the registers
             const c64* q = A.bkf + ((size_t)i * KPL + r) * 2 * M + lane;
             c64 kv[R];
+            if constexpr (CT) {
+                G::fwd1(x, lane, fb);
+                wave_sync();
+                G::fwd2(x, lane, fb, fwb);
+                wave_sync();
+                G::fwd3(x, lane, fb);
+                // key digit polynomial (i, r): component 0 at q[s * 64], component 1 at
+                // q[M + s * 64]; component 0 requested before the last stage, component 1
+                // as its products free the registers
 #pragma unroll
-            for (int s = 0; s < R; s++) kv[s] = q[s * 64];
-            wave_sync();
-            F::fwd4(x, lane, fb);
+                for (int s = 0; s < R; s++) kv[s] = q[s * 64];
+                wave_sync();
+                G::fwd4(x, lane, fb, fwc);
+            } else {
+                F::template fwd1<64>(x, lane, fb, tw1);
+                wave_sync();
+                F::fwd2(x, lane, fb, tw2);
+                wave_sync();
+                F::fwd3(x, lane, fb);
+#pragma unroll
+                for (int s = 0; s < R; s++) kv[s] = q[s * 64];
+                wave_sync();
+                F::fwd4(x, lane, fb);
+            }
 #pragma unroll
             for (int s = 0; s < R; s++) {
                 cmac(r0[s], x[s], kv[s]);
@@ -454,18 +493,46 @@ __device__ __forceinline__ void br4_body(BrArgs A, const int32_t* __restrict__ m
 #pragma unroll
             for (int s = 0; s < R; s++) cmac(r1[s], x[s], kv[s]);
         }
-        // inverse transforms and ACC += result (lwe-bootstrapping-functions-fft.cpp:67-76)
+        // inverse transforms and ACC += result (lwe-bootstrapping-functions-fft.cpp:67-76);
+        // FftCT: the inverse tables, loaded once per key row for both components
+        // (through an opaque pointer, so not hoisted out of the row loop)
+        ct64 iwb[4], iwa[4];
+        c64 iut[R];
+        if constexpr (CT) {
+            const ct64* twc = opaque(twc_base) + lane;
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                iwb[k] = twc[G::IB + 64 * k];
+                iwa[k] = twc[G::IA + 64 * k];
+            }
+#pragma unroll
+            for (int k = 0; k < R; k++) iut[k] = ((const c64*)twc)[G::UT + 64 * k];
+        }
 #pragma unroll
         for (int cc = 0; cc < 2; cc++) {
             c64* x = cc ? r1 : r0;
-            wave_sync();
-            F::inv1(x, lane, fb);
-            wave_sync();
-            F::inv2(x, lane, fb, tw2);
-            wave_sync();
-            F::inv3(x, lane, fb);
-            wave_sync();
-            F::template inv4<64>(x, lane, fb, tw1);
+            if constexpr (CT) {
+                const ct64* wb = iwb;
+                const ct64* wa = iwa;
+                const c64* ut = iut;
+                wave_sync();
+                G::inv1(x, lane, fb);
+                wave_sync();
+                G::inv2(x, lane, fb, wb);
+                wave_sync();
+                G::inv3(x, lane, fb);
+                wave_sync();
+                G::inv4(x, lane, fb, wa, ut);
+            } else {
+                wave_sync();
+                F::inv1(x, lane, fb);
+                wave_sync();
+                F::inv2(x, lane, fb, tw2);
+                wave_sync();
+                F::inv3(x, lane, fb);
+                wave_sync();
+                F::template inv4<64>(x, lane, fb, tw1);
+            }
 #pragma unroll
             for (int s = 0; s < R; s++) {
                 acc[cc][s] += (uint32_t)round_mod32(x[s].re);
@@ -486,12 +553,12 @@ __device__ __forceinline__ void br4_body(BrArgs A, const int32_t* __restrict__ m
     if (lane == 0) e[N] = (int32_t)acc[1][0];
 }
 
-template <int L>
+template <int L, bool CT = false>
 __global__ __launch_bounds__(64, 8) void br4_kernel(BrArgs A, const int32_t* __restrict__ mux_gate,
                                                    const int32_t* __restrict__ n_jobs) {
     __shared__ c64 fb[Br4Lds::BUF_C64];
     __shared__ int16_t bara[1024];
-    br4_body<L>(A, mux_gate, n_jobs, fb, bara);
+    br4_body<L, CT>(A, mux_gate, n_jobs, fb, bara);
 }
 
 // One launch over several clients' batches (segments), each with its own
@@ -513,6 +580,8 @@ struct BrSegTable {
 
 template <int L>
 __global__ __launch_bounds__(64, 8) void br4_seg_kernel(BrSegTable T) {
+    // segments run the batched kernel's body on the fused-butterfly transform
+    // (their keys' bkc and the tables in T.base.twc)
     const BrSeg& g = T.seg[blockIdx.y];
     BrArgs A = T.base;
     A.bkf = g.bkf;
@@ -524,7 +593,7 @@ __global__ __launch_bounds__(64, 8) void br4_seg_kernel(BrSegTable T) {
     A.in2 += g.off;
     __shared__ c64 fb[Br4Lds::BUF_C64];
     __shared__ int16_t bara[1024];
-    br4_body<L>(A, g.mux_gate, g.n_jobs, fb, bara);
+    br4_body<L, true>(A, g.mux_gate, g.n_jobs, fb, bara);
 }
 
 // largest LWE dimension the latency kernel's LDS table holds
@@ -1364,6 +1433,36 @@ __global__ __launch_bounds__(64) void bk_convert_kernel(const int32_t* __restric
     for (int s = 0; s < R; s++) bkf[poly * M + s * 64 + lane] = x[s];
 }
 
+// Bootstrapping-key polynomials into the fused-butterfly transform's order
+// (N = 1024): one wavefront per polynomial, output [poly][slot][lane].
+__global__ __launch_bounds__(64) void bk_convert_ct_kernel(const int32_t* __restrict__ bk, c64* __restrict__ bkc,
+                                                           const c64* __restrict__ twc) {
+    using G = FftCT;
+    __shared__ c64 s_fft[G::LDS_C64];
+    const int lane = threadIdx.x;
+    const size_t poly = blockIdx.x;
+    const int32_t* p = bk + poly * 1024;
+    const ct64* tw = (const ct64*)twc + lane;
+    c64 x[8];
+    ct64 wb[4], wc[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        wb[k] = tw[G::FB + 64 * k];
+        wc[k] = tw[G::FC + 64 * k];
+    }
+#pragma unroll
+    for (int s = 0; s < 8; s++) x[s] = c64{(double)p[64 * s + lane], (double)p[64 * s + lane + 512]};
+    G::fwd1(x, lane, s_fft);
+    __syncthreads();
+    G::fwd2(x, lane, s_fft, wb);
+    __syncthreads();
+    G::fwd3(x, lane, s_fft);
+    __syncthreads();
+    G::fwd4(x, lane, s_fft, wc);
+#pragma unroll
+    for (int s = 0; s < 8; s++) bkc[poly * 512 + s * 64 + lane] = x[s];
+}
+
 // ext: [2*count][nin+1]; op may be null (no MUX). Writes abar[i][cpad] = a_i + prec_offset
 // (masks transposed so the keyswitch reads one word per gate per i) and binit[g] = b.
 __global__ __launch_bounds__(256) void ks_prep_kernel(const int32_t* __restrict__ ext,
@@ -1913,6 +2012,7 @@ int gates_device_locked(Ctx& C, tfhe_amd_device_key* dk, int count, const int32_
     }
     if (mode == 5 && (dk->M != 512 || dk->P.n > BRL_NMAX)) mode = 3;
     if (mode == 5 && !dk->bkh) mode = 3;
+    if (mode == 2 && !dk->bkc) mode = 7;
     if (mode == 5) {
         const int blocks = 2 * count;  // one bootstrap per workgroup, at most 2*count jobs
         if (dk->P.l == 3) hipLaunchKernelGGL(brl_kernel<3>, dim3(blocks), dim3(768), 0, s, A, dk->bkh, dk->twh, mux_gate, n_jobs);
@@ -1923,6 +2023,12 @@ int gates_device_locked(Ctx& C, tfhe_amd_device_key* dk, int count, const int32_
         else hipLaunchKernelGGL(brp_kernel<3>, dim3(blocks), dim3(256), 0, s, A, dk->bkp, dk->twp, mux_gate, n_jobs);
     } else if (mode == 2) {
         const int blocks = 2 * count;  // one bootstrap per workgroup, at most 2*count jobs
+        A.bkf = dk->bkc;               // the fused-butterfly transform's key and tables
+        A.twc = dk->twc;
+        if (dk->P.l == 3) hipLaunchKernelGGL((br4_kernel<3, true>), dim3(blocks), dim3(64), 0, s, A, mux_gate, n_jobs);
+        else hipLaunchKernelGGL((br4_kernel<2, true>), dim3(blocks), dim3(64), 0, s, A, mux_gate, n_jobs);
+    } else if (mode == 7) {
+        const int blocks = 2 * count;  // the same kernel on the radix-8 transform Fft<512> (for comparison)
         if (dk->P.l == 3) hipLaunchKernelGGL(br4_kernel<3>, dim3(blocks), dim3(64), 0, s, A, mux_gate, n_jobs);
         else hipLaunchKernelGGL(br4_kernel<2>, dim3(blocks), dim3(64), 0, s, A, mux_gate, n_jobs);
     } else if (mode == 3) {
@@ -1971,7 +2077,7 @@ int engine_launch_batch_locked(tfhe_amd_device_key* dk, int count, const int32_t
 }
 
 bool engine_can_segment(const tfhe_amd_device_key* dk) {
-    return dk && dk->M == 512 && dk->bkf && (dk->P.l == 2 || dk->P.l == 3);
+    return dk && dk->M == 512 && dk->bkc && dk->twc && (dk->P.l == 2 || dk->P.l == 3);
 }
 
 size_t engine_segments_scratch_bytes(tfhe_amd_device_key* const* dks, const int32_t* counts, int nseg) {
@@ -2002,6 +2108,7 @@ int engine_launch_segments_locked(tfhe_amd_device_key* const* dks, const int32_t
     BrSegTable T;
     memset(&T, 0, sizeof T);
     T.base = base_args(dks[0]);
+    T.base.twc = dks[0]->twc;  // the tables are the same for every key
     T.base.op = d_idx;
     T.base.in0 = d_idx + total;
     T.base.in1 = d_idx + 2 * (size_t)total;
@@ -2018,7 +2125,7 @@ int engine_launch_segments_locked(tfhe_amd_device_key* const* dks, const int32_t
         int32_t* n_jobs = (int32_t*)(jp + 2 * align256(sizeof(int32_t) * (size_t)c));
         p += ext_bytes(P, c);
         hipLaunchKernelGGL(mux_scan_kernel, dim3(1), dim3(1024), 0, s, d_idx + off, c, mux_gate, second[g], n_jobs);
-        T.seg[g] = BrSeg{dks[g]->bkf, ext, mux_gate, n_jobs, off, c};
+        T.seg[g] = BrSeg{dks[g]->bkc, ext, mux_gate, n_jobs, off, c};
     }
     // at most 2*count jobs per segment; workgroups past a segment's job count leave at once
     if (P.l == 3) hipLaunchKernelGGL(br4_seg_kernel<3>, dim3(2 * maxc, nseg), dim3(64), 0, s, T);
@@ -2071,6 +2178,8 @@ static int upload_key(Ctx& C, const LweBootstrappingKeyFFT* bf, tfhe_amd_device_
     dk->bkh = nullptr;
     dk->twh = nullptr;
     dk->kp = nullptr;
+    dk->bkc = nullptr;
+    dk->twc = nullptr;
     auto fail = [&](int rc) {
         (void)hipFree(dk->bkf);
         (void)hipFree(dk->ksk);
@@ -2080,6 +2189,8 @@ static int upload_key(Ctx& C, const LweBootstrappingKeyFFT* bf, tfhe_amd_device_
         (void)hipFree(dk->bkh);
         (void)hipFree(dk->twh);
         (void)hipFree(dk->kp);
+        (void)hipFree(dk->bkc);
+        (void)hipFree(dk->twc);
         delete dk;
         return rc;
     };
@@ -2098,6 +2209,11 @@ static int upload_key(Ctx& C, const LweBootstrappingKeyFFT* bf, tfhe_amd_device_
         if (hipMalloc(&dk->bkh, bkf_bytes) != hipSuccess || hipMalloc(&dk->twh, HALF_TW_COUNT * sizeof(c64)) != hipSuccess)
             return fail(set_error("hipMalloc(half-transform key, %zu bytes) failed", bkf_bytes));
         dk->bytes += bkf_bytes + HALF_TW_COUNT * sizeof(c64);
+    }
+    if (M == 512 && k == 1) {
+        if (hipMalloc(&dk->bkc, bkf_bytes) != hipSuccess || hipMalloc(&dk->twc, FftCT::COUNT * sizeof(c64)) != hipSuccess)
+            return fail(set_error("hipMalloc(fused-transform key, %zu bytes) failed", bkf_bytes));
+        dk->bytes += bkf_bytes + FftCT::COUNT * sizeof(c64);
     }
     if (hipMalloc(&dk->bkf, bkf_bytes) != hipSuccess) return fail(set_error("hipMalloc(bootstrapping key, %zu bytes) failed", bkf_bytes));
     if (hipMalloc(&dk->ksk, ksk_bytes) != hipSuccess) return fail(set_error("hipMalloc(keyswitch key, %zu bytes) failed", ksk_bytes));
@@ -2134,6 +2250,12 @@ static int upload_key(Ctx& C, const LweBootstrappingKeyFFT* bf, tfhe_amd_device_
         build_twiddles_pair(twp.data());
         if (e == hipSuccess) e = hipMemcpyAsync(dk->twp, twp.data(), PAIR_TW_COUNT * sizeof(c64), hipMemcpyHostToDevice, C.stream);
     }
+    std::vector<c64> twct;
+    if (dk->twc) {
+        twct.resize(FftCT::COUNT);
+        build_twiddles_ct(twct.data());
+        if (e == hipSuccess) e = hipMemcpyAsync(dk->twc, twct.data(), FftCT::COUNT * sizeof(c64), hipMemcpyHostToDevice, C.stream);
+    }
     std::vector<c64> twh;
     if (dk->twh) {
         twh.resize(HALF_TW_COUNT);
@@ -2146,6 +2268,10 @@ static int upload_key(Ctx& C, const LweBootstrappingKeyFFT* bf, tfhe_amd_device_
         e = hipGetLastError();
         if (e == hipSuccess && dk->bkp) {
             hipLaunchKernelGGL(bk_convert_pair_kernel, dim3((unsigned)npoly), dim3(128), 0, C.stream, d_coef, dk->bkp, dk->twp);
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess && dk->bkc) {
+            hipLaunchKernelGGL(bk_convert_ct_kernel, dim3((unsigned)npoly), dim3(64), 0, C.stream, d_coef, dk->bkc, dk->twc);
             e = hipGetLastError();
         }
         if (e == hipSuccess && dk->bkh) {
@@ -2202,6 +2328,8 @@ static void free_key(Ctx& C, tfhe_amd_device_key* dk) {
     (void)hipFree(dk->bkh);
     (void)hipFree(dk->twh);
     (void)hipFree(dk->kp);
+    (void)hipFree(dk->bkc);
+    (void)hipFree(dk->twc);
     delete dk;
 }
 

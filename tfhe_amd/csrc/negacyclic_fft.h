@@ -529,6 +529,244 @@ struct FftHalf {
     }
 };
 
+// Negacyclic transform for N = 1024 (M = 512) with fused butterflies, for the
+// batched kernel. Same folding z_j = p_j + i p_{j+512} and same wavefront
+// layouts and LDS transposes as Fft<512>, but the arithmetic is a negacyclic
+// Cooley-Tukey split of U(X) = sum_j u_j X^j modulo X^512 - i: a block held
+// modulo X^{2m} - rho splits into X^m - r and X^m + r (r^2 = rho) with the
+// butterfly (x_j, x_{j+m}) -> (x_j + r x_{j+m}, x_j - r x_{j+m}). The twist by
+// psi^j is part of the roots, so there is no separate twist, and every
+// butterfly has its twiddle on an input, which fuses: with r = c (1 + i t),
+//   v = (y.re - t y.im, y.im + t y.re)      (2 FMA)
+//   x +- r y = (x.re +- c v.re, x.im +- c v.im)   (4 FMA)
+// six FMAs per butterfly instead of a complex product and four additions. The
+// variants i r, -i r and -r use the same (c, t) with the parts of v swapped or
+// negated. Starting from rho = i = psi^512 (psi = exp(i pi/1024)), a block whose
+// root is psi^e has children psi^{e/2} and psi^{e/2 + 1024}; after nine levels
+// position P (9 bits, most significant = first split) holds U(psi^{1 + 4t})
+// with t = bitrev9(P), i.e. the cyclic spectrum of u_j psi^j in bit-reversed
+// order. The inverse is therefore a decimation-in-time cyclic inverse DFT
+// (again twiddles on inputs, fused) from bit-reversed input to natural order,
+// followed by the untwist psi^{-j} / 512.
+//   forward  stage A (registers, slot a = position bits 8..6): levels 1-3,
+//            compile-time roots; T1 (LDS): lane b0 + 8a, slot b1;
+//            stage B: levels 4-6, roots of lane's a (table FB);
+//            T2 (LDS): lane q + 8a, slot b0; stage C: levels 7-9 (table FC).
+//            Register (lane q + 8a, slot p) holds position P = 64a + 8q + p.
+//   inverse  stage C' (spans 1, 2, 4 over the slot, compile-time twiddles);
+//            T2 backwards: lane p + 8a, slot q; stage B' (spans 8, 16, 32,
+//            table IB, p = lane & 7); T1 backwards: lane b, slot a; stage A'
+//            (spans 64, 128, 256, table IA, b = lane); untwist (table UT).
+// Each function either computes and writes the LDS buffer or reads it and
+// computes, so the same code runs on the GPU and in the CPU emulation.
+struct __attribute__((aligned(16))) ct64 {
+    double c, t;  // r = c (1 + i t)
+};
+
+// (x, y) <- (x + w y, x - w y) with w = r (V = 0), i r (1), -i r (2), -r (3)
+template <int V>
+TA_HD void ctb(c64& x, c64& y, double c, double t) {
+    const double vr = fma(-t, y.im, y.re), vi = fma(t, y.re, y.im);
+    double pr, pi;  // w y = c (pr, pi)
+    if (V == 0) { pr = vr; pi = vi; }
+    else if (V == 1) { pr = -vi; pi = vr; }
+    else if (V == 2) { pr = vi; pi = -vr; }
+    else { pr = -vr; pi = -vi; }
+    const c64 a = x;
+    x = c64{fma(c, pr, a.re), fma(c, pi, a.im)};
+    y = c64{fma(-c, pr, a.re), fma(-c, pi, a.im)};
+}
+template <int V>
+TA_HD void ctb(c64& x, c64& y, ct64 w) { ctb<V>(x, y, w.c, w.t); }
+// (x, y) <- (x + y, x - y)
+TA_HD void bf1(c64& x, c64& y) {
+    const c64 a = x;
+    x = cadd(a, y);
+    y = csub(a, y);
+}
+// (x, y) <- (x - i y, x + i y)
+TA_HD void bfmi(c64& x, c64& y) {
+    const c64 a = x;
+    x = c64{a.re + y.im, a.im - y.re};
+    y = c64{a.re - y.im, a.im + y.re};
+}
+
+struct FftCT {
+    static constexpr int R = 8;
+    static constexpr int LDS_C64 = Fft<512>::LDS_C64;
+    // compile-time roots (cos, tan) of pi/4, pi/8, pi/16, -3 pi/16
+    static constexpr double C4 = 0.70710678118654752440, T4 = 1.0;
+    static constexpr double C8 = 0.92387953251128675613, T8 = 0.41421356237309504880;
+    static constexpr double C16 = 0.98078528040323044913, T16 = 0.19891236737965800691;
+    static constexpr double C316 = 0.83146961230254523708, T316 = -0.66817863791929891999;
+    // per-lane tables (ct64 / c64 [k][lane]), offsets in 16-byte entries
+    static constexpr int FB = 0, FC = 4 * 64, IB = 8 * 64, IA = 12 * 64, UT = 16 * 64;
+    static constexpr int COUNT = 24 * 64;
+
+    TA_HD static int a1(int k1, int b) { return Fft<512>::a1(k1, b); }
+    TA_HD static int a2(int k1, int b0, int k2a) { return Fft<512>::a2(k1, b0, k2a); }
+
+    // levels 1-3 on slots 0..7 with roots psi^256; psi^128, i psi^128;
+    // psi^64, i psi^64, psi^320 = i e^{-3 i pi/16}, psi^832 = -e^{-3 i pi/16}
+    TA_HD static void stageA(c64* x) {
+#pragma unroll
+        for (int a = 0; a < 4; a++) ctb<0>(x[a], x[a + 4], C4, T4);
+        ctb<0>(x[0], x[2], C8, T8);
+        ctb<0>(x[1], x[3], C8, T8);
+        ctb<1>(x[4], x[6], C8, T8);
+        ctb<1>(x[5], x[7], C8, T8);
+        ctb<0>(x[0], x[1], C16, T16);
+        ctb<1>(x[2], x[3], C16, T16);
+        ctb<1>(x[4], x[5], C316, T316);
+        ctb<3>(x[6], x[7], C316, T316);
+    }
+    // levels with table roots w0 (level 1 of the stage), w1 (level 2), w2, w3 (level 3)
+    TA_HD static void stage_tab(c64* x, const ct64* w) {
+#pragma unroll
+        for (int q = 0; q < 4; q++) ctb<0>(x[q], x[q + 4], w[0]);
+        ctb<0>(x[0], x[2], w[1]);
+        ctb<0>(x[1], x[3], w[1]);
+        ctb<1>(x[4], x[6], w[1]);
+        ctb<1>(x[5], x[7], w[1]);
+        ctb<0>(x[0], x[1], w[2]);
+        ctb<1>(x[2], x[3], w[2]);
+        ctb<0>(x[4], x[5], w[3]);
+        ctb<1>(x[6], x[7], w[3]);
+    }
+    // ---- forward: x[a] = p_{64a+b} + i p_{64a+b+512} at lane b ----
+    TA_HD static void fwd1(c64* x, int b, c64* lds) {
+        stageA(x);
+#pragma unroll
+        for (int a = 0; a < R; a++) lds[a1(a, b)] = x[a];
+    }
+    // reads in the order 0, 4, 1, 5, ...: the first butterflies pair q and q + 4
+    TA_HD static void fwd2(c64* x, int lane, const c64* lds, const ct64* wb) {
+        const int b0 = lane & 7, a = lane >> 3;
+#pragma unroll
+        for (int k = 0; k < R; k++) {
+            const int q = (k >> 1) + 4 * (k & 1);
+            x[q] = lds[a1(a, b0 + 8 * q)];
+        }
+        stage_tab(x, wb);
+    }
+    TA_HD static void fwd3(const c64* x, int lane, c64* lds) {
+        const int b0 = lane & 7, a = lane >> 3;
+#pragma unroll
+        for (int q = 0; q < R; q++) lds[a2(a, b0, q)] = x[q];
+    }
+    TA_HD static void fwd4(c64* x, int lane, const c64* lds, const ct64* wc) {
+        const int q = lane & 7, a = lane >> 3;
+#pragma unroll
+        for (int k = 0; k < R; k++) {
+            const int p = (k >> 1) + 4 * (k & 1);
+            x[p] = lds[a2(a, p, q)];
+        }
+        stage_tab(x, wc);
+    }
+    // ---- inverse (times 512 before the untwist, which divides it out) ----
+    // spans 1, 2, 4 of the decimation-in-time inverse DFT: twiddles 1; 1, -i;
+    // 1, e^{-i pi/4}, -i, -i e^{-i pi/4}
+    TA_HD static void stageC_inv(c64* x) {
+#pragma unroll
+        for (int p = 0; p < R; p += 2) bf1(x[p], x[p + 1]);
+        bf1(x[0], x[2]);
+        bfmi(x[1], x[3]);
+        bf1(x[4], x[6]);
+        bfmi(x[5], x[7]);
+        bf1(x[0], x[4]);
+        ctb<0>(x[1], x[5], C4, -T4);
+        bfmi(x[2], x[6]);
+        ctb<2>(x[3], x[7], C4, -T4);
+    }
+    // spans m, 2m, 4m with table twiddles w0 (span m), w1 (2m), w2, w3 (4m: slots 0, 1; 2, 3 use -i w2, -i w3)
+    TA_HD static void stage_inv_tab(c64* x, const ct64* w) {
+#pragma unroll
+        for (int q = 0; q < R; q += 2) ctb<0>(x[q], x[q + 1], w[0]);
+        ctb<0>(x[0], x[2], w[1]);
+        ctb<2>(x[1], x[3], w[1]);
+        ctb<0>(x[4], x[6], w[1]);
+        ctb<2>(x[5], x[7], w[1]);
+        ctb<0>(x[0], x[4], w[2]);
+        ctb<0>(x[1], x[5], w[3]);
+        ctb<2>(x[2], x[6], w[2]);
+        ctb<2>(x[3], x[7], w[3]);
+    }
+    TA_HD static void inv1(c64* x, int lane, c64* lds) {
+        const int q = lane & 7, a = lane >> 3;
+        stageC_inv(x);
+#pragma unroll
+        for (int p = 0; p < R; p++) lds[a2(a, p, q)] = x[p];
+    }
+    TA_HD static void inv2(c64* x, int lane, const c64* lds, const ct64* wb) {
+        const int p = lane & 7, a = lane >> 3;
+#pragma unroll
+        for (int q = 0; q < R; q++) x[q] = lds[a2(a, p, q)];
+        stage_inv_tab(x, wb);
+    }
+    TA_HD static void inv3(const c64* x, int lane, c64* lds) {
+        const int p = lane & 7, a = lane >> 3;
+#pragma unroll
+        for (int q = 0; q < R; q++) lds[a1(a, p + 8 * q)] = x[q];
+    }
+    // result x[a] = p_{64a+b} + i p_{64a+b+512} (ut[a] = psi^{-(64a+b)} / 512)
+    TA_HD static void inv4(c64* x, int b, const c64* lds, const ct64* wa, const c64* ut) {
+#pragma unroll
+        for (int a = 0; a < R; a++) x[a] = lds[a1(a, b)];
+        stage_inv_tab(x, wa);
+#pragma unroll
+        for (int a = 0; a < R; a++) x[a] = cmul(x[a], ut[a]);
+    }
+};
+
+// Per-lane tables of FftCT, [k][lane], built in long double and rounded once:
+// FB (forward levels 4-6, lane b0 + 8a): psi^{E/2}, psi^{E/4}, psi^{E/8}, psi^{E/8+256}
+//    with psi^E the root of block a after three levels;
+// FC (levels 7-9, lane q + 8a): the same with the root of block (a, q) after six;
+// IB (spans 8-32, p = lane & 7): w16^-p, w32^-p, w64^-p, e^{-i pi/4} w64^-p;
+// IA (spans 64-256, b = lane): w128^-b, w256^-b, w512^-b, e^{-i pi/4} w512^-b
+//    (w_L = exp(2 pi i / L)); UT [a][lane]: psi^{-(64a+b)} / 512 as c64.
+inline void build_twiddles_ct(c64* out) {
+    const long double pi = 3.141592653589793238462643383279502884L;
+    // (cos, tan) of ang as two doubles in a c64 slot. At odd multiples of pi/2
+    // (w = +-i, tan infinite) w is represented as 2^-60 (1 + i (+-2^60)): off by
+    // 2^-60 in the real part, far below FP64 rounding, and in the fused
+    // butterfly c v is an exact power-of-two scaling of the rounded v, so the
+    // result keeps full precision.
+    auto ct = [&](long double ang) {
+        const long double c = cosl(ang), s = sinl(ang);
+        if (fabsl(c) < 1e-12L) return c64{0x1p-60, (double)(s > 0 ? 0x1p60L : -0x1p60L)};
+        return c64{(double)c, (double)(s / c)};
+    };
+    auto root = [](int bits, int path) {  // exponent e of psi^e for a block after `bits` levels
+        int e = 512;
+        for (int k = bits - 1; k >= 0; k--) e = e / 2 + 1024 * ((path >> k) & 1);
+        return e;
+    };
+    const long double ps = pi / 1024.0L;
+    for (int lane = 0; lane < 64; lane++) {
+        const int a = lane >> 3, q = lane & 7;
+        const int E = root(3, a), E6 = root(6, 8 * a + q);
+        const int fb[4] = {E / 2, E / 4, E / 8, E / 8 + 256}, fc[4] = {E6 / 2, E6 / 4, E6 / 8, E6 / 8 + 256};
+        for (int k = 0; k < 4; k++) {
+            out[FftCT::FB + 64 * k + lane] = ct(ps * fb[k]);
+            out[FftCT::FC + 64 * k + lane] = ct(ps * fc[k]);
+        }
+        const int p = lane & 7, b = lane;
+        out[FftCT::IB + 0 * 64 + lane] = ct(-2 * pi * p / 16);
+        out[FftCT::IB + 1 * 64 + lane] = ct(-2 * pi * p / 32);
+        out[FftCT::IB + 2 * 64 + lane] = ct(-2 * pi * p / 64);
+        out[FftCT::IB + 3 * 64 + lane] = ct(-2 * pi * p / 64 - pi / 4);
+        out[FftCT::IA + 0 * 64 + lane] = ct(-2 * pi * b / 128);
+        out[FftCT::IA + 1 * 64 + lane] = ct(-2 * pi * b / 256);
+        out[FftCT::IA + 2 * 64 + lane] = ct(-2 * pi * b / 512);
+        out[FftCT::IA + 3 * 64 + lane] = ct(-2 * pi * b / 512 - pi / 4);
+        for (int aa = 0; aa < 8; aa++) {
+            const long double ang = -ps * (64 * aa + b);
+            out[FftCT::UT + 64 * aa + lane] = c64{(double)(cosl(ang) / 512.0L), (double)(sinl(ang) / 512.0L)};
+        }
+    }
+}
+
 // Twiddles of the half transform: [h][lane][10]: tw1_h[k1] = psi^{(1+4h) b} w_256^{b k1}
 // (psi = exp(i pi/1024), b = lane), then w_64^{g mu} (g = lane & 15, mu = 1..3) and
 // w_16^{g0 nu} (g0 = lane & 3, nu = 1..3).
