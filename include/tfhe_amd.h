@@ -294,13 +294,61 @@ TFHE_AMD_EXPORT int32_t tfhe_amd_netlist_levels(const tfhe_amd_netlist* nl, int3
 TFHE_AMD_EXPORT int32_t tfhe_amd_netlist_run(tfhe_amd_netlist* nl, int32_t* d_pool, void* stream, int32_t use_graph);
 TFHE_AMD_EXPORT void tfhe_amd_netlist_destroy(tfhe_amd_netlist* nl);
 
-/* ---- Key, parameter and ciphertext files: include/tfhe_io.h:30-97, 379-485 --
+/* ---- keys and key material as separate objects: include/lwekey.h:69-78,
+ *      include/tlwe.h:159-172, include/tgsw.h:118-177, include/lwekeyswitch.h:50-55,
+ *      include/lwebootstrappingkey.h:82-115. Parameters are not owned (as in the
+ *      reference); new_LweBootstrappingKey also builds its keyswitch key (input
+ *      dimension k N), and new_LweBootstrappingKeyFFT makes the handle that the
+ *      gates and tfhe_bootstrap_FFT take (the device key is built from bk, which
+ *      must outlive it, on first use). */
+TFHE_AMD_EXPORT LweKey* new_LweKey(const LweParams* params);
+TFHE_AMD_EXPORT void delete_LweKey(LweKey* key);
+TFHE_AMD_EXPORT TLweKey* new_TLweKey(const TLweParams* params);
+TFHE_AMD_EXPORT void delete_TLweKey(TLweKey* obj);
+TFHE_AMD_EXPORT TGswKey* new_TGswKey(const TGswParams* params);
+TFHE_AMD_EXPORT void delete_TGswKey(TGswKey* obj);
+TFHE_AMD_EXPORT TGswSample* new_TGswSample(const TGswParams* params);
+TFHE_AMD_EXPORT void delete_TGswSample(TGswSample* obj);
+TFHE_AMD_EXPORT LweKeySwitchKey* new_LweKeySwitchKey(int32_t n, int32_t t, int32_t basebit, const LweParams* out_params);
+TFHE_AMD_EXPORT void delete_LweKeySwitchKey(LweKeySwitchKey* obj);
+TFHE_AMD_EXPORT LweBootstrappingKey* new_LweBootstrappingKey(const int32_t ks_t, const int32_t ks_basebit,
+                                                             const LweParams* in_out_params, const TGswParams* bk_params);
+TFHE_AMD_EXPORT void delete_LweBootstrappingKey(LweBootstrappingKey* obj);
+TFHE_AMD_EXPORT LweBootstrappingKeyFFT* new_LweBootstrappingKeyFFT(const LweBootstrappingKey* bk);
+TFHE_AMD_EXPORT void delete_LweBootstrappingKeyFFT(LweBootstrappingKeyFFT* obj);
+
+/* ---- Key, parameter, sample and ciphertext files: include/tfhe_io.h:30-496 --
  * The reference's wire format (libtfhe/tfhe_io.cpp): text parameter
  * sections plus native-order binary key material, so files written by either
- * library load in the other. Readers return NULL on malformed input (reason in
- * tfhe_amd_last_error); writers abort on bad arguments, as the reference. The
- * std::ostream / std::istream variants of the reference are C++-only and not
- * part of this C ABI. */
+ * library load in the other. Constructors (new_*_from*) return NULL on
+ * malformed input (reason in tfhe_amd_last_error; parsed sizes are checked
+ * before allocating); importers into caller-allocated objects and writers
+ * abort on bad input or arguments, as the reference. Parameter objects that
+ * readers of keys create live until the process ends, as the reference's
+ * garbage-collected ones; the parameter objects new_*Params_from* return are
+ * the caller's. The std::ostream / std::istream forms follow below for C++. */
+TFHE_AMD_EXPORT void export_lweParams_toFile(FILE* F, const LweParams* params);
+TFHE_AMD_EXPORT LweParams* new_lweParams_fromFile(FILE* F);
+TFHE_AMD_EXPORT void export_lweSample_toFile(FILE* F, const LweSample* sample, const LweParams* params);
+TFHE_AMD_EXPORT void import_lweSample_fromFile(FILE* F, LweSample* sample, const LweParams* params);
+TFHE_AMD_EXPORT void export_lweKey_toFile(FILE* F, const LweKey* key);
+TFHE_AMD_EXPORT LweKey* new_lweKey_fromFile(FILE* F);
+TFHE_AMD_EXPORT void export_tLweParams_toFile(FILE* F, const TLweParams* params);
+TFHE_AMD_EXPORT TLweParams* new_tLweParams_fromFile(FILE* F);
+TFHE_AMD_EXPORT void export_tlweSample_toFile(FILE* F, const TLweSample* sample, const TLweParams* params);
+TFHE_AMD_EXPORT void import_tlweSample_fromFile(FILE* F, TLweSample* sample, const TLweParams* params);
+TFHE_AMD_EXPORT void export_tlweKey_toFile(FILE* F, const TLweKey* key);
+TFHE_AMD_EXPORT TLweKey* new_tlweKey_fromFile(FILE* F);
+TFHE_AMD_EXPORT void export_tGswParams_toFile(FILE* F, const TGswParams* params);
+TFHE_AMD_EXPORT TGswParams* new_tGswParams_fromFile(FILE* F);
+TFHE_AMD_EXPORT void export_tgswSample_toFile(FILE* F, const TGswSample* sample, const TGswParams* params);
+TFHE_AMD_EXPORT void import_tgswSample_fromFile(FILE* F, TGswSample* sample, const TGswParams* params);
+TFHE_AMD_EXPORT void export_tgswKey_toFile(FILE* F, const TGswKey* key);
+TFHE_AMD_EXPORT TGswKey* new_tgswKey_fromFile(FILE* F);
+TFHE_AMD_EXPORT void export_lweKeySwitchKey_toFile(FILE* F, const LweKeySwitchKey* ks);
+TFHE_AMD_EXPORT LweKeySwitchKey* new_lweKeySwitchKey_fromFile(FILE* F);
+TFHE_AMD_EXPORT void export_lweBootstrappingKey_toFile(FILE* F, const LweBootstrappingKey* bk);
+TFHE_AMD_EXPORT LweBootstrappingKey* new_lweBootstrappingKey_fromFile(FILE* F);
 TFHE_AMD_EXPORT void export_tfheGateBootstrappingParameterSet_toFile(FILE* F,
                                                                      const TFheGateBootstrappingParameterSet* params);
 TFHE_AMD_EXPORT TFheGateBootstrappingParameterSet* new_tfheGateBootstrappingParameterSet_fromFile(FILE* F);
@@ -313,14 +361,47 @@ TFHE_AMD_EXPORT void export_gate_bootstrapping_ciphertext_toFile(FILE* F, const 
                                                                  const TFheGateBootstrappingParameterSet* params);
 TFHE_AMD_EXPORT void import_gate_bootstrapping_ciphertext_fromFile(FILE* F, LweSample* sample,
                                                                    const TFheGateBootstrappingParameterSet* params);
-TFHE_AMD_EXPORT void export_lweSample_toFile(FILE* F, const LweSample* sample, const LweParams* params);
-TFHE_AMD_EXPORT void import_lweSample_fromFile(FILE* F, LweSample* sample, const LweParams* params);
-TFHE_AMD_EXPORT void export_lweParams_toFile(FILE* F, const LweParams* params);
-TFHE_AMD_EXPORT LweParams* new_lweParams_fromFile(FILE* F);
-TFHE_AMD_EXPORT void export_lweKey_toFile(FILE* F, const LweKey* key);
-/* The returned key owns its parameters: free it with delete_LweKey (include/lwekey.h:18-19). */
-TFHE_AMD_EXPORT LweKey* new_lweKey_fromFile(FILE* F);
-TFHE_AMD_EXPORT void delete_LweKey(LweKey* key);
+
+#ifdef __cplusplus
+/* the same over C++ streams (include/tfhe_io.h:44-496; C linkage, as the
+ * reference's EXPORT gives them) */
+#include <iosfwd>
+TFHE_AMD_EXPORT void export_lweParams_toStream(std::ostream& F, const LweParams* params);
+TFHE_AMD_EXPORT LweParams* new_lweParams_fromStream(std::istream& F);
+TFHE_AMD_EXPORT void export_lweSample_toStream(std::ostream& F, const LweSample* sample, const LweParams* params);
+TFHE_AMD_EXPORT void import_lweSample_fromStream(std::istream& F, LweSample* sample, const LweParams* params);
+TFHE_AMD_EXPORT void export_lweKey_toStream(std::ostream& F, const LweKey* key);
+TFHE_AMD_EXPORT LweKey* new_lweKey_fromStream(std::istream& F);
+TFHE_AMD_EXPORT void export_tLweParams_toStream(std::ostream& F, const TLweParams* params);
+TFHE_AMD_EXPORT TLweParams* new_tLweParams_fromStream(std::istream& F);
+TFHE_AMD_EXPORT void export_tlweSample_toStream(std::ostream& F, const TLweSample* sample, const TLweParams* params);
+TFHE_AMD_EXPORT void import_tlweSample_fromStream(std::istream& F, TLweSample* sample, const TLweParams* params);
+TFHE_AMD_EXPORT void export_tlweKey_toStream(std::ostream& F, const TLweKey* key);
+TFHE_AMD_EXPORT TLweKey* new_tlweKey_fromStream(std::istream& F);
+TFHE_AMD_EXPORT void export_tGswParams_toStream(std::ostream& F, const TGswParams* params);
+TFHE_AMD_EXPORT TGswParams* new_tGswParams_fromStream(std::istream& F);
+TFHE_AMD_EXPORT void export_tgswSample_toStream(std::ostream& F, const TGswSample* sample, const TGswParams* params);
+TFHE_AMD_EXPORT void import_tgswSample_fromStream(std::istream& F, TGswSample* sample, const TGswParams* params);
+TFHE_AMD_EXPORT void export_tgswKey_toStream(std::ostream& F, const TGswKey* key);
+TFHE_AMD_EXPORT TGswKey* new_tgswKey_fromStream(std::istream& F);
+TFHE_AMD_EXPORT void export_lweKeySwitchKey_toStream(std::ostream& F, const LweKeySwitchKey* ks);
+TFHE_AMD_EXPORT LweKeySwitchKey* new_lweKeySwitchKey_fromStream(std::istream& F);
+TFHE_AMD_EXPORT void export_lweBootstrappingKey_toStream(std::ostream& F, const LweBootstrappingKey* bk);
+TFHE_AMD_EXPORT LweBootstrappingKey* new_lweBootstrappingKey_fromStream(std::istream& F);
+TFHE_AMD_EXPORT void export_tfheGateBootstrappingParameterSet_toStream(std::ostream& F,
+                                                                       const TFheGateBootstrappingParameterSet* params);
+TFHE_AMD_EXPORT TFheGateBootstrappingParameterSet* new_tfheGateBootstrappingParameterSet_fromStream(std::istream& F);
+TFHE_AMD_EXPORT void export_tfheGateBootstrappingCloudKeySet_toStream(std::ostream& F,
+                                                                      const TFheGateBootstrappingCloudKeySet* key);
+TFHE_AMD_EXPORT TFheGateBootstrappingCloudKeySet* new_tfheGateBootstrappingCloudKeySet_fromStream(std::istream& F);
+TFHE_AMD_EXPORT void export_tfheGateBootstrappingSecretKeySet_toStream(std::ostream& F,
+                                                                       const TFheGateBootstrappingSecretKeySet* key);
+TFHE_AMD_EXPORT TFheGateBootstrappingSecretKeySet* new_tfheGateBootstrappingSecretKeySet_fromStream(std::istream& F);
+TFHE_AMD_EXPORT void export_gate_bootstrapping_ciphertext_toStream(std::ostream& F, const LweSample* sample,
+                                                                   const TFheGateBootstrappingParameterSet* params);
+TFHE_AMD_EXPORT void import_gate_bootstrapping_ciphertext_fromStream(std::istream& F, LweSample* sample,
+                                                                     const TFheGateBootstrappingParameterSet* params);
+#endif
 
 /* Client-side batch encryption / decryption on the device, over contiguous
  * ciphertext rows [count][n+1]. decrypt: bits[g] = (phase > 0) as

@@ -18,6 +18,18 @@
 //                               with the reference's file functions (tfhe_io.cpp)
 //   readfiles SEED DIR COUNT    read those files with the reference's readers; print the
 //                               decrypted bits, NAND(ct0, ct1) decrypted, and key hashes
+//   smallkeys SEED DIR COUNT    key set at the 110-bit parameters except n = 16 (the
+//                               reference's transforms are fixed at N = 1024): every object
+//                               its file functions write, one file each (tfhe_io.h:30-496),
+//                               and COUNT gates (all eleven types in turn) on fresh
+//                               encryptions: gates.bin = per gate op, 3 bits, 3 input
+//                               samples, the output sample (n+1 int32 each) and its phase
+//   extprod SEED OUTFILE COUNT  on the smallkeys key set of the same seed: COUNT one-row
+//                               blind rotations tfhe_blindRotate_FFT(acc, &bkFFT[i], &a, 1)
+//                               (lwe-bootstrapping-functions-fft.cpp:88-112: acc += row i (x)
+//                               ((X^a - 1) acc), the external product on the reference's
+//                               transform); per case i, a, the input and the output
+//                               accumulators ((k+1) N int32 each)
 #include <tfhe.h>
 #include <tfhe_io.h>
 #include <lwesamples.h>
@@ -119,13 +131,104 @@ static FILE* open_or_die(const std::string& path, const char* mode) {
     return f;
 }
 
+// A key set at explicit parameters (the reference's C++ parameter-set constructor,
+// include/tfhe_gate_bootstrapping_structures.h).
+static TFheGateBootstrappingSecretKeySet* make_keys_with(uint32_t seed, int n, int N, int l, int Bgbit, int t,
+                                                         int basebit, double a_lwe, double a_bk) {
+    uint32_t s[1] = {seed};
+    tfhe_random_generator_setSeed(s, 1);
+    LweParams* lp = new_LweParams(n, a_lwe, 0.012467);
+    TLweParams* tp = new_TLweParams(N, 1, a_bk, 0.012467);
+    TGswParams* gp = new_TGswParams(l, Bgbit, tp);
+    auto* P = new TFheGateBootstrappingParameterSet(t, basebit, lp, gp);
+    return new_random_gate_bootstrapping_secret_keyset(P);
+}
+
+static int write_small_keys(uint32_t seed, const std::string& dir, int count) {
+    TFheGateBootstrappingSecretKeySet* key = make_keys_with(seed, 16, 1024, 3, 7, 8, 2, 1. / 32768, 1. / 33554432);
+    const TFheGateBootstrappingParameterSet* P = key->params;
+    const int n = P->in_out_params->n;
+    const TGswParams* gp = P->tgsw_params;
+    const TLweParams* tp = gp->tlwe_params;
+    const LweBootstrappingKey* bk = key->cloud.bk;
+    std::mt19937_64 bitgen(0x5eed0000ULL + seed);
+    auto out = [&](const char* name) { return open_or_die(dir + "/" + name, "wb"); };
+    FILE* f;
+    f = out("secret.key"); export_tfheGateBootstrappingSecretKeySet_toFile(f, key); fclose(f);
+    f = out("cloud.key"); export_tfheGateBootstrappingCloudKeySet_toFile(f, &key->cloud); fclose(f);
+    f = out("params.gb"); export_tfheGateBootstrappingParameterSet_toFile(f, P); fclose(f);
+    f = out("lweparams.txt"); export_lweParams_toFile(f, P->in_out_params); fclose(f);
+    f = out("lwekey.bin"); export_lweKey_toFile(f, key->lwe_key); fclose(f);
+    f = out("tlweparams.txt"); export_tLweParams_toFile(f, tp); fclose(f);
+    f = out("tlwekey.bin"); export_tlweKey_toFile(f, &key->tgsw_key->tlwe_key); fclose(f);
+    f = out("tlwesample.bin"); export_tlweSample_toFile(f, &bk->bk[0].all_sample[1], tp); fclose(f);
+    f = out("tgswparams.txt"); export_tGswParams_toFile(f, gp); fclose(f);
+    f = out("tgswkey.bin"); export_tgswKey_toFile(f, key->tgsw_key); fclose(f);
+    f = out("tgswsample.bin"); export_tgswSample_toFile(f, &bk->bk[1], gp); fclose(f);
+    f = out("kskey.bin"); export_lweKeySwitchKey_toFile(f, bk->ks); fclose(f);
+    f = out("bk.bin"); export_lweBootstrappingKey_toFile(f, bk); fclose(f);
+    {
+        LweSample* s1 = new_gate_bootstrapping_ciphertext(P);
+        bootsSymEncrypt(s1, 1, key);
+        f = out("lwesample.bin"); export_lweSample_toFile(f, s1, P->in_out_params); fclose(f);
+        delete_gate_bootstrapping_ciphertext(s1);
+    }
+    f = open_or_die(dir + "/gates.bin", "wb");
+    int32_t hdr[2] = {count, n};
+    write_all(f, hdr, sizeof hdr);
+    LweSample* ct = new_gate_bootstrapping_ciphertext_array(4, P);
+    for (int g = 0; g < count; g++) {
+        const int op = g % 11;
+        int32_t rec[4] = {op, 0, 0, 0};
+        for (int j = 0; j < 3; j++) {
+            rec[1 + j] = (int32_t)(bitgen() & 1);
+            bootsSymEncrypt(&ct[j], rec[1 + j], key);
+        }
+        run_gate(op, &ct[3], &ct[0], &ct[1], &ct[2], &key->cloud);
+        write_all(f, rec, sizeof rec);
+        for (int j = 0; j < 4; j++) put_sample(f, &ct[j], n);
+        const int32_t ph = lwePhase(&ct[3], key->lwe_key);
+        write_all(f, &ph, sizeof ph);
+    }
+    fclose(f);
+    return 0;
+}
+
+static int write_extprod(uint32_t seed, const std::string& path, int count) {
+    TFheGateBootstrappingSecretKeySet* key = make_keys_with(seed, 16, 1024, 3, 7, 8, 2, 1. / 32768, 1. / 33554432);
+    const TFheGateBootstrappingParameterSet* P = key->params;
+    const TGswParams* gp = P->tgsw_params;
+    const int N = gp->tlwe_params->N, k = gp->tlwe_params->k, n = P->in_out_params->n;
+    std::mt19937_64 gen(0xe7e7ULL + seed);
+    TLweSample* acc = new_TLweSample(gp->tlwe_params);
+    FILE* f = open_or_die(path, "wb");
+    int32_t hdr[3] = {count, k, N};
+    write_all(f, hdr, sizeof hdr);
+    for (int c = 0; c < count; c++) {
+        const int32_t row = c % n;
+        int32_t a = (int32_t)(1 + gen() % (2 * N - 1));
+        if (c == 1) a = N;  // a pure negation
+        for (int q = 0; q <= k; q++)
+            for (int j = 0; j < N; j++) acc->a[q].coefsT[j] = (int32_t)(uint32_t)gen();
+        int32_t rec[2] = {row, a};
+        write_all(f, rec, sizeof rec);
+        for (int q = 0; q <= k; q++) write_all(f, acc->a[q].coefsT, sizeof(int32_t) * N);
+        tfhe_blindRotate_FFT(acc, key->cloud.bkFFT->bkFFT + row, &a, 1, gp);
+        for (int q = 0; q <= k; q++) write_all(f, acc->a[q].coefsT, sizeof(int32_t) * N);
+    }
+    fclose(f);
+    return 0;
+}
+
 int main(int argc, char** argv) {
     if (argc < 3) {
-        fprintf(stderr, "usage: ref_harness keys|gates|ks|bench SEED ...\n");
+        fprintf(stderr, "usage: ref_harness keys|gates|ks|bench|files|readfiles|components|smallkeys SEED ...\n");
         return 2;
     }
     const std::string cmd = argv[1];
     const uint32_t seed = (uint32_t)strtoul(argv[2], nullptr, 10);
+    if (cmd == "smallkeys" && argc >= 5) return write_small_keys(seed, argv[3], atoi(argv[4]));
+    if (cmd == "extprod" && argc >= 5) return write_extprod(seed, argv[3], atoi(argv[4]));
     TFheGateBootstrappingSecretKeySet* key = make_keys(seed);
     const TFheGateBootstrappingParameterSet* P = key->params;
     const int n = P->in_out_params->n;

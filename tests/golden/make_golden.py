@@ -6,10 +6,14 @@ sources); run in a container where those sources exist. The vectors are data
 only: hashes of the reference's seed-0 keys, a few reference keyswitch
 input/output pairs, the reference's output phases for gates on encryptions
 that the oracle replays from the same seed, and the small files of the
-reference's key-file format (parameters, LWE key, ciphertexts) for seed 11.
+reference's key-file format (parameters, LWE key, ciphertexts) for seed 11,
+and every object of that format for a key set with n = 16 (seed 7) with the
+reference's outputs of 22 gates under it.
 
     python tests/golden/make_golden.py
 """
+import hashlib
+import json
 import os
 import subprocess
 import sys
@@ -74,6 +78,40 @@ def main():
         for f in ("params.gb", "lwekey.bin", "cts.bin", "nand.bin", "bits.txt"):
             with open(os.path.join(tmp, f), "rb") as src, open(os.path.join(io_dir, f), "wb") as dst:
                 dst.write(src.read())
+        # every object of the reference's file functions, from a key set at the
+        # 110-bit parameters with n = 16 (3 MB), and 22 gates (each type twice) with
+        # their reference outputs; the three large derived files (keyswitch key,
+        # bootstrapping key, cloud key, byte-identical sections of secret.key) are
+        # kept as SHA-256 digests
+        sk_dir = os.path.join(HERE, "ref_io_n16_seed7")
+        os.makedirs(sk_dir, exist_ok=True)
+        run("smallkeys", 7, tmp, 22)
+        for f in ("secret.key", "params.gb", "lweparams.txt", "lwekey.bin", "lwesample.bin", "tlweparams.txt",
+                  "tlwekey.bin", "tlwesample.bin", "tgswparams.txt", "tgswkey.bin", "tgswsample.bin"):
+            with open(os.path.join(tmp, f), "rb") as src, open(os.path.join(sk_dir, f), "wb") as dst:
+                dst.write(src.read())
+        digests = {}
+        for f in ("kskey.bin", "bk.bin", "cloud.key"):
+            with open(os.path.join(tmp, f), "rb") as src:
+                data = src.read()
+            digests[f] = {"bytes": len(data), "sha256": hashlib.sha256(data).hexdigest()}
+        with open(os.path.join(sk_dir, "digests.json"), "w") as dst:
+            json.dump(digests, dst, indent=1, sort_keys=True)
+        raw = np.fromfile(os.path.join(tmp, "gates.bin"), np.int32)
+        count, n = raw[:2]
+        rec = raw[2:].reshape(count, 4 + 4 * (n + 1) + 1)
+        np.savez_compressed(os.path.join(sk_dir, "gates.npz"), ops=rec[:, 0].copy(), bits=rec[:, 1:4].copy(),
+                            inputs=rec[:, 4:4 + 3 * (n + 1)].reshape(count, 3, n + 1).copy(),
+                            outputs=rec[:, 4 + 3 * (n + 1):4 + 4 * (n + 1)].copy(), phase=rec[:, -1].copy())
+        # one-row blind rotations on the reference's own key rows (its transform)
+        path = os.path.join(tmp, "extprod.bin")
+        run("extprod", 7, path, 12)
+        raw = np.fromfile(path, np.int32)
+        count, k, N = raw[:3]
+        rec = raw[3:].reshape(count, 2 + 2 * (k + 1) * N)
+        np.savez_compressed(os.path.join(sk_dir, "blind_rotate_rows.npz"), row=rec[:, 0].copy(),
+                            abar=rec[:, 1].copy(), acc_in=rec[:, 2:2 + (k + 1) * N].reshape(count, k + 1, N).copy(),
+                            acc_out=rec[:, 2 + (k + 1) * N:].reshape(count, k + 1, N).copy())
     for f in sorted(os.listdir(HERE)):
         print(f, os.path.getsize(os.path.join(HERE, f)))
 

@@ -174,3 +174,83 @@ def test_reference_reads_this_librarys_files(tmp_path):
     assert int(vals["ks_hash"]) == O.fnv1a(K["ksk"])
     assert int(vals["bk_hash"]) == O.fnv1a(K["bk"])
     assert vals["params"].split()[:7] == ["630", "1024", "1", "3", "7", "8", "2"]
+
+
+# ---------------------------------------- every object of the format (n = 16 key set)
+
+GOLD16 = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "ref_io_n16_seed7")
+
+
+@pytest.fixture(scope="module")
+def io_components(tmp_path_factory):
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    exe = str(tmp_path_factory.mktemp("ioc") / "io_components")
+    subprocess.run(["g++", "-std=c++17", "-O1", "-Wall", "-Werror", "-o", exe,
+                    os.path.join(root, "tests", "c", "io_components.cpp"), "-I", os.path.join(root, "include"),
+                    "-L", os.path.join(root, "tfhe_amd"), "-ltfhe_amd",
+                    "-Wl,-rpath," + os.path.join(root, "tfhe_amd")], check=True)
+    return exe
+
+
+def test_every_object_file_and_stream_form_matches_reference_bytes(io_components, tmp_path):
+    """Files the reference wrote for one key set (parameters of each kind, LWE /
+    TLWE / TGSW keys and samples, the secret key set): each is read through the
+    FILE* reader and written through the std::ostream writer, and read through the
+    std::istream reader and written through the FILE* writer; both copies equal
+    the reference's bytes. The keyswitch key, bootstrapping key and cloud key
+    set this library writes from the loaded secret key set have the SHA-256
+    digests of the reference's own files."""
+    import hashlib
+    import json
+    out = subprocess.run([io_components, GOLD16, str(tmp_path)], capture_output=True, text=True)
+    assert out.returncode == 0, out.stdout[-2000:] + out.stderr[-2000:]
+    assert out.stdout.strip().endswith("done")
+    assert out.stdout.count("ok ") >= 40
+    digests = json.load(open(os.path.join(GOLD16, "digests.json")))
+    for name, d in digests.items():
+        data = open(tmp_path / name, "rb").read()
+        assert len(data) == d["bytes"] and hashlib.sha256(data).hexdigest() == d["sha256"], name
+
+
+def test_n16_reference_gates_decrypt_under_the_loaded_key():
+    """The reference's gate outputs under the n = 16 key set decrypt, with the
+    secret key set read from its file, to the plaintext gates."""
+    sk = T.SecretKey.load(os.path.join(GOLD16, "secret.key"))
+    try:
+        g = np.load(os.path.join(GOLD16, "gates.npz"))
+        assert sk.params.n == 16
+        bits = g["bits"]
+        assert np.array_equal(sk.decrypt(g["inputs"].reshape(-1, 17)).reshape(-1, 3), bits)
+        exp = np.array([O.TRUTH[O.GATES[o]](*b) for o, b in zip(g["ops"], bits)])
+        assert np.array_equal(sk.decrypt(g["outputs"]), exp)
+        assert np.array_equal(sk.phases(g["outputs"]), g["phase"])
+    finally:
+        sk.close()
+
+
+@pytest.mark.parametrize("field,value,reason", [
+    ("N", "1000", "power of two"),
+    ("N", "1048576", "out of range"),
+    ("k", "0", "out of range"),
+    ("l", "9", "out of range"),
+    ("n", "-3", "out of range"),
+    ("n", "99999999999", "out of range"),
+])
+def test_malformed_headers_are_rejected_before_allocating(tmp_path, field, value, reason):
+    """Parameter values outside the readers' limits (a non-power-of-two or huge
+    N, k = 0, l Bgbit > 32, a negative or huge n) are rejected with a reason
+    and NULL, before any key material is allocated."""
+    import re
+    text = open(os.path.join(GOLD16, "secret.key"), "rb").read()
+    head_end = text.index(b"-----END TGSWPARAMS-----\n") + len(b"-----END TGSWPARAMS-----\n")
+    head = text[:head_end].decode()
+    section = {"N": "TLWEPARAMS", "k": "TLWEPARAMS", "l": "TGSWPARAMS", "n": "LWEPARAMS"}[field]
+    a = head.index(f"-----BEGIN {section}-----")
+    b = head.index(f"-----END {section}-----")
+    body = re.sub(rf"^{field}: .*$", f"{field}: {value}", head[a:b], flags=re.M)
+    bad = head[:a] + body + head[b:]
+    (tmp_path / "bad.key").write_bytes(bad.encode() + text[head_end:])
+    with T._CFile(tmp_path / "bad.key", "rb") as f:
+        p = T.lib().new_tfheGateBootstrappingSecretKeySet_fromFile(f)
+    assert not p
+    assert reason in T.lib().tfhe_amd_last_error().decode()

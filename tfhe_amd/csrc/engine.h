@@ -87,6 +87,19 @@ TFheGateBootstrappingSecretKeySet* host_secret_from_arrays(const TFheGateBootstr
                                                            double bk_var, const int32_t* lwe_key,
                                                            const int32_t* tlwe_key);
 
+// Key sets around a coefficient-domain bootstrapping key already built (it and
+// its keyswitch key become the key set's), for the key-file readers; lwe_key
+// [n], tlwe_key [k N].
+TFheGateBootstrappingCloudKeySet* host_cloud_from_bk(const TFheGateBootstrappingParameterSet* P,
+                                                     LweBootstrappingKey* bk);
+TFheGateBootstrappingSecretKeySet* host_secret_from_bk(const TFheGateBootstrappingParameterSet* P,
+                                                       LweBootstrappingKey* bk, const int32_t* lwe_key,
+                                                       const int32_t* tlwe_key);
+// Parameters that live until the process ends (parameters read from files).
+const LweParams* host_keep(LweParams* p);
+const TLweParams* host_keep(TLweParams* p);
+const TGswParams* host_keep(TGswParams* p);
+
 // Levelisation of a gate netlist (schedule 0 = as soon as possible, 1 = as
 // late as possible): gate order sorted by level and the gates per level.
 // Validates gate codes, wire ranges, single drivers and acyclicity.

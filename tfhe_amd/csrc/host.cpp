@@ -138,11 +138,10 @@ KsOwner* new_ks(int32_t nin, int32_t t, int32_t basebit, const LweParams* out_pa
     return o;
 }
 
-BkOwner* new_bk(const TFheGateBootstrappingParameterSet* P, LweKeySwitchKey* ks) {
+BkOwner* new_bk(const LweParams* in_out, const TGswParams* gp, LweKeySwitchKey* ks) {
     auto* o = new BkOwner();
-    const TGswParams* gp = P->tgsw_params;
     const TLweParams* tp = gp->tlwe_params;
-    const int n = P->in_out_params->n, N = tp->N, k = tp->k, l = gp->l, kpl = gp->kpl;
+    const int n = in_out->n, N = tp->N, k = tp->k, l = gp->l, kpl = gp->kpl;
     o->coefs.assign((size_t)n * kpl * (k + 1) * N, 0);
     o->polys.resize((size_t)n * kpl * (k + 1));
     for (size_t q = 0; q < o->polys.size(); q++) o->polys[q] = TorusPolynomial{N, o->coefs.data() + q * N};
@@ -155,8 +154,11 @@ BkOwner* new_bk(const TFheGateBootstrappingParameterSet* P, LweKeySwitchKey* ks)
         for (int b = 0; b <= k; b++) o->blocs[(size_t)i * (k + 1) + b] = o->samples.data() + (size_t)i * kpl + b * l;
         o->gsw[i] = TGswSample{o->samples.data() + (size_t)i * kpl, o->blocs.data() + (size_t)i * (k + 1), k, l};
     }
-    o->pub = LweBootstrappingKey{P->in_out_params, gp, tp, &tp->extracted_lweparams, o->gsw.data(), ks};
+    o->pub = LweBootstrappingKey{in_out, gp, tp, &tp->extracted_lweparams, o->gsw.data(), ks};
     return o;
+}
+BkOwner* new_bk(const TFheGateBootstrappingParameterSet* P, LweKeySwitchKey* ks) {
+    return new_bk(P->in_out_params, P->tgsw_params, ks);
 }
 
 BkFftOwner* new_bkfft(const LweBootstrappingKey* bk) {
@@ -318,6 +320,26 @@ TFheGateBootstrappingParameterSet* host_make_params(int32_t n, double lwe_alpha_
 }
 }  // namespace tfhe_amd
 
+namespace tfhe_amd {
+// Parameter objects read from files live until the process ends, as the
+// reference's TfheGarbageCollector keeps them (libtfhe/tfhe_garbage_collector.cpp).
+const LweParams* host_keep(LweParams* p) {
+    std::lock_guard<std::mutex> g(g_param_mu);
+    g_lwe_params.emplace_back(p);
+    return p;
+}
+const TLweParams* host_keep(TLweParams* p) {
+    std::lock_guard<std::mutex> g(g_param_mu);
+    g_tlwe_params.emplace_back(p);
+    return p;
+}
+const TGswParams* host_keep(TGswParams* p) {
+    std::lock_guard<std::mutex> g(g_param_mu);
+    g_tgsw_params.push_back(p);
+    return p;
+}
+}  // namespace tfhe_amd
+
 TFHE_AMD_EXPORT TFheGateBootstrappingParameterSet* new_gate_bootstrapping_parameters(
     int32_t n, int32_t N, int32_t k, int32_t bk_l, int32_t bk_Bgbit, int32_t ks_t, int32_t ks_basebit,
     double ks_stdev, double bk_stdev, double max_stdev) {
@@ -374,6 +396,84 @@ TFHE_AMD_EXPORT void delete_TLweSample(TLweSample* obj) {
     delete[] obj->a;
     delete obj;
 }
+
+// ---- keys and key material as separate objects (include/lwekey.h:69-78,
+//      include/tlwe.h:159-172, include/tgsw.h:118-177, include/lwekeyswitch.h:50-55,
+//      include/lwebootstrappingkey.h:82-115). Parameters are not owned.
+TFHE_AMD_EXPORT LweKey* new_LweKey(const LweParams* params) { return new LweKey{params, new int32_t[params->n]()}; }
+TFHE_AMD_EXPORT void delete_LweKey(LweKey* key) {
+    if (!key) return;
+    delete[] key->key;
+    delete key;
+}
+TFHE_AMD_EXPORT TLweKey* new_TLweKey(const TLweParams* params) {
+    auto* polys = new IntPolynomial[params->k];
+    for (int i = 0; i < params->k; i++) polys[i] = IntPolynomial{params->N, new int32_t[params->N]()};
+    return new TLweKey{params, polys};
+}
+TFHE_AMD_EXPORT void delete_TLweKey(TLweKey* obj) {
+    if (!obj) return;
+    for (int i = 0; i < obj->params->k; i++) delete[] obj->key[i].coefs;
+    delete[] obj->key;
+    delete obj;
+}
+// the TGSW key is its TLWE key (libtfhe/tgsw.cpp: key aliases tlwe_key.key)
+TFHE_AMD_EXPORT TGswKey* new_TGswKey(const TGswParams* params) {
+    TLweKey* t = new_TLweKey(params->tlwe_params);
+    auto* k = new TGswKey{params, params->tlwe_params, t->key, *t};
+    delete t;  // the polynomials now belong to k
+    return k;
+}
+TFHE_AMD_EXPORT void delete_TGswKey(TGswKey* obj) {
+    if (!obj) return;
+    for (int i = 0; i < obj->tlwe_params->k; i++) delete[] obj->key[i].coefs;
+    delete[] obj->key;
+    delete obj;
+}
+// rows all_sample[(k+1) l], block b = rows b l .. b l + l - 1 (libtfhe/tgsw.cpp)
+TFHE_AMD_EXPORT TGswSample* new_TGswSample(const TGswParams* params) {
+    const int k = params->tlwe_params->k, l = params->l, kpl = params->kpl;
+    auto* rows = new TLweSample[kpl];
+    for (int r = 0; r < kpl; r++) {
+        TLweSample* t = new_TLweSample(params->tlwe_params);
+        rows[r] = *t;
+        delete t;
+    }
+    auto* blocs = new TLweSample*[k + 1];
+    for (int b = 0; b <= k; b++) blocs[b] = rows + b * l;
+    return new TGswSample{rows, blocs, k, l};
+}
+TFHE_AMD_EXPORT void delete_TGswSample(TGswSample* obj) {
+    if (!obj) return;
+    const int kpl = (obj->k + 1) * obj->l;
+    for (int r = 0; r < kpl; r++) {
+        for (int c = 0; c <= obj->k; c++) delete[] obj->all_sample[r].a[c].coefsT;
+        delete[] obj->all_sample[r].a;
+    }
+    delete[] obj->all_sample;
+    delete[] obj->bloc_sample;
+    delete obj;
+}
+TFHE_AMD_EXPORT LweKeySwitchKey* new_LweKeySwitchKey(int32_t n, int32_t t, int32_t basebit, const LweParams* out_params) {
+    return &new_ks(n, t, basebit, out_params)->pub;
+}
+TFHE_AMD_EXPORT void delete_LweKeySwitchKey(LweKeySwitchKey* obj) { delete reinterpret_cast<KsOwner*>(obj); }
+// with its keyswitch key from the extracted dimension k N (libtfhe/lwebootstrappingkey.cpp)
+TFHE_AMD_EXPORT LweBootstrappingKey* new_LweBootstrappingKey(const int32_t ks_t, const int32_t ks_basebit,
+                                                             const LweParams* in_out_params, const TGswParams* bk_params) {
+    const TLweParams* tp = bk_params->tlwe_params;
+    KsOwner* ks = new_ks(tp->N * tp->k, ks_t, ks_basebit, in_out_params);
+    return &new_bk(in_out_params, bk_params, &ks->pub)->pub;
+}
+TFHE_AMD_EXPORT void delete_LweBootstrappingKey(LweBootstrappingKey* obj) { delete_bk(obj); }
+// the transform-domain view used by the gates and tfhe_bootstrap_FFT: here a
+// handle on the device key, built from bk (which must outlive it) on first use
+TFHE_AMD_EXPORT LweBootstrappingKeyFFT* new_LweBootstrappingKeyFFT(const LweBootstrappingKey* bk) {
+    LweBootstrappingKeyFFT* bf = &new_bkfft(bk)->pub;
+    register_ks(bk->ks, bf);
+    return bf;
+}
+TFHE_AMD_EXPORT void delete_LweBootstrappingKeyFFT(LweBootstrappingKeyFFT* obj) { delete_bkfft(obj); }
 
 TFHE_AMD_EXPORT LweSample* new_gate_bootstrapping_ciphertext(const TFheGateBootstrappingParameterSet* params) {
     return new_LweSample(params->in_out_params);
@@ -522,10 +622,16 @@ TFheGateBootstrappingCloudKeySet* host_cloud_from_arrays(const TFheGateBootstrap
     return &co->pub;
 }
 
-TFheGateBootstrappingSecretKeySet* host_secret_from_arrays(const TFheGateBootstrappingParameterSet* P,
-                                                           const int32_t* bkc, const int32_t* ksk, double ks_var,
-                                                           double bk_var, const int32_t* lwe_key,
-                                                           const int32_t* tlwe_key) {
+TFheGateBootstrappingCloudKeySet* host_cloud_from_bk(const TFheGateBootstrappingParameterSet* P,
+                                                     LweBootstrappingKey* bk) {
+    auto* co = new CloudOwner();
+    co->pub = make_cloud(P, bk);
+    return &co->pub;
+}
+
+TFheGateBootstrappingSecretKeySet* host_secret_from_bk(const TFheGateBootstrappingParameterSet* P,
+                                                       LweBootstrappingKey* bk, const int32_t* lwe_key,
+                                                       const int32_t* tlwe_key) {
     const TGswParams* gp = P->tgsw_params;
     const TLweParams* tp = gp->tlwe_params;
     const int n = P->in_out_params->n, N = tp->N, k = tp->k;
@@ -536,9 +642,16 @@ TFheGateBootstrappingSecretKeySet* host_secret_from_arrays(const TFheGateBootstr
     so->tlwe_polys.resize(k);
     for (int i = 0; i < k; i++) so->tlwe_polys[i] = IntPolynomial{N, so->tlwe_bits.data() + (size_t)i * N};
     so->tgsw_key = TGswKey{gp, tp, so->tlwe_polys.data(), TLweKey{tp, so->tlwe_polys.data()}};
-    so->pub = TFheGateBootstrappingSecretKeySet{P, &so->lwe_key, &so->tgsw_key,
-                                                make_cloud(P, bk_from_arrays(P, bkc, ksk, ks_var, bk_var))};
+    so->pub = TFheGateBootstrappingSecretKeySet{P, &so->lwe_key, &so->tgsw_key, make_cloud(P, bk)};
     return &so->pub;
+}
+
+TFheGateBootstrappingSecretKeySet* host_secret_from_arrays(const TFheGateBootstrappingParameterSet* P,
+                                                           const int32_t* bkc, const int32_t* ksk, double ks_var,
+                                                           double bk_var, const int32_t* lwe_key,
+                                                           const int32_t* tlwe_key) {
+    return host_secret_from_bk(P, const_cast<LweBootstrappingKey*>(bk_from_arrays(P, bkc, ksk, ks_var, bk_var)),
+                               lwe_key, tlwe_key);
 }
 }  // namespace tfhe_amd
 
