@@ -6,6 +6,7 @@
 #include <cstdlib>
 #include <cstdint>
 #include <random>
+#include <string>
 #include <vector>
 #include <cmath>
 
@@ -272,7 +273,79 @@ int run_with(Em& E, int trials, unsigned seed, int dbits) {
     return mismatches ? 1 : 0;
 }
 
+// Multiply-accumulate as the blind-rotation kernels do it: sum over r < kpl of
+// F(d_r) F(t_r), one inverse, rounding; compared with the exact sum of the
+// negacyclic products mod 2^32. worst = 0: digits uniform in [-2^(b-1), 2^(b-1))
+// and torus values uniform (what a bootstrap sees: the key is uniformly random);
+// worst = 1: every digit at its extreme and every key value at +-2^31 with signs
+// aligned so that output coefficient 0 reaches kpl N 2^(b-1) 2^31 in magnitude.
+// Prints the largest distance of an output to the nearest integer, the
+// mismatches, and log2 of the largest exact output magnitude.
+template <class Em, int M, int SLOTS>
+int run_mac(Em& E, int kpl, int trials, unsigned seed, int dbits, int worst) {
+    const int N = 2 * M;
+    constexpr int LANES = M / SLOTS;
+    std::mt19937_64 rng(seed);
+    static c64 X[LANES][SLOTS], Y[LANES][SLOTS], A[LANES][SLOTS];
+    std::vector<double> pd(N), pt(N), out(N);
+    double maxerr = 0, maxmag = 0;
+    long mismatches = 0;
+    const int64_t half = (int64_t)1 << (dbits - 1);
+    for (int tr = 0; tr < trials; tr++) {
+        std::vector<std::vector<int64_t>> d(kpl, std::vector<int64_t>(N)), t(kpl, std::vector<int64_t>(N));
+        for (int r = 0; r < kpl; r++)
+            for (int j = 0; j < N; j++) {
+                if (worst) {
+                    // coefficient 0 = sum_i d_i t_{-i} with the wrap sign -1 for i > 0
+                    const int64_t sgn = (rng() & 1) ? 1 : -1;
+                    d[r][j] = sgn > 0 ? half - 1 : -half;
+                    const int64_t tv = (j == 0) ? sgn : -sgn;  // sign of t_{-j} that aligns the product
+                    t[r][(N - j) % N] = tv > 0 ? INT32_MAX : INT32_MIN;
+                } else {
+                    d[r][j] = (int64_t)(rng() % (uint64_t)(2 * half)) - half;
+                    t[r][j] = (int32_t)(uint32_t)rng();
+                }
+            }
+        for (int l = 0; l < LANES; l++)
+            for (int s = 0; s < SLOTS; s++) A[l][s] = c64{0, 0};
+        for (int r = 0; r < kpl; r++) {
+            for (int j = 0; j < N; j++) pd[j] = (double)d[r][j], pt[j] = (double)t[r][j];
+            E.forward(pd.data(), X);
+            E.forward(pt.data(), Y);
+            for (int l = 0; l < LANES; l++)
+                for (int s = 0; s < SLOTS; s++) cmac(A[l][s], X[l][s], Y[l][s]);
+        }
+        E.inverse(A, out.data());
+        for (int k = 0; k < N; k++) {
+            __int128 ex = 0;
+            for (int r = 0; r < kpl; r++)
+                for (int i = 0; i < N; i++) {
+                    const int j = k - i;
+                    const __int128 p = (__int128)d[r][i] * t[r][j >= 0 ? j : j + N];
+                    ex += j >= 0 ? p : -p;
+                }
+            maxmag = std::fmax(maxmag, std::fabs((double)ex));
+            const double rr = std::nearbyint(out[k]);
+            maxerr = std::fmax(maxerr, std::fabs(out[k] - rr));
+            if ((uint32_t)(int64_t)rr != (uint32_t)(uint64_t)ex) mismatches++;
+        }
+    }
+    printf("maxerr %.6g mismatches %ld log2max %.3f\n", maxerr, mismatches, std::log2(maxmag));
+    return 0;
+}
+
 int main(int argc, char** argv) {
+    if (argc > 1 && std::string(argv[1]) == "mac") {
+        // fft_emu mac N KPL TRIALS SEED DIGIT_BITS WORST   (N = 1: N = 1024 fused; -2048: pair)
+        const int N = atoi(argv[2]), kpl = atoi(argv[3]), trials = atoi(argv[4]), dbits = atoi(argv[6]);
+        const unsigned seed = atoi(argv[5]);
+        const int worst = atoi(argv[7]);
+        if (N == 1) { EmuCT E; return run_mac<EmuCT, 512, 8>(E, kpl, trials, seed, dbits, worst); }
+        if (N == 1024) { Emu<512> E; return run_mac<Emu<512>, 512, 8>(E, kpl, trials, seed, dbits, worst); }
+        if (N == -2048) { EmuPair E; return run_mac<EmuPair, 1024, 8>(E, kpl, trials, seed, dbits, worst); }
+        if (N == -1024) { EmuHalf E; return run_mac<EmuHalf, 512, 4>(E, kpl, trials, seed, dbits, worst); }
+        return 2;
+    }
     const int N = argc > 1 ? atoi(argv[1]) : 1024;
     const int trials = argc > 2 ? atoi(argv[2]) : 4;
     const unsigned seed = argc > 3 ? atoi(argv[3]) : 1;

@@ -260,3 +260,27 @@ def test_wavefront_transform_emulation(fft_emu, N, bits):
     assert f[0] == "maxerr" and float(f[1]) < 0.1
     assert int(f[3]) == 0
     assert float(f[5]) < 1e-4
+
+
+@pytest.mark.parametrize("N,kpl,bits,worst,exact", [
+    (1, 6, 7, 0, True), (1, 6, 7, 1, True),          # P128, fused transform: typical and sign-aligned worst case
+    (1024, 6, 7, 1, True), (-1024, 6, 7, 1, True),   # P128, radix-8 and half transforms, worst case
+    (-2048, 4, 10, 0, True),                         # P_big, pair transform: typical
+    (-2048, 4, 10, 1, False),                        # P_big worst case: |sum| = 2^53, past FP64 exactness
+])
+def test_transform_exactness_bound(fft_emu, N, kpl, bits, worst, exact):
+    """Where the FP64 product stops being exact. The kernels sum kpl digit x key
+    products in the transform domain before one inverse; the emulation does the
+    same lane by lane. With uniform digits and keys (worst = 0) outputs stay
+    within 0.07 of an integer; with every digit and key value at its extreme and
+    signs aligned on one output coefficient (worst = 1) P128 reaches 2^49.6 and
+    is still exact (within 0.13), while P_big reaches 2^53 and is not. The key
+    is uniformly random and secret, so that alignment has probability about
+    2^-N; see the DESIGN note on the exactness bound."""
+    out = subprocess.run([fft_emu, "mac", str(N), str(kpl), "2", "3", str(bits), str(worst)], capture_output=True,
+                         text=True, check=True).stdout.split()
+    err, miss, mag = float(out[1]), int(out[3]), float(out[5])
+    if exact:
+        assert miss == 0 and err < 0.2, out
+    else:
+        assert miss > 0 and mag > 52.9, out
