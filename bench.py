@@ -204,7 +204,9 @@ def bench_serving(T, P, args):
         st = S.stats()
         row = {"requests": len(reqs), "gates": gates, "offered_gates_per_s": rate,
                "gates_per_s": round(gates / (end - start), 1), "window_s": round(end - start, 2),
-               "p50_ms": round(float(np.percentile(lat, 50)), 2), "p99_ms": round(float(np.percentile(lat, 99)), 2),
+               "p50_ms": round(float(np.percentile(lat, 50)), 2),
+               # a 99th percentile needs at least 100 samples; below that only the maximum is reported
+               "p99_ms": round(float(np.percentile(lat, 99)), 2) if len(lat) >= 100 else None,
                "max_ms": round(float(lat.max()), 2), "output_bit_errors": errors,
                "launches_total": st["launches"], "packed_launches_total": st["packed_launches"],
                "max_launch_gates": st["max_launch_gates"],
@@ -457,6 +459,30 @@ def bench_host_api(T, sk, P, B, seed):
     return res
 
 
+def usable_cpus():
+    """Cores this process may run on: the affinity mask (what nproc prints), capped
+    by a cgroup v2 CPU quota when one is set; plus the CPU model and flags."""
+    aff = len(os.sched_getaffinity(0))
+    quota = None
+    try:
+        q, per = open("/sys/fs/cgroup/cpu.max").read().split()
+        if q != "max":
+            quota = max(1, int(int(q) / int(per)))
+    except (OSError, ValueError):
+        pass
+    model, flags = "", ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name") and not model:
+                model = line.split(":", 1)[1].strip()
+            if line.startswith("flags") and not flags:
+                flags = line.split(":", 1)[1]
+    except OSError:
+        pass
+    return {"affinity_cpus": aff, "cgroup_quota_cpus": quota, "usable": min(aff, quota) if quota else aff,
+            "model": model, "avx512": " avx512f " in f" {flags} ", "flags": flags}
+
+
 def cpu_baseline(count, seed):
     """Reference spqlios-FMA CPU path (oracle/_ref, built from the reference's
     own sources) timed on this host; falls back to the oracle port."""
@@ -471,9 +497,12 @@ def cpu_baseline(count, seed):
                "sample": f"{count} bootsNAND on fresh encryptions, P128 keys from seed {seed}, "
                          f"reference spqlios-FMA build, 1 thread",
                "p50_ms": r["p50_ms"], "p99_ms": r["p99_ms"]}
-        # the same reference build on every host thread (at most 16), independent gates per
-        # thread sharing one cloud key; a short sample of 100 gates per thread
-        threads = max(1, min(16, os.cpu_count() or 1))
+        # the same reference build on every core this process may use (the nproc of the
+        # affinity mask, capped by a cgroup CPU quota if one is set), independent gates
+        # per thread sharing one cloud key; a short sample of 100 gates per thread
+        cpus = usable_cpus()
+        threads = cpus["usable"]
+        res["host"] = cpus
         if threads > 1:
             out = subprocess.run([harness, "bench", str(seed), str(100 * threads), str(threads)],
                                  capture_output=True, text=True, timeout=600,
@@ -481,6 +510,17 @@ def cpu_baseline(count, seed):
             m = json.loads(out)
             res["all_threads"] = {"threads": threads, "gates_per_s": round(m["gates_per_s"], 3),
                                   "p50_ms": m["p50_ms"], "p99_ms": m["p99_ms"], "gates": 100 * threads}
+        # the reference built for AVX-512 hosts (-march=x86-64-v4, closer to its own
+        # -march=native build) beside the portable AVX2/FMA build, one thread, where
+        # the host CPU has AVX-512
+        h4 = os.path.join(ROOT, "oracle", "_ref", "v4", "ref_harness")
+        if os.path.exists(h4) and cpus["avx512"]:
+            out = subprocess.run([h4, "bench", str(seed), str(count), "1"], capture_output=True, text=True,
+                                 timeout=600, check=True).stdout.strip().splitlines()[-1]
+            r4 = json.loads(out)
+            res["avx512_build"] = {"flags": "-O2 -march=x86-64-v4", "gates_per_s": round(r4["gates_per_s"], 3),
+                                   "p50_ms": r4["p50_ms"], "threads": 1}
+        res["host"] = {k: v for k, v in cpus.items() if k != "flags"}
         return res
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_py as O
@@ -661,14 +701,22 @@ def main():
         "host_api": host_api,
         "kernel_ms": {"blind_rotate_avg": round(br_ms / max(launches, 1), 4),
                       "keyswitch_avg": round(ks_ms / max(launches, 1), 4)},
-        "roofline": {"bound": "mfma", "achieved": round(achieved_tflops, 3) if achieved_tflops else None,
+        "roofline": {"bound": "fp64-valu", "achieved": round(achieved_tflops, 3) if achieved_tflops else None,
                      "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(achieved_tflops / FP64_PEAK_TFLOPS, 4) if achieved_tflops else None,
                      "traffic": profiled_traffic() if B == 4096 else None,
                      "kernel": "br4_kernel (blind rotation, FP64); traffic from profiles/br4_kernel_p128_batch4096.md",
-                     "note": "FP64 peak: MI355X vector and matrix FP64 are both 78.6 TFLOP/s"},
+                     "note": "the blind rotation runs on the FP64 vector ALUs (no matrix-core instruction); "
+                             "MI355X FP64 vector peak 78.6 TFLOP/s"},
         "hbm_views": hbm,
     }
+    # everything this run put on the device is released before the line is printed;
+    # the only child processes are the reference harness runs of cpu_baseline, each
+    # awaited by subprocess.run
+    for x in (d_pool, d_ops, d_i0, d_i1, d_o):
+        x.free()
+    sk.close()
+    T.synchronize()
     try:
         res["cpu_baseline"] = cpu_baseline(args.cpu_gates, args.seed)
     except Exception as e:  # the GPU result stands on its own

@@ -10,7 +10,7 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_DIR = os.path.join(ROOT, "oracle")
-ORACLE_SO = os.path.join(ORACLE_DIR, "liboracle.so")
+ORACLE_SO = os.environ.get("TFHE_AMD_ORACLE_LIB") or os.path.join(ORACLE_DIR, "liboracle.so")
 REF_DIR = os.path.join(ORACLE_DIR, "_ref")
 REF_HARNESS = os.path.join(REF_DIR, "ref_harness")
 

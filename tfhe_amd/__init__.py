@@ -11,7 +11,9 @@ import os
 import numpy as np
 
 PKG = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(PKG, "libtfhe_amd.so")
+# TFHE_AMD_LIB selects another build of the same library (the sanitizer build of
+# the host code, tfhe_amd/_build_san/libtfhe_amd.so, for the CPU tests)
+LIB_PATH = os.environ.get("TFHE_AMD_LIB") or os.path.join(PKG, "libtfhe_amd.so")
 
 GATES = ["NAND", "AND", "OR", "XOR", "XNOR", "NOR", "ANDNY", "ANDYN", "ORNY", "ORYN", "MUX"]
 GATE_CODE = {g: i for i, g in enumerate(GATES)}

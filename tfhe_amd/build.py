@@ -55,5 +55,36 @@ def build(verbose_resources=False):
     return OUT
 
 
+SAN = os.path.join(PKG, "_build_san")
+SAN_FLAGS = ["-fsanitize=address,undefined", "-fno-sanitize-recover=undefined", "-fno-omit-frame-pointer", "-g", "-O1"]
+
+
+def build_sanitized():
+    """The same library with its host C++ (host.cpp, io.cpp) built under
+    AddressSanitizer and UndefinedBehaviorSanitizer by g++, linked by g++ against
+    the HIP objects of the normal build (GPU code is not instrumented), into
+    tfhe_amd/_build_san/libtfhe_amd.so; and the CPU oracle likewise into
+    tfhe_amd/_build_san/liboracle.so. Load them with TFHE_AMD_LIB /
+    TFHE_AMD_ORACLE_LIB and gcc's libasan and libubsan preloaded."""
+    os.makedirs(SAN, exist_ok=True)
+    objs = [os.path.join(BUILD, f) for f in ("engine.o", "netlist.o", "serve.o", "client.o")]
+    if not all(os.path.exists(o) for o in objs):
+        build()
+    host_o, io_o = os.path.join(SAN, "host.o"), os.path.join(SAN, "io.o")
+    _run([CXX] + HOST_FLAGS + SAN_FLAGS + ["-c", os.path.join(CSRC, "host.cpp"), "-o", host_o])
+    _run([CXX] + HOST_FLAGS + SAN_FLAGS + ["-c", os.path.join(CSRC, "io.cpp"), "-o", io_o])
+    rocm = os.path.dirname(os.path.dirname(os.path.realpath(HIPCC)))
+    _run([CXX, "-shared", "-fPIC"] + SAN_FLAGS + ["-o", os.path.join(SAN, "libtfhe_amd.so")] + objs +
+         [host_o, io_o, "-L" + os.path.join(rocm, "lib"), "-lamdhip64", "-Wl,-rpath," + os.path.join(rocm, "lib"),
+          "-lpthread"])
+    _run([CXX, "-std=c++17", "-fPIC", "-shared"] + SAN_FLAGS + ["-o", os.path.join(SAN, "liboracle.so"),
+                                                                os.path.join(ROOT, "oracle", "tfhe_oracle.cpp"),
+                                                                "-lpthread"])
+    return SAN
+
+
 if __name__ == "__main__":
-    build(verbose_resources="-v" in sys.argv)
+    if "--sanitize" in sys.argv:
+        build_sanitized()
+    else:
+        build(verbose_resources="-v" in sys.argv)
