@@ -211,6 +211,82 @@ struct EmuCT {
     }
 };
 
+// N = 2048 on a pair of wavefronts with fused butterflies (FftCTPair): X[h*64 + lane][slot]
+struct EmuCTPair {
+    using F = FftCT;
+    std::vector<c64> tw;
+    EmuCTPair() : tw(FftCTPair::COUNT) { build_twiddles_ct_pair(tw.data()); }
+    void lane_tab(int h, int off, int lane, ct64* w, int n) const {
+        const c64* t = tw.data() + h * FftCT::COUNT;
+        for (int k = 0; k < n; k++) w[k] = ct64{t[off + 64 * k + lane].re, t[off + 64 * k + lane].im};
+    }
+    void forward(const double* p, c64 (*x)[8]) {  // x = X[2*64][8]
+        c64 u[2][64][8];
+        for (int h = 0; h < 2; h++)
+            for (int b = 0; b < 64; b++)
+                for (int a = 0; a < 8; a++) {
+                    const int j = 512 * h + 64 * a + b;
+                    u[h][b][a] = c64{p[j], p[j + 1024]};
+                }
+        for (int b = 0; b < 64; b++) {
+            for (int a = 0; a < 8; a++) x[b][a] = u[0][b][a], x[64 + b][a] = u[1][b][a];
+            FftPair::pair_fwd<0>(x[b], u[1][b]);
+            FftPair::pair_fwd<1>(x[64 + b], u[0][b]);
+        }
+        for (int h = 0; h < 2; h++) {
+            std::vector<c64> lds(F::LDS_C64);
+            c64 (*xh)[8] = x + 64 * h;
+            for (int l = 0; l < 64; l++) {
+                if (h == 0) FftCTPair::fwd1<0>(xh[l], l, lds.data());
+                else FftCTPair::fwd1<1>(xh[l], l, lds.data());
+            }
+            for (int l = 0; l < 64; l++) {
+                ct64 w[4];
+                lane_tab(h, FftCT::FB, l, w, 4);
+                F::fwd2(xh[l], l, lds.data(), w);
+            }
+            for (int l = 0; l < 64; l++) F::fwd3(xh[l], l, lds.data());
+            for (int l = 0; l < 64; l++) {
+                ct64 w[4];
+                lane_tab(h, FftCT::FC, l, w, 4);
+                F::fwd4(xh[l], l, lds.data(), w);
+            }
+        }
+    }
+    void inverse(c64 (*x)[8], double* p) {
+        for (int h = 0; h < 2; h++) {
+            std::vector<c64> lds(F::LDS_C64);
+            c64 (*xh)[8] = x + 64 * h;
+            for (int l = 0; l < 64; l++) F::inv1(xh[l], l, lds.data());
+            for (int l = 0; l < 64; l++) {
+                ct64 w[4];
+                lane_tab(h, FftCT::IB, l, w, 4);
+                F::inv2(xh[l], l, lds.data(), w);
+            }
+            for (int l = 0; l < 64; l++) F::inv3(xh[l], l, lds.data());
+            for (int l = 0; l < 64; l++) {
+                ct64 w[4];
+                c64 ut[8];
+                lane_tab(h, FftCT::IA, l, w, 4);
+                for (int a = 0; a < 8; a++) ut[a] = tw[h * FftCT::COUNT + FftCT::UT + 64 * a + l];
+                F::inv4(xh[l], l, lds.data(), w, ut);
+            }
+        }
+        for (int b = 0; b < 64; b++) {
+            c64 y0[8], y1[8], u0[8], u1[8];
+            for (int a = 0; a < 8; a++) y0[a] = x[b][a], y1[a] = x[64 + b][a];
+            for (int a = 0; a < 8; a++) u0[a] = y0[a], u1[a] = y1[a];
+            FftPair::pair_inv<0>(u0, y1);
+            FftPair::pair_inv<1>(u1, y0);
+            for (int a = 0; a < 8; a++) {
+                const int j0 = 64 * a + b, j1 = 512 + 64 * a + b;
+                p[j0] = u0[a].re, p[j0 + 1024] = u0[a].im;
+                p[j1] = u1[a].re, p[j1 + 1024] = u1[a].im;
+            }
+        }
+    }
+};
+
 template <class Em, int M, int SLOTS>
 int run_with(Em& E, int trials, unsigned seed, int dbits);
 
@@ -336,13 +412,14 @@ int run_mac(Em& E, int kpl, int trials, unsigned seed, int dbits, int worst) {
 
 int main(int argc, char** argv) {
     if (argc > 1 && std::string(argv[1]) == "mac") {
-        // fft_emu mac N KPL TRIALS SEED DIGIT_BITS WORST   (N = 1: N = 1024 fused; -2048: pair)
+        // fft_emu mac N KPL TRIALS SEED DIGIT_BITS WORST   (N = 1: N = 1024 fused; -2048: pair; 2: pair fused)
         const int N = atoi(argv[2]), kpl = atoi(argv[3]), trials = atoi(argv[4]), dbits = atoi(argv[6]);
         const unsigned seed = atoi(argv[5]);
         const int worst = atoi(argv[7]);
         if (N == 1) { EmuCT E; return run_mac<EmuCT, 512, 8>(E, kpl, trials, seed, dbits, worst); }
         if (N == 1024) { Emu<512> E; return run_mac<Emu<512>, 512, 8>(E, kpl, trials, seed, dbits, worst); }
         if (N == -2048) { EmuPair E; return run_mac<EmuPair, 1024, 8>(E, kpl, trials, seed, dbits, worst); }
+        if (N == 2) { EmuCTPair E; return run_mac<EmuCTPair, 1024, 8>(E, kpl, trials, seed, dbits, worst); }
         if (N == -1024) { EmuHalf E; return run_mac<EmuHalf, 512, 4>(E, kpl, trials, seed, dbits, worst); }
         return 2;
     }
@@ -355,6 +432,10 @@ int main(int argc, char** argv) {
     if (N == -2048) {  // pair-of-wavefronts transform
         EmuPair E;
         return run_with<EmuPair, 1024, 8>(E, trials, seed, dbits);
+    }
+    if (N == 2) {  // N = 2048, pair of wavefronts, fused butterflies
+        EmuCTPair E;
+        return run_with<EmuCTPair, 1024, 8>(E, trials, seed, dbits);
     }
     if (N == 1) {  // N = 1024, fused butterflies
         EmuCT E;

@@ -250,11 +250,12 @@ def fft_emu(tmp_path_factory):
     return exe
 
 
-@pytest.mark.parametrize("N,bits", [(1024, 7), (2048, 10), (-2048, 10), (-1024, 7), (1, 7), (1, 10)])
+@pytest.mark.parametrize("N,bits", [(1024, 7), (2048, 10), (-2048, 10), (-1024, 7), (1, 7), (1, 10), (2, 10)])
 def test_wavefront_transform_emulation(fft_emu, N, bits):
     """The GPU transform's per-lane code, run lane by lane on the CPU, gives the
     exact negacyclic product after rounding (digit x torus polynomials). N = 1 is
-    the N = 1024 transform with fused butterflies (FftCT)."""
+    the N = 1024 transform with fused butterflies (FftCT), N = 2 the N = 2048
+    pair-of-wavefronts transform with fused butterflies (FftCTPair)."""
     out = subprocess.run([fft_emu, str(N), "6", "7", str(bits)], capture_output=True, text=True, check=True).stdout
     f = out.split()
     assert f[0] == "maxerr" and float(f[1]) < 0.1
@@ -267,6 +268,7 @@ def test_wavefront_transform_emulation(fft_emu, N, bits):
     (1024, 6, 7, 1, True), (-1024, 6, 7, 1, True),   # P128, radix-8 and half transforms, worst case
     (-2048, 4, 10, 0, True),                         # P_big, pair transform: typical
     (-2048, 4, 10, 1, False),                        # P_big worst case: |sum| = 2^53, past FP64 exactness
+    (2, 4, 10, 0, True), (2, 4, 10, 1, False),       # P_big, pair transform with fused butterflies
 ])
 def test_transform_exactness_bound(fft_emu, N, kpl, bits, worst, exact):
     """Where the FP64 product stops being exact. The kernels sum kpl digit x key
@@ -276,7 +278,7 @@ def test_transform_exactness_bound(fft_emu, N, kpl, bits, worst, exact):
     signs aligned on one output coefficient (worst = 1) P128 reaches 2^49.6 and
     is still exact (within 0.13), while P_big reaches 2^53 and is not. The key
     is uniformly random and secret, so that alignment has probability about
-    2^-N; see the DESIGN note on the exactness bound."""
+    2^-N."""
     out = subprocess.run([fft_emu, "mac", str(N), str(kpl), "2", "3", str(bits), str(worst)], capture_output=True,
                          text=True, check=True).stdout.split()
     err, miss, mag = float(out[1]), int(out[3]), float(out[5])

@@ -718,14 +718,19 @@ struct FftCT {
     }
 };
 
-// Per-lane tables of FftCT, [k][lane], built in long double and rounded once:
-// FB (forward levels 4-6, lane b0 + 8a): psi^{E/2}, psi^{E/4}, psi^{E/8}, psi^{E/8+256}
-//    with psi^E the root of block a after three levels;
+// Per-lane tables of the fused-butterfly transform of a polynomial modulo
+// X^512 - psi^E0, [k][lane], built in long double and rounded once (psi =
+// exp(i pi / (half)), a half turn being `half` units; a square root of psi^e is
+// psi^{e/2} or psi^{e/2 + half}):
+// FB (forward levels 4-6, lane b0 + 8a): psi^{E/2}, psi^{E/4}, psi^{E/8}, psi^{E/8+Q}
+//    with psi^E the root of block a after three levels and Q a quarter of `half`
+//    (psi^Q = e^{i pi/4});
 // FC (levels 7-9, lane q + 8a): the same with the root of block (a, q) after six;
 // IB (spans 8-32, p = lane & 7): w16^-p, w32^-p, w64^-p, e^{-i pi/4} w64^-p;
 // IA (spans 64-256, b = lane): w128^-b, w256^-b, w512^-b, e^{-i pi/4} w512^-b
-//    (w_L = exp(2 pi i / L)); UT [a][lane]: psi^{-(64a+b)} / 512 as c64.
-inline void build_twiddles_ct(c64* out) {
+//    (w_L = exp(2 pi i / L)); UT [a][lane]: psi^{-(E0/512)(64a+b)} * scale as c64.
+// FftCT (N = 1024): E0 = 512, half = 1024 (psi^E0 = i), scale 1/512.
+inline void build_twiddles_ct_base(c64* out, int E0, int half, long double scale) {
     const long double pi = 3.141592653589793238462643383279502884L;
     // (cos, tan) of ang as two doubles in a c64 slot. At odd multiples of pi/2
     // (w = +-i, tan infinite) w is represented as 2^-60 (1 + i (+-2^60)): off by
@@ -737,16 +742,17 @@ inline void build_twiddles_ct(c64* out) {
         if (fabsl(c) < 1e-12L) return c64{0x1p-60, (double)(s > 0 ? 0x1p60L : -0x1p60L)};
         return c64{(double)c, (double)(s / c)};
     };
-    auto root = [](int bits, int path) {  // exponent e of psi^e for a block after `bits` levels
-        int e = 512;
-        for (int k = bits - 1; k >= 0; k--) e = e / 2 + 1024 * ((path >> k) & 1);
+    auto root = [&](int bits, int path) {  // exponent e of psi^e for a block after `bits` levels
+        int e = E0;
+        for (int k = bits - 1; k >= 0; k--) e = e / 2 + half * ((path >> k) & 1);
         return e;
     };
-    const long double ps = pi / 1024.0L;
+    const long double ps = pi / half;
+    const int Q = half / 4, tw = E0 / 512;
     for (int lane = 0; lane < 64; lane++) {
         const int a = lane >> 3, q = lane & 7;
         const int E = root(3, a), E6 = root(6, 8 * a + q);
-        const int fb[4] = {E / 2, E / 4, E / 8, E / 8 + 256}, fc[4] = {E6 / 2, E6 / 4, E6 / 8, E6 / 8 + 256};
+        const int fb[4] = {E / 2, E / 4, E / 8, E / 8 + Q}, fc[4] = {E6 / 2, E6 / 4, E6 / 8, E6 / 8 + Q};
         for (int k = 0; k < 4; k++) {
             out[FftCT::FB + 64 * k + lane] = ct(ps * fb[k]);
             out[FftCT::FC + 64 * k + lane] = ct(ps * fc[k]);
@@ -761,10 +767,75 @@ inline void build_twiddles_ct(c64* out) {
         out[FftCT::IA + 2 * 64 + lane] = ct(-2 * pi * b / 512);
         out[FftCT::IA + 3 * 64 + lane] = ct(-2 * pi * b / 512 - pi / 4);
         for (int aa = 0; aa < 8; aa++) {
-            const long double ang = -ps * (64 * aa + b);
-            out[FftCT::UT + 64 * aa + lane] = c64{(double)(cosl(ang) / 512.0L), (double)(sinl(ang) / 512.0L)};
+            const long double ang = -ps * (long double)((tw * (64 * aa + b)) % (2 * half));
+            out[FftCT::UT + 64 * aa + lane] = c64{(double)(cosl(ang) * scale), (double)(sinl(ang) * scale)};
         }
     }
+}
+inline void build_twiddles_ct(c64* out) { build_twiddles_ct_base(out, 512, 1024, 1.0L / 512); }
+
+// N = 2048 on a pair of wavefronts with the fused butterflies. With u_j = p_j +
+// i p_{j+1024} (j < 1024) the folded polynomial is taken modulo X^1024 - i
+// (psi = exp(i pi / 2048), i = psi^1024). The first level splits it into
+// X^512 - zeta_h, zeta_0 = e^{i pi/4} = psi^512, zeta_1 = -zeta_0 = psi^2560:
+//   y_h[j'] = u_{j'} + (-1)^h e^{i pi/4} u_{j'+512}   (FftPair::pair_fwd),
+// and wavefront h runs the nine remaining levels on y_h, modulo X^512 - zeta_h,
+// exactly as FftCT does modulo X^512 - i but with the roots of zeta_h: stage A
+// with compile-time roots per h, then FftCT's stages with the h tables
+// (build_twiddles_ct_base with E0 = 512 + 2048 h, half = 2048). The inverse is
+// FftCT's cyclic inverse, whose untwist psi^{-(1+4h) j} / 1024 also halves y_h
+// for the pair step u_{j'} = y0 + y1, u_{j'+512} = (y0 - y1) e^{-i pi/4}.
+struct FftCTPair {
+    static constexpr int R = 8;
+    static constexpr int COUNT = 2 * FftCT::COUNT;  // [h][FftCT table layout]
+    // (cos, tan) of pi/8, pi/16, -3 pi/16, pi/32, 5 pi/32, 3 pi/32, 7 pi/32
+    static constexpr double C8 = FftCT::C8, T8 = FftCT::T8, C16 = FftCT::C16, T16 = FftCT::T16;
+    static constexpr double C316 = FftCT::C316, T316 = FftCT::T316;
+    static constexpr double C132 = 0.99518472667219692873, T132 = 0.098491403357164247767;
+    static constexpr double C532 = 0.88192126434835504956, T532 = 0.53451113595079169372;
+    static constexpr double C332 = 0.95694033573220882438, T332 = 0.30334668360734240444;
+    static constexpr double C732 = 0.77301045336273699338, T732 = 0.8206787908286603539;
+
+    // levels 2-4 of the whole transform (1-3 of wavefront h) on slots 0..7:
+    // h = 0: roots psi^256 = e^{i pi/8}; e^{i pi/16}, i e^{i pi/16};
+    //        e^{i pi/32}, i e^{i pi/32}, e^{9 i pi/32} = i e^{-7 i pi/32}, -e^{-7 i pi/32}
+    // h = 1: e^{5 i pi/8} = i e^{i pi/8}; e^{5 i pi/16} = i e^{-3 i pi/16}, -e^{-3 i pi/16};
+    //        e^{5 i pi/32}, i e^{5 i pi/32}, e^{13 i pi/32} = i e^{-3 i pi/32}, -e^{-3 i pi/32}
+    template <int H>
+    TA_HD static void stageA(c64* x) {
+        if (H == 0) {
+#pragma unroll
+            for (int a = 0; a < 4; a++) ctb<0>(x[a], x[a + 4], C8, T8);
+            ctb<0>(x[0], x[2], C16, T16);
+            ctb<0>(x[1], x[3], C16, T16);
+            ctb<1>(x[4], x[6], C16, T16);
+            ctb<1>(x[5], x[7], C16, T16);
+            ctb<0>(x[0], x[1], C132, T132);
+            ctb<1>(x[2], x[3], C132, T132);
+            ctb<1>(x[4], x[5], C732, -T732);
+            ctb<3>(x[6], x[7], C732, -T732);
+        } else {
+#pragma unroll
+            for (int a = 0; a < 4; a++) ctb<1>(x[a], x[a + 4], C8, T8);
+            ctb<1>(x[0], x[2], C316, T316);
+            ctb<1>(x[1], x[3], C316, T316);
+            ctb<3>(x[4], x[6], C316, T316);
+            ctb<3>(x[5], x[7], C316, T316);
+            ctb<0>(x[0], x[1], C532, T532);
+            ctb<1>(x[2], x[3], C532, T532);
+            ctb<1>(x[4], x[5], C332, -T332);
+            ctb<3>(x[6], x[7], C332, -T332);
+        }
+    }
+    template <int H>
+    TA_HD static void fwd1(c64* x, int b, c64* lds) {
+        stageA<H>(x);
+#pragma unroll
+        for (int a = 0; a < R; a++) lds[FftCT::a1(a, b)] = x[a];
+    }
+};
+inline void build_twiddles_ct_pair(c64* out) {
+    for (int h = 0; h < 2; h++) build_twiddles_ct_base(out + h * FftCT::COUNT, 512 + 2048 * h, 2048, 1.0L / 1024);
 }
 
 // Twiddles of the half transform: [h][lane][10]: tw1_h[k1] = psi^{(1+4h) b} w_256^{b k1}
