@@ -322,6 +322,18 @@ def bench_pbig(T, args):
                                                        "max": int(scaled.max())}),
            "batch_8192": timed([8192]),
            "decrypt_errors": errors}
+    if not args.no_compare_modes:
+        # the same 8192-gate launch on the pair kernel's radix-8 form (mode 8) and again on
+        # the default fused-butterfly form, alternating, in this run
+        cmp = {"radix8_transform": [], "fused_butterfly_transform": []}
+        for _ in range(2):
+            for mode, name in ((8, "radix8_transform"), (0, "fused_butterfly_transform")):
+                T.set_launch_config(mode)
+                try:
+                    cmp[name].append(timed([8192])["gates_per_s"])
+                finally:
+                    T.set_launch_config(0)
+        out["batch_8192_kernel_variants_gates_per_s"] = cmp
     run(scaled, streams)  # warm-up of the streams' workspaces
     out["drawn_4_streams"] = timed_streams(drawn)
     out["scaled_4_streams"] = timed_streams(scaled)

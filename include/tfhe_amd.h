@@ -493,7 +493,9 @@ TFHE_AMD_EXPORT void tfhe_amd_server_destroy(tfhe_amd_server* srv);
  * transform), 7 = the same kernel on the radix-8 transform (kept for
  * comparison), 3 = two wavefronts per bootstrap, 5 = 4l wavefronts per
  * bootstrap (lowest latency; modes 2, 3, 5, 7 for N = 1024), 4 = the N = 2048
- * pair-transform kernel (the automatic choice for N = 2048). All modes give
+ * pair-transform kernel on the fused-butterfly transform (the automatic choice
+ * for N = 2048), 8 = the same kernel on the radix-8 stages (kept for
+ * comparison; N = 2048 only, N = 1024 runs mode 2 for it). All modes give
  * bit-identical results. In automatic mode, for N = 1024, batches of
  * at most latency_max gates use mode 5, then up to small_batch_max mode 3,
  * larger ones mode 2 (a negative value keeps the current one). */

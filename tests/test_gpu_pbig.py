@@ -113,6 +113,33 @@ def test_pbig_pair_kernel_matches_one_wave_kernel(pbig):
     assert np.array_equal(sk.decrypt(pair), exp)
 
 
+@pytest.mark.parametrize("count", [1, 37, 600])
+def test_pbig_pair_kernel_forms_bitexact(pbig, count):
+    """The pair kernel on the fused-butterfly transform (the default, mode 4) and
+    on the radix-8 stages (mode 8) give identical ciphertexts, equal to the
+    oracle's (first gates) and decrypting to the plaintext gates; 600 gates with
+    MUX take more workgroups than the GPU holds at once."""
+    T, P, sk, keys = pbig
+    rng = np.random.default_rng(47 + count)
+    ops = rng.integers(0, 11, size=count).astype(np.int32)
+    ops[:min(3, count)] = 10
+    bits = rng.integers(0, 2, size=(3, count)).astype(np.int32)
+    a, b, c = (sk.encrypt(x) for x in bits)
+    print(f"[test] N=2048 pair kernel forms, {count} gates", flush=True)
+    got = {}
+    for mode in (4, 8):
+        T.set_launch_config(mode)
+        try:
+            got[mode] = T.run_gates(sk.cloud, ops, a, b, c)
+        finally:
+            T.set_launch_config(0)
+    assert np.array_equal(got[4], got[8])
+    k = min(count, 6)
+    assert np.array_equal(got[4][:k], O.gates(keys, ops[:k], a[:k], b[:k], c[:k], threads=16))
+    exp = np.array([O.TRUTH[O.GATES[o]](x, y, z) for o, x, y, z in zip(ops, *bits)])
+    assert np.array_equal(sk.decrypt(got[4]), exp)
+
+
 @pytest.fixture(scope="module")
 def n2048_l3(gpu):
     """N = 2048 with three gadget digits (n = 630, Bgbit = 7, P128's noise): the l = 3
@@ -128,9 +155,9 @@ def n2048_l3(gpu):
 
 
 def test_n2048_l3_pair_kernel_bitexact(n2048_l3):
-    """Every gate type (and extra MUX, odd batch) through the l = 3 pair kernel:
-    bit-exact against the oracle and the one-wavefront kernel, and decrypting to
-    the plaintext gate."""
+    """Every gate type (and extra MUX, odd batch) through the l = 3 pair kernel
+    (fused-butterfly and radix-8 forms): bit-exact against the oracle and the
+    one-wavefront kernel, and decrypting to the plaintext gate."""
     T, P, sk, keys = n2048_l3
     rng = np.random.default_rng(46)
     ops = np.array(list(range(11)) + [10, 10], np.int32)
@@ -142,7 +169,13 @@ def test_n2048_l3_pair_kernel_bitexact(n2048_l3):
         one = T.run_gates(sk.cloud, ops, a, b, c)
     finally:
         T.set_launch_config(0)
+    T.set_launch_config(8)
+    try:
+        radix8 = T.run_gates(sk.cloud, ops, a, b, c)
+    finally:
+        T.set_launch_config(0)
     assert np.array_equal(pair, one)
+    assert np.array_equal(pair, radix8)
     assert np.array_equal(pair, O.gates(keys, ops, a, b, c, threads=16))
     exp = np.array([O.TRUTH[O.GATES[o]](x, y, z) for o, x, y, z in zip(ops, *bits)])
     assert np.array_equal(sk.decrypt(pair), exp)

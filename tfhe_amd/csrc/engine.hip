@@ -61,6 +61,8 @@ struct tfhe_amd_device_key {
     int8_t* kp;    // t = 8, basebit = 2: keyswitch key as signed byte planes [4][k*N][npad][32] (int8 matrix cores)
     c64* bkc;      // M = 512 only: [n][kpl][k+1][M] in the fused-butterfly transform's order (FftCT)
     c64* twc;      // M = 512 only: FftCT::COUNT per-lane tables
+    c64* bkq;      // M = 1024 only: like bkp, in the pair transform with fused butterflies (FftCTPair)
+    c64* twq;      // M = 1024 only: FftCTPair::COUNT per-lane tables
     size_t bytes;
     uint64_t gen;  // unique per upload: a re-uploaded key never compares equal to a freed one
     int pins;      // holders besides the host key (servers); the memory is freed when the last one lets go
@@ -108,12 +110,16 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_wave_barrier();
 }
 
-// The same pointer, opaque to the optimiser (an empty asm that "modifies" it):
-// loads through it are not hoisted out of loops.
-template <class T>
-__device__ __forceinline__ const T* opaque(const T* p) {
-    asm volatile("" : "+s"(p));
-    return p;
+// Zero, opaque to the optimiser (an empty asm that "modifies" it): loads at
+// p + opaque_zero() are not hoisted out of loops. The offset, not the pointer,
+// goes through the asm, so the pointer keeps the global address space the
+// compiler infers from the kernel argument and the loads stay global_load (a
+// flat load would also count against the LDS wait counter, so every LDS wait
+// after it would wait for the load as well).
+__device__ __forceinline__ int opaque_zero() {
+    int z = 0;
+    asm volatile("" : "+s"(z));
+    return z;
 }
 
 // round to nearest, result mod 2^32 (valid for |v| < 2^51)
@@ -495,11 +501,11 @@ __device__ __forceinline__ void br4_body(BrArgs A, const int32_t* __restrict__ m
         }
         // inverse transforms and ACC += result (lwe-bootstrapping-functions-fft.cpp:67-76);
         // FftCT: the inverse tables, loaded once per key row for both components
-        // (through an opaque pointer, so not hoisted out of the row loop)
+        // (at an opaque offset, so not hoisted out of the row loop)
         ct64 iwb[4], iwa[4];
         c64 iut[R];
         if constexpr (CT) {
-            const ct64* twc = opaque(twc_base) + lane;
+            const ct64* twc = twc_base + opaque_zero() + lane;
 #pragma unroll
             for (int k = 0; k < 4; k++) {
                 iwb[k] = twc[G::IB + 64 * k];
@@ -1089,23 +1095,31 @@ constexpr int BRP_NMAX = 768;
 // both wavefronts of the pair form the whole update (the partner's old
 // quarters come through s_xacc). Four workgroup barriers per key row; the two
 // workgroups on a CU never wait for each other.
-template <int L>
+// CT: the transform with fused butterflies (FftCTPair; key in its order, twp
+// its tables: the forward ones staged in LDS where s_tw1 was, the inverse ones
+// read once per key row) instead of the radix-8 stages of Fft<512>.
+template <int L, bool CT = false>
 __global__ __launch_bounds__(256, 2) void brp_kernel(BrArgs A, const c64* __restrict__ bkp, const c64* __restrict__ twp,
                                                     const int32_t* __restrict__ mux_gate,
                                                     const int32_t* __restrict__ n_jobs) {
     using F = Fft<512>;
     using P = FftPair;
+    using G = FftCT;
     constexpr int R = 8, N = 2048, KPL = 2 * L, LOG2N2 = 12;
     constexpr int NMAX = BRP_NMAX;
     __shared__ c64 s_fft[4][F::LDS_C64];
     __shared__ int32_t s_xacc[4][2 * R * 64];  // wavefront w's old quarters of ACC_c, for its partner
-    __shared__ c64 s_tw1[2][R * 64];           // [h][k1 * 64 + lane]
+    // radix-8: [h][k1 * 64 + lane] stage-1 twiddles; CT: [h][k * 64 + lane] the FB (k < 4)
+    // and FC (k >= 4) entries of wavefront h's tables, as ct64
+    __shared__ c64 s_tw1[2][R * 64];
     __shared__ int16_t s_bara[NMAX];
     __shared__ int32_t s_barb;
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
-    const int w = tid >> 6;
+    // wave-uniform. The fused form takes it as a scalar: its c/h offsets then sit in SGPRs and
+    // it needs no spill within 256 VGPRs (the radix-8 form spills less with the vector value)
+    const int w = CT ? __builtin_amdgcn_readfirstlane(tid >> 6) : tid >> 6;
     const int c = w >> 1, h = w & 1;
     const int job = blockIdx.x;
     if (job >= *n_jobs) return;  // whole block idle (uniform)
@@ -1140,10 +1154,15 @@ __global__ __launch_bounds__(256, 2) void brp_kernel(BrArgs A, const c64* __rest
     }
     for (int k = tid; k < 2 * R * 64; k += 256) {
         const int hh = k >> 9, k1 = (k >> 6) & 7, l = k & 63;
-        s_tw1[hh][k1 * 64 + l] = twp[(hh * 64 + l) * 16 + k1];
+        if constexpr (CT)  // FB entries 0..3, then FC entries 0..3
+            s_tw1[hh][k1 * 64 + l] = twp[hh * G::COUNT + (k1 < 4 ? G::FB : G::FC - 4 * 64) + k1 * 64 + l];
+        else
+            s_tw1[hh][k1 * 64 + l] = twp[(hh * 64 + l) * 16 + k1];
     }
     const c64* tw1 = s_tw1[h] + lane;
-    const c64 tw2 = twp[(h * 64 + lane) * 16 + 9];
+    const c64 tw2 = CT ? c64{0.0, 0.0} : twp[(h * 64 + lane) * 16 + 9];
+    const ct64* const ctw = (const ct64*)s_tw1[h] + lane;  // CT: FB at [k * 64], FC at [(4 + k) * 64]
+    const c64* const twq_base = twp + h * G::COUNT;         // CT: this wavefront's tables in global memory
     const uint32_t offset = A.offset;
     const int Bgbit = A.Bgbit;
     const int32_t dhalf = 1 << (Bgbit - 1);
@@ -1253,24 +1272,50 @@ __global__ __launch_bounds__(256, 2) void brp_kernel(BrArgs A, const c64* __rest
                         dig[pp][1][a] = dig[pp + 1][1][a];
                     }
             }
-            if (h == 0) {
-                P::pair_fwd<0>(x, ov);
-                P::template fwd1<0, 64>(x, lane, fb, tw1);
+            if constexpr (CT) {
+                if (h == 0) {
+                    P::pair_fwd<0>(x, ov);
+                    FftCTPair::fwd1<0>(x, lane, fb);
+                } else {
+                    P::pair_fwd<1>(x, ov);
+                    FftCTPair::fwd1<1>(x, lane, fb);
+                }
+                wave_sync();
+                {
+                    ct64 w[4];
+#pragma unroll
+                    for (int k = 0; k < 4; k++) w[k] = ctw[k * 64];
+                    G::fwd2(x, lane, fb, w);
+                }
+                wave_sync();
+                G::fwd3(x, lane, fb);
             } else {
-                P::pair_fwd<1>(x, ov);
-                P::template fwd1<1, 64>(x, lane, fb, tw1);
+                if (h == 0) {
+                    P::pair_fwd<0>(x, ov);
+                    P::template fwd1<0, 64>(x, lane, fb, tw1);
+                } else {
+                    P::pair_fwd<1>(x, ov);
+                    P::template fwd1<1, 64>(x, lane, fb, tw1);
+                }
+                wave_sync();
+                F::fwd2(x, lane, fb, tw2);
+                wave_sync();
+                F::fwd3(x, lane, fb);
             }
-            wave_sync();
-            F::fwd2(x, lane, fb, tw2);
-            wave_sync();
-            F::fwd3(x, lane, fb);
             // key row (i, c L + p), half h: output component 0 at bk[s * 64], 1 at bk[1024 + s * 64]
             const c64* bk = bkp + ((size_t)i * KPL + c * L + p) * 2048 + h * 512 + lane;
             c64 kv[R];
 #pragma unroll
             for (int s = 0; s < R; s++) kv[s] = bk[s * 64];
             wave_sync();
-            F::fwd4(x, lane, fb);
+            if constexpr (CT) {
+                ct64 w[4];
+#pragma unroll
+                for (int k = 0; k < 4; k++) w[k] = ctw[(4 + k) * 64];
+                G::fwd4(x, lane, fb, w);
+            } else {
+                F::fwd4(x, lane, fb);
+            }
 #pragma unroll
             for (int s = 0; s < R; s++) {
                 if constexpr (FIRST) p0[s] = cmul_first(x[s], kv[s]);
@@ -1316,7 +1361,29 @@ __global__ __launch_bounds__(256, 2) void brp_kernel(BrArgs A, const c64* __rest
             }
         }
         __syncthreads();
-        {
+        if constexpr (CT) {
+            // this wavefront's inverse tables, read once per key row (opaque: kept out of
+            // the row loop, where they would hold registers for the whole row)
+            const c64* tq = twq_base + opaque_zero() + lane;
+            ct64 w[4];
+            G::inv1(x, lane, fb);
+#pragma unroll
+            for (int k = 0; k < 4; k++) w[k] = ((const ct64*)tq)[G::IB + 64 * k];
+            wave_sync();
+            G::inv2(x, lane, fb, w);
+            wave_sync();
+            G::inv3(x, lane, fb);
+            c64 ut[R];
+#pragma unroll
+            for (int k = 0; k < 4; k++) w[k] = ((const ct64*)tq)[G::IA + 64 * k];
+#pragma unroll
+            for (int k = 0; k < R; k++) ut[k] = tq[G::UT + 64 * k];
+            wave_sync();
+            G::inv4(x, lane, fb, w, ut);
+            wave_sync();
+#pragma unroll
+            for (int s = 0; s < R; s++) fb[s * 64 + lane] = x[s];
+        } else {
             F::inv1(x, lane, fb);
             wave_sync();
             F::inv2(x, lane, fb, tw2);
@@ -1403,6 +1470,51 @@ __global__ __launch_bounds__(128) void bk_convert_pair_kernel(const int32_t* __r
     wave_sync();
     F::fwd4(x, lane, fb);
     c64* out = bkp + poly * 1024 + h * 512;
+#pragma unroll
+    for (int s = 0; s < R; s++) out[s * 64 + lane] = x[s];
+}
+
+// The same on the pair transform with fused butterflies (FftCTPair): output
+// [poly][h][slot][lane] in that transform's order, twq its [h] tables.
+__global__ __launch_bounds__(128) void bk_convert_pair_ct_kernel(const int32_t* __restrict__ bk, c64* __restrict__ bkq,
+                                                                 const c64* __restrict__ twq) {
+    using G = FftCT;
+    using P = FftPair;
+    constexpr int R = 8;
+    __shared__ c64 s_fft[2][G::LDS_C64];
+    const int lane = threadIdx.x & 63, h = threadIdx.x >> 6;
+    const size_t poly = blockIdx.x;
+    const int32_t* p = bk + poly * 2048;
+    c64* fb = s_fft[h];
+    const ct64* tw = (const ct64*)(twq + h * G::COUNT) + lane;
+    c64 x[R];
+#pragma unroll
+    for (int a = 0; a < R; a++) {
+        x[a] = c64{(double)p[512 * h + 64 * a + lane], (double)p[512 * h + 64 * a + lane + 1024]};
+        fb[a * 64 + lane] = x[a];
+    }
+    __syncthreads();
+    c64 ov[R];
+#pragma unroll
+    for (int a = 0; a < R; a++) ov[a] = s_fft[h ^ 1][a * 64 + lane];
+    if (h == 0) P::pair_fwd<0>(x, ov);
+    else P::pair_fwd<1>(x, ov);
+    __syncthreads();
+    ct64 wb[4], wc[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        wb[k] = tw[G::FB + 64 * k];
+        wc[k] = tw[G::FC + 64 * k];
+    }
+    if (h == 0) FftCTPair::fwd1<0>(x, lane, fb);
+    else FftCTPair::fwd1<1>(x, lane, fb);
+    wave_sync();
+    G::fwd2(x, lane, fb, wb);
+    wave_sync();
+    G::fwd3(x, lane, fb);
+    wave_sync();
+    G::fwd4(x, lane, fb, wc);
+    c64* out = bkq + poly * 1024 + h * 512;
 #pragma unroll
     for (int s = 0; s < R; s++) out[s * 64 + lane] = x[s];
 }
@@ -2001,9 +2113,11 @@ int gates_device_locked(Ctx& C, tfhe_amd_device_key* dk, int count, const int32_
     // kernel choice: 1 = one wavefront per bootstrap (accumulator in LDS), 2 = one per workgroup of one wavefront,
     // 3 = two wavefronts per bootstrap, 5 = 4l wavefronts per bootstrap (half transforms); 0 = automatic
     // (5 for the smallest batches, 3 while the batch leaves CUs idle, 2 for large
-    // batches). N = 2048 uses the pair-transform kernel (4) unless mode 1 is forced.
+    // batches). N = 2048 uses the pair-transform kernel (4: fused butterflies, 8: radix-8 stages)
+    // unless mode 1 is forced.
     int mode = C.br_mode;
-    if (dk->M != 512) mode = (mode == 1 || dk->P.n > BRP_NMAX || !dk->bkp) ? 1 : 4;
+    if (dk->M != 512) mode = (mode == 1 || dk->P.n > BRP_NMAX || !dk->bkp) ? 1 : (mode == 8 || !dk->bkq) ? 8 : 4;
+    else if (mode == 8) mode = 2;
     else if (mode == 4) mode = 2;
     if (mode == 3 && dk->P.n > BR2_NMAX) mode = 2;
     if (mode == 0) {
@@ -2019,6 +2133,10 @@ int gates_device_locked(Ctx& C, tfhe_amd_device_key* dk, int count, const int32_
         else hipLaunchKernelGGL(brl_kernel<2>, dim3(blocks), dim3(512), 0, s, A, dk->bkh, dk->twh, mux_gate, n_jobs);
     } else if (mode == 4) {
         const int blocks = 2 * count;  // one bootstrap per workgroup, at most 2*count jobs
+        if (dk->P.l == 2) hipLaunchKernelGGL((brp_kernel<2, true>), dim3(blocks), dim3(256), 0, s, A, dk->bkq, dk->twq, mux_gate, n_jobs);
+        else hipLaunchKernelGGL((brp_kernel<3, true>), dim3(blocks), dim3(256), 0, s, A, dk->bkq, dk->twq, mux_gate, n_jobs);
+    } else if (mode == 8) {
+        const int blocks = 2 * count;  // the same kernel on the radix-8 stages (for comparison)
         if (dk->P.l == 2) hipLaunchKernelGGL(brp_kernel<2>, dim3(blocks), dim3(256), 0, s, A, dk->bkp, dk->twp, mux_gate, n_jobs);
         else hipLaunchKernelGGL(brp_kernel<3>, dim3(blocks), dim3(256), 0, s, A, dk->bkp, dk->twp, mux_gate, n_jobs);
     } else if (mode == 2) {
@@ -2180,6 +2298,8 @@ static int upload_key(Ctx& C, const LweBootstrappingKeyFFT* bf, tfhe_amd_device_
     dk->kp = nullptr;
     dk->bkc = nullptr;
     dk->twc = nullptr;
+    dk->bkq = nullptr;
+    dk->twq = nullptr;
     auto fail = [&](int rc) {
         (void)hipFree(dk->bkf);
         (void)hipFree(dk->ksk);
@@ -2191,6 +2311,8 @@ static int upload_key(Ctx& C, const LweBootstrappingKeyFFT* bf, tfhe_amd_device_
         (void)hipFree(dk->kp);
         (void)hipFree(dk->bkc);
         (void)hipFree(dk->twc);
+        (void)hipFree(dk->bkq);
+        (void)hipFree(dk->twq);
         delete dk;
         return rc;
     };
@@ -2204,6 +2326,9 @@ static int upload_key(Ctx& C, const LweBootstrappingKeyFFT* bf, tfhe_amd_device_
         if (hipMalloc(&dk->bkp, bkf_bytes) != hipSuccess || hipMalloc(&dk->twp, PAIR_TW_COUNT * sizeof(c64)) != hipSuccess)
             return fail(set_error("hipMalloc(pair-transform key, %zu bytes) failed", bkf_bytes));
         dk->bytes += bkf_bytes + PAIR_TW_COUNT * sizeof(c64);
+        if (hipMalloc(&dk->bkq, bkf_bytes) != hipSuccess || hipMalloc(&dk->twq, FftCTPair::COUNT * sizeof(c64)) != hipSuccess)
+            return fail(set_error("hipMalloc(fused pair-transform key, %zu bytes) failed", bkf_bytes));
+        dk->bytes += bkf_bytes + FftCTPair::COUNT * sizeof(c64);
     }
     if (M == 512 && k == 1 && n <= BRL_NMAX) {
         if (hipMalloc(&dk->bkh, bkf_bytes) != hipSuccess || hipMalloc(&dk->twh, HALF_TW_COUNT * sizeof(c64)) != hipSuccess)
@@ -2250,6 +2375,12 @@ static int upload_key(Ctx& C, const LweBootstrappingKeyFFT* bf, tfhe_amd_device_
         build_twiddles_pair(twp.data());
         if (e == hipSuccess) e = hipMemcpyAsync(dk->twp, twp.data(), PAIR_TW_COUNT * sizeof(c64), hipMemcpyHostToDevice, C.stream);
     }
+    std::vector<c64> twq;
+    if (dk->twq) {
+        twq.resize(FftCTPair::COUNT);
+        build_twiddles_ct_pair(twq.data());
+        if (e == hipSuccess) e = hipMemcpyAsync(dk->twq, twq.data(), FftCTPair::COUNT * sizeof(c64), hipMemcpyHostToDevice, C.stream);
+    }
     std::vector<c64> twct;
     if (dk->twc) {
         twct.resize(FftCT::COUNT);
@@ -2268,6 +2399,10 @@ static int upload_key(Ctx& C, const LweBootstrappingKeyFFT* bf, tfhe_amd_device_
         e = hipGetLastError();
         if (e == hipSuccess && dk->bkp) {
             hipLaunchKernelGGL(bk_convert_pair_kernel, dim3((unsigned)npoly), dim3(128), 0, C.stream, d_coef, dk->bkp, dk->twp);
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess && dk->bkq) {
+            hipLaunchKernelGGL(bk_convert_pair_ct_kernel, dim3((unsigned)npoly), dim3(128), 0, C.stream, d_coef, dk->bkq, dk->twq);
             e = hipGetLastError();
         }
         if (e == hipSuccess && dk->bkc) {
@@ -2330,6 +2465,8 @@ static void free_key(Ctx& C, tfhe_amd_device_key* dk) {
     (void)hipFree(dk->kp);
     (void)hipFree(dk->bkc);
     (void)hipFree(dk->twc);
+    (void)hipFree(dk->bkq);
+    (void)hipFree(dk->twq);
     delete dk;
 }
 
