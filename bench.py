@@ -368,7 +368,9 @@ def bench_adders(T, sk, P, n_adders, bits, reps, seed):
     pool[zero, P.n] = -(1 << 29)  # trivial encryption of 0 as carry-in
     out = {}
     d = T.DeviceArray.from_numpy(pool)
-    for sched, name in ((0, "asap"), (1, "alap")):
+    # balanced: the as-soon-as-possible depth, levels filled to widths of the GPU's SIMD
+    # count times a power of two
+    for sched, name in ((0, "asap"), (1, "alap"), (2, "balanced")):
         n_levels, lv = levelize(C, sched)
         widths = np.bincount(lv, minlength=n_levels + 1)[1:]
         nl = Netlist(sk.cloud, C, sched)
@@ -385,7 +387,7 @@ def bench_adders(T, sk, P, n_adders, bits, reps, seed):
         got = (dec << np.arange(bits + 1)).sum(axis=1)
         out[name] = {"gates_per_s": round(len(C.op) / dt, 1), "ms_per_run": round(dt * 1e3, 3),
                      "levels": int(n_levels), "width_min": int(widths.min()), "width_max": int(widths.max()),
-                     "wrong_sums": int((got != x + y).sum())}
+                     "width_median": int(np.median(widths)), "wrong_sums": int((got != x + y).sum())}
         print(f"[bench] adders {name}: {out[name]}", file=sys.stderr, flush=True)
     d.free()
     return {"workload": f"{n_adders} independent {bits}-bit ripple-carry adders, {len(C.op)} bootstraps, "

@@ -274,9 +274,14 @@ TFHE_AMD_EXPORT int32_t tfhe_amd_keyswitch_device(const TFheGateBootstrappingClo
  * writes out[g]; each wire is written by at most one gate, wires written by no
  * gate are inputs. Gates are levelised (schedule 0: as soon as possible,
  * 1: as late as possible, except that gates whose outputs nothing reads keep
- * their earliest level); each level is one batch launch, and with
- * use_graph the whole sequence is captured once into a hipGraph and replayed
- * (the graph is re-captured when the pool or stream changes). */
+ * their earliest level; 2: balanced, the as-soon-as-possible depth with each
+ * level holding the gates whose as-late-as-possible level it is, filled with
+ * ready gates up to the smallest width base * 2^k that holds those, where base
+ * is the current GPU's SIMD count (4 per CU; 1024 without a GPU) and a MUX
+ * counts twice; a value of 64 or more: balanced with that base); each level is
+ * one batch launch, and with use_graph the whole sequence is captured once
+ * into a hipGraph and replayed (the graph is re-captured when the pool or
+ * stream changes). Other schedule values are an error. */
 typedef struct tfhe_amd_netlist tfhe_amd_netlist;
 TFHE_AMD_EXPORT tfhe_amd_netlist* tfhe_amd_netlist_create(const TFheGateBootstrappingCloudKeySet* ck, int32_t n_wires,
                                                           int32_t n_gates, const int32_t* op, const int32_t* in0,

@@ -2,7 +2,7 @@
 
 - 512 independent 32-bit ripple-carry adders (81,920 bootstraps; the reference's
   adder circuit, test/test-addition-boot.cpp:30-111), levelled as soon as
-  possible and as late as possible, run from a captured hipGraph: the level
+  possible, as late as possible and balanced, run from a captured hipGraph: the level
   widths equal the ones derived from the circuit by hand, every sum equals the
   plaintext addition, and gates of the first and of the deepest level are
   bit-exact against the CPU oracle on the GPU's own input ciphertexts.
@@ -36,6 +36,10 @@ def _expected_widths(schedule):
     their earliest level): t1_i at 2i+1, s_i, t3_i and t2_i at 2i+2, c_{i+1} at
     2i+3."""
     depth = 2 * BITS + 1
+    if schedule == 1024:
+        # balanced with base 1024: the carry chains and the gates they need fill every
+        # level but the last to 1024; the sum bits, which nothing reads, fill the last
+        return np.array([1024] * (depth - 1) + [N_ADDERS * BITS], np.int32)
     w = np.zeros(depth + 1, np.int64)
     for i in range(BITS):
         if schedule == 0:
@@ -77,7 +81,7 @@ def _check_sums(sk, out, sums, x, y):
     return int((got != x + y).sum())
 
 
-@pytest.mark.parametrize("schedule", [0, 1], ids=["asap", "alap"])
+@pytest.mark.parametrize("schedule", [0, 1, 1024], ids=["asap", "alap", "balanced"])
 def test_512_adders_32bit_full_size(adders, schedule):
     T, P, sk, C, sums, x, y, pool = adders
     from tfhe_amd.circuits import Netlist

@@ -28,8 +28,10 @@ def test_levels_respect_dependencies():
         op = int(rng.integers(0, 11))
         a, b, c = (int(v) for v in rng.choice(wires, 3))
         wires.append(C.gate(op, a, b, c if op == 10 else None))
-    for sched in (0, 1):
+    n0 = levelize(C, 0)[0]
+    for sched in (0, 1, 2, 64, 4096):
         n, lv = levelize(C, sched)
+        assert n == n0
         prod = {w: g for g, w in enumerate(C.out)}
         for g in range(len(C.op)):
             ins = [C.in0[g], C.in1[g]] + ([C.in2[g]] if C.op[g] == 10 else [])
@@ -37,6 +39,57 @@ def test_levels_respect_dependencies():
                 if w in prod:
                     assert lv[prod[w]] < lv[g]
         assert lv.min() >= 1 and lv.max() == n
+
+
+def _deadlines(C, depth):
+    """As-late-as-possible level of every gate, gates nobody reads at the last level."""
+    prod = {w: g for g, w in enumerate(C.out)}
+    users = [[] for _ in C.op]
+    for g in range(len(C.op)):
+        ins = [C.in0[g], C.in1[g]] + ([C.in2[g]] if C.op[g] == 10 else [])
+        for w in ins:
+            if w in prod:
+                users[prod[w]].append(g)
+    dl = np.full(len(C.op), depth)
+    for g in range(len(C.op) - 1, -1, -1):  # gates are appended after their inputs
+        for u in users[g]:
+            dl[g] = min(dl[g], dl[u] - 1)
+    return dl
+
+
+@pytest.mark.parametrize("base", [64, 1024, 2048])
+def test_balanced_schedule_widths(base):
+    """Balanced levels on the 512 adders and on a random circuit with MUX gates:
+    the as-soon-as-possible depth, no gate after its deadline, and every level's
+    bootstraps (MUX = 2) within the smallest base * 2^k that holds the gates whose
+    deadline it is. On the adders with base 1024 (the SIMD count of a 256-CU GPU)
+    the carry chains fill 64 levels of exactly 1024 bootstraps and the sum bits,
+    which nothing reads, fill the last level."""
+    rng = np.random.default_rng(base)
+    R = Circuit(16)
+    wires = list(range(16))
+    for _ in range(3000):
+        op = int(rng.integers(0, 11))
+        a, b, c = (int(v) for v in rng.choice(wires[-200:], 3))
+        wires.append(R.gate(op, a, b, c if op == 10 else None))
+    A = ripple_carry_adders(512, 32)[0]
+    for C in (A, R):
+        depth = levelize(C, 0)[0]
+        n, lv = levelize(C, base)
+        assert n == depth
+        dl = _deadlines(C, depth)
+        assert (lv <= dl).all()
+        wt = np.where(np.asarray(C.op) == 10, 2, 1)
+        for level in range(1, n + 1):
+            sel = lv == level
+            forced = int(wt[sel & (dl == level)].sum())
+            cap = base
+            while cap < forced:
+                cap *= 2
+            assert wt[sel].sum() <= cap, (level, forced, wt[sel].sum(), cap)
+    if base == 1024:
+        w = np.bincount(levelize(A, base)[1])[1:]
+        assert (w[:-1] == 1024).all() and w[-1] == 512 * 32
 
 
 def test_levelize_errors():
@@ -60,6 +113,11 @@ def test_levelize_errors():
     C.op[0] = 17
     with pytest.raises(RuntimeError, match="bad gate code"):
         levelize(C)
+    C = Circuit(2)
+    C.gate(0, 0, 1)
+    for bad in (-1, 3, 63):
+        with pytest.raises(RuntimeError, match="unknown schedule"):
+            levelize(C, bad)
 
 
 def test_mux_without_third_input_is_rejected():

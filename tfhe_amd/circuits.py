@@ -84,7 +84,12 @@ def ripple_carry_adders(n_adders, bits):
     return C, sums, zero
 
 
-def levelize(circuit, schedule=0):
+# level schedules of tfhe_amd_netlist_create / tfhe_amd_netlist_levelize; a
+# value of 64 or more is the balanced schedule with that base width
+ASAP, ALAP, BALANCED = 0, 1, 2
+
+
+def levelize(circuit, schedule=ASAP):
     """Host-only levelisation: (number of levels, level of every gate)."""
     L = _bind()
     op, in0, in1, in2, out = circuit.arrays()
@@ -98,7 +103,7 @@ def levelize(circuit, schedule=0):
 
 
 class Netlist:
-    def __init__(self, cloud, circuit, schedule=0):
+    def __init__(self, cloud, circuit, schedule=ASAP):
         L = _bind()
         op, in0, in1, in2, out = circuit.arrays()
         self.n_wires = circuit.n_wires

@@ -2,14 +2,16 @@
 // reference's circuits are: test/test-addition-boot.cpp:30-111,
 // test/test-long-run.cpp:61-185). A netlist is a set of bootstrapped gates
 // over wires; wires are slots of a device ciphertext pool. Gates are
-// levelised (as soon as possible, or as late as possible), each level is one
-// batch launch (blind rotation + keyswitch), and the whole sequence of
-// launches is captured once into a hipGraph and replayed.
+// levelised (as soon as possible, as late as possible, or balanced to the
+// widths the GPU runs at the same cost), each level is one batch launch
+// (blind rotation + keyswitch), and the whole sequence of launches is
+// captured once into a hipGraph and replayed.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cstring>
 #include <mutex>
+#include <queue>
 #include <vector>
 
 #include "engine.h"
@@ -32,9 +34,24 @@ struct tfhe_amd_netlist {
 namespace tfhe_amd {
 namespace {
 
+// Base width (bootstraps) of the balanced schedule: schedule values of 64 and
+// above give it directly; 2 takes the SIMD count of the current GPU (four per
+// CU), the number of bootstraps the one-wavefront batched kernel runs at one
+// wavefront per SIMD, i.e. at its shortest latency; 1024 without a GPU.
+int balanced_base(int schedule) {
+    if (schedule >= 64) return schedule;
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && cus > 0)
+        return 4 * cus;
+    (void)hipGetLastError();
+    return 1024;
+}
+
 int build_levels(tfhe_amd_netlist* nl, const int32_t* op, const int32_t* in0, const int32_t* in1,
                  const int32_t* in2, const int32_t* out, int schedule, std::vector<int32_t>& order,
                  std::vector<int32_t>& level) {
+    if (schedule < 0 || (schedule > 2 && schedule < 64)) return set_error("netlist: unknown schedule %d", schedule);
     const int G = nl->n_gates, W = nl->n_wires;
     std::vector<int32_t> producer(W, -1);
     for (int g = 0; g < G; g++) {
@@ -87,6 +104,59 @@ int build_levels(tfhe_amd_netlist* nl, const int32_t* op, const int32_t* in0, co
             for (int u : users[g]) lv = std::min(lv, level[u] - 1);
             level[g] = lv;
         }
+    } else if (schedule >= 2) {
+        // balanced: the same depth, each level filled to the width the GPU runs at the
+        // cost of the gates that must run there. A launch of up to `base` bootstraps
+        // takes one bootstrap's latency (one wavefront per SIMD), up to 2 base about
+        // 1.7 times that, and so on; so a level takes the gates whose deadline it is
+        // (the as-late-as-possible level; gates nobody reads may wait until the last
+        // level), sets its width to the smallest base * 2^k that holds them, and fills
+        // it with ready gates in order of deadline. A MUX gate counts as two
+        // bootstraps. The deadlines keep the depth: every gate's inputs have earlier
+        // deadlines, so a gate is ready at its deadline at the latest.
+        const int base = balanced_base(schedule);
+        std::vector<int32_t> deadline(G);
+        for (int q = G - 1; q >= 0; q--) {
+            const int g = queue[q];
+            int lv = depth;
+            for (int u : users[g]) lv = std::min(lv, deadline[u] - 1);
+            deadline[g] = lv;
+        }
+        auto weight = [&](int g) { return op[g] == TFHE_AMD_MUX ? 2 : 1; };
+        for (int g = 0; g < G; g++) pending[g] = 0;
+        for (int g = 0; g < G; g++)
+            for (int u : users[g]) pending[u]++;
+        // ready gates ordered by deadline (earliest first), ties by gate index
+        using Item = std::pair<int32_t, int32_t>;
+        std::priority_queue<Item, std::vector<Item>, std::greater<Item>> ready;
+        for (int g = 0; g < G; g++)
+            if (!pending[g]) ready.push({deadline[g], g});
+        std::vector<int32_t> taken;
+        for (int lv = 1; lv <= depth; lv++) {
+            taken.clear();
+            long forced = 0;
+            while (!ready.empty() && ready.top().first <= lv) {
+                const int g = ready.top().second;
+                ready.pop();
+                forced += weight(g);
+                taken.push_back(g);
+            }
+            long width = base;
+            while (width < forced) width *= 2;
+            long used = forced;
+            while (!ready.empty() && used + weight(ready.top().second) <= width) {
+                const int g = ready.top().second;
+                ready.pop();
+                used += weight(g);
+                taken.push_back(g);
+            }
+            for (int g : taken) level[g] = lv;
+            // users become ready for the next level once all their inputs are placed
+            for (int g : taken)
+                for (int u : users[g])
+                    if (--pending[u] == 0) ready.push({deadline[u], u});
+        }
+        if (!ready.empty()) return set_error("netlist: balanced schedule left gates unplaced");
     }
     order.resize(G);
     for (int g = 0; g < G; g++) order[g] = g;
