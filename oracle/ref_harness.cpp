@@ -11,7 +11,8 @@
 // Commands (all binary outputs are little-endian int32 unless stated):
 //   keys  SEED OUTDIR           dump lwe_key, tlwe_key, bk (coefficient domain), ksk
 //   gates SEED COUNT OPS OUTFILE  encrypt random bits (global RNG, in order) and run gates
-//   ks    SEED COUNT OUTFILE    keyswitch random extracted-dimension samples
+//   ks    SEED COUNT OUTFILE    keyswitch random extracted-dimension samples (input, output,
+//                               the output's current_variance as a double)
 //   bench SEED COUNT THREADS    time bootsNAND on fresh encryptions; prints one JSON line
 //   files SEED DIR COUNT        write params.gb, secret.key, cloud.key, lwekey.bin, COUNT
 //                               ciphertexts cts.bin (bits in bits.txt) and nand.bin = NAND(ct0, ct1)
@@ -362,6 +363,7 @@ int main(int argc, char** argv) {
             lweKeySwitch(out, ks, in);
             put_sample(f, in, ext->n);
             put_sample(f, out, n);
+            write_all(f, &out->current_variance, sizeof(double));
         }
         fclose(f);
         return 0;

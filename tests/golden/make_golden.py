@@ -55,9 +55,10 @@ def main():
         run("ks", 0, 4, path)
         raw = np.fromfile(path, np.int32)
         cnt, nx, n = raw[:3]
-        body = raw[3:].reshape(cnt, nx + 1 + n + 1)
+        body = raw[3:].reshape(cnt, nx + 1 + n + 1 + 2)
         np.savez_compressed(os.path.join(HERE, "ref_p128_seed0_keyswitch.npz"),
-                            inputs=body[:, :nx + 1].copy(), outputs=body[:, nx + 1:].copy())
+                            inputs=body[:, :nx + 1].copy(), outputs=body[:, nx + 1:nx + 1 + n + 1].copy(),
+                            variance=body[:, nx + 1 + n + 1:].copy().view(np.float64).reshape(cnt))
         # gates: 1000 NAND, 256 uniform over the six two-input gates, 64 MUX
         for name, count, ops in (("nand1000", 1000, "a"), ("rand6_256", 256, "rand6"), ("mux64", 64, "k")):
             path = os.path.join(tmp, name + ".bin")

@@ -240,11 +240,35 @@ def _gold(name):
 
 def test_keyswitch_matches_reference_vectors(p128):
     """Reference lweKeySwitch outputs (seed-0 keys, which this library's key
-    generation reproduces bit-exactly) on the GPU."""
+    generation reproduces bit-exactly) on the GPU, through the device API and
+    through lweKeySwitch on host samples, whose current_variance equals the
+    reference's bit for bit (it depends on the input: one addition of the key
+    rows' variance per nonzero digit)."""
     T, P, sk, keys = p128
+    L = T.lib()
     g = _gold("ref_p128_seed0_keyswitch.npz")
     got = T.keyswitch(sk.cloud, P, g["inputs"])
     assert np.array_equal(got, g["outputs"])
+    ks = sk.cloud.contents.bkFFT.contents.ks
+    ext_params = ctypes.pointer(P.ptr.contents.tgsw_params.contents.tlwe_params.contents.extracted_lweparams)
+    nx = P.k * P.N
+    assert len(set(g["variance"].tolist())) > 1
+    for r in range(len(g["inputs"])):
+        src = L.new_LweSample(ext_params)
+        dst = L.new_LweSample(P.ptr.contents.in_out_params)
+        try:
+            u = np.ascontiguousarray(g["inputs"][r])
+            ctypes.memmove(src.contents.a, u.ctypes.data, 4 * nx)
+            src.contents.b = int(u[nx])
+            L.lweKeySwitch(dst, ks, src)
+            out = np.zeros(P.n + 1, np.int32)
+            ctypes.memmove(out.ctypes.data, dst.contents.a, 4 * P.n)
+            out[P.n] = dst.contents.b
+            assert np.array_equal(out, g["outputs"][r])
+            assert dst.contents.current_variance == g["variance"][r], r
+        finally:
+            L.delete_LweSample(src)
+            L.delete_LweSample(dst)
 
 
 @pytest.mark.parametrize("name", ["nand1000", "rand6_256", "mux64"])
@@ -340,3 +364,97 @@ def test_autotune_crossovers_and_auto_mode(p128):
             assert np.array_equal(auto, forced), count
     finally:
         T.set_launch_config(0, 256, 1024)
+
+
+def _ks_variance_expected(ext, P, row_var):
+    """current_variance the reference gives a keyswitched sample: one addition of
+    the key rows' variance per nonzero digit of the input's mask
+    (libtfhe/lwe-keyswitch-functions.cpp:114-129, lwe-functions.cpp:140-145),
+    counted here from the keyswitch input rows ext [count][kN+1]."""
+    t, bb = P.ptr.contents.ks_t, P.ptr.contents.ks_basebit
+    prec = np.uint64(1 << (32 - (1 + bb * t)))
+    ab = (ext[:, :-1].astype(np.int64).astype(np.uint64) + prec) & np.uint64(0xFFFFFFFF)
+    nz = np.zeros(len(ext), np.int64)
+    for j in range(t):
+        nz += (((ab >> np.uint64(32 - (j + 1) * bb)) & np.uint64((1 << bb) - 1)) != 0).sum(axis=1)
+    out = []
+    for c in nz:
+        v = 0.0
+        for _ in range(int(c)):
+            v += row_var
+        out.append(v)
+    return np.array(out), nz
+
+
+def test_bootstrapped_current_variance_follows_keyswitch_digits(p128):
+    """Outputs of bootsGate_batch, bootsNAND/bootsMUX, tfhe_bootstrap_FFT and
+    lweKeySwitch carry the reference's current_variance: zero, plus the key
+    rows' variance once per nonzero keyswitch digit, so it depends on the
+    input; the expected counts come from the keyswitch inputs the GPU's
+    bootstrap without keyswitch produces (bit-exact with the oracle)."""
+    T, P, sk, keys = p128
+    L = T.lib()
+    rng = np.random.default_rng(31)
+    ops = np.array(list(range(11)) + [10, 3], np.int32)
+    bits, a, b, c = _encrypt_inputs(sk, rng, len(ops))
+    rows = ctypes.cast(sk.cloud.contents.bkFFT.contents.ks.contents.ks0_raw, ctypes.POINTER(T.LweSample))
+    row_var = rows[1].current_variance  # ks[0][0][1]; every row carries the same variance
+    assert row_var > 0
+    ext = T.run_gates_woks(sk.cloud, P, ops, a, b, c)
+    exp, nz = _ks_variance_expected(ext, P, row_var)
+    assert len(set(nz.tolist())) > 1  # the count depends on the input
+    n = len(ops)
+    H = T.HostCiphertexts(P, 4 * n)
+    try:
+        H.set(np.concatenate([a, b, c, np.zeros_like(a)]))
+        arr = lambda xs: (T.LSP * n)(*xs)
+        rc = L.bootsGate_batch(n, ops.ctypes.data_as(T.i32p), arr([H.item(3 * n + g) for g in range(n)]),
+                               arr([H.item(g) for g in range(n)]), arr([H.item(n + g) for g in range(n)]),
+                               arr([H.item(2 * n + g) for g in range(n)]), sk.cloud)
+        assert rc == 0, L.tfhe_amd_last_error()
+        got = np.array([H.item(3 * n + g).contents.current_variance for g in range(n)])
+        assert np.array_equal(got, exp), (got, exp)
+        # one gate per call
+        L.bootsNAND(H.item(3 * n), H.item(0), H.item(n), sk.cloud)
+        assert H.item(3 * n).contents.current_variance == exp[0]
+        L.bootsMUX(H.item(3 * n + 10), H.item(10), H.item(n + 10), H.item(2 * n + 10), sk.cloud)
+        assert H.item(3 * n + 10).contents.current_variance == exp[10]
+    finally:
+        H.close()
+    # lweKeySwitch on random inputs of the extracted dimension
+    ks = sk.cloud.contents.bkFFT.contents.ks
+    ext_params = ctypes.pointer(P.ptr.contents.tgsw_params.contents.tlwe_params.contents.extracted_lweparams)
+    nx = P.k * P.N
+    u = rng.integers(-2 ** 31, 2 ** 31, size=(2, nx + 1), dtype=np.int64).astype(np.int32)
+    u[1, :nx] = 0  # zero mask: every digit of the offset mask is zero, variance 0
+    exp_u, _ = _ks_variance_expected(u, P, row_var)
+    assert exp_u[1] == 0 and exp_u[0] > 0
+    for r in range(2):
+        src = L.new_LweSample(ext_params)
+        dst = L.new_LweSample(P.ptr.contents.in_out_params)
+        try:
+            ctypes.memmove(src.contents.a, u[r].ctypes.data, 4 * nx)
+            src.contents.b = int(u[r][nx])
+            L.lweKeySwitch(dst, ks, src)
+            assert dst.contents.current_variance == exp_u[r], r
+        finally:
+            L.delete_LweSample(src)
+            L.delete_LweSample(dst)
+    # tfhe_bootstrap_FFT: the keyswitch input is what tfhe_bootstrap_woKS_FFT returns
+    mu = 1 << 29
+    x = L.new_LweSample(P.ptr.contents.in_out_params)
+    wo = L.new_LweSample(ext_params)
+    res = L.new_LweSample(P.ptr.contents.in_out_params)
+    try:
+        ctypes.memmove(x.contents.a, a[0].ctypes.data, 4 * P.n)
+        x.contents.b = int(a[0][P.n])
+        L.tfhe_bootstrap_woKS_FFT(wo, sk.cloud.contents.bkFFT, mu, x)
+        w = np.zeros((1, nx + 1), np.int32)
+        ctypes.memmove(w.ctypes.data, wo.contents.a, 4 * nx)
+        w[0, nx] = wo.contents.b
+        assert wo.contents.current_variance == 0.0
+        L.tfhe_bootstrap_FFT(res, sk.cloud.contents.bkFFT, mu, x)
+        assert res.contents.current_variance == _ks_variance_expected(w, P, row_var)[0][0]
+    finally:
+        for smp in (x, wo, res):
+            L.delete_LweSample(smp)

@@ -30,15 +30,16 @@ int set_error(const char* fmt, ...);
 tfhe_amd_device_key* engine_key(const LweBootstrappingKeyFFT* bf);
 
 // Gates on host ciphertexts (stages through pinned memory). in2 may be null.
+// nz (with keyswitch, if not null): per gate, the number of key rows the keyswitch added.
 int engine_gates_host(const LweBootstrappingKeyFFT* bf, int32_t count, const int32_t* ops,
                       LweSample* const* out, const LweSample* const* in0, const LweSample* const* in1,
-                      const LweSample* const* in2, int32_t mu, bool keyswitch);
+                      const LweSample* const* in2, int32_t mu, bool keyswitch, int32_t* nz = nullptr);
 
 // Subsystem entry points on host buffers.
 // acc: (k+1)*N int32 in/out, bara: n entries, rows [row0, row0+n) of the key.
 int engine_blind_rotate_host(tfhe_amd_device_key* dk, int32_t row0, int32_t n, const int32_t* bara, int32_t* acc,
                              int32_t* ext_out);
-int engine_keyswitch_host(tfhe_amd_device_key* dk, const int32_t* in, int32_t* out);
+int engine_keyswitch_host(tfhe_amd_device_key* dk, const int32_t* in, int32_t* out, int32_t* nz = nullptr);
 
 // Drops the host key's device copy. A copy pinned by a server stays allocated
 // (and in use by that server) until its last unpin.

@@ -1577,21 +1577,40 @@ __global__ __launch_bounds__(64) void bk_convert_ct_kernel(const int32_t* __rest
 
 // ext: [2*count][nin+1]; op may be null (no MUX). Writes abar[i][cpad] = a_i + prec_offset
 // (masks transposed so the keyswitch reads one word per gate per i) and binit[g] = b.
+// nz (if not null): nz[g] = the number of nonzero keyswitch digits of gate g (t digits of
+// basebit bits per coefficient), i.e. of key rows the keyswitch adds
+// (libtfhe/lwe-keyswitch-functions.cpp:114-129: one lweSubTo per nonzero digit).
 __global__ __launch_bounds__(256) void ks_prep_kernel(const int32_t* __restrict__ ext,
                                                       const int32_t* __restrict__ second, int nin, int32_t mu,
                                                       uint32_t prec_offset, uint32_t* __restrict__ abar, int cpad,
-                                                      int32_t* __restrict__ binit) {
+                                                      int32_t* __restrict__ binit, int t, int basebit,
+                                                      int32_t* __restrict__ nz) {
+    __shared__ int s_nz[4];
     const int g = blockIdx.x;
     const int j2 = second ? second[g] : -1;
     const bool mux = j2 >= 0;
     const int32_t* e0 = ext + (size_t)g * (nin + 1);
     const int32_t* e1 = ext + (size_t)(mux ? j2 : g) * (nin + 1);
+    const uint32_t mask = (1u << basebit) - 1;
+    int cnt = 0;
     for (int i = threadIdx.x; i <= nin; i += blockDim.x) {
         uint32_t v = (uint32_t)e0[i];
         // libtfhe/boot-gates.cpp:358-361: (0, 1/8) + u1 + u2
         if (mux) v += (uint32_t)e1[i] + (i == nin ? (uint32_t)mu : 0u);
-        if (i < nin) abar[(size_t)i * cpad + g] = v + prec_offset;
-        else binit[g] = (int32_t)v;
+        if (i < nin) {
+            const uint32_t ab = v + prec_offset;
+            abar[(size_t)i * cpad + g] = ab;
+            if (nz)
+                for (int j = 0; j < t; j++) cnt += ((ab >> (32 - (j + 1) * basebit)) & mask) != 0;
+        } else {
+            binit[g] = (int32_t)v;
+        }
+    }
+    if (nz) {  // block sum: wavefront sums, then the four partials
+        for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o);
+        if ((threadIdx.x & 63) == 0) s_nz[threadIdx.x >> 6] = cnt;
+        __syncthreads();
+        if (threadIdx.x == 0) nz[g] = s_nz[0] + s_nz[1] + s_nz[2] + s_nz[3];
     }
 }
 
@@ -2023,8 +2042,9 @@ int ks_path_now(const tfhe_amd_device_key* dk) {
 // scratch region starting at `scr` (ks_scratch_bytes(count) bytes).
 // path: -1 = the current setting (ks_path_now), else the path to take (0 only
 // for keys with byte planes).
+// nz (if not null): per gate, the number of nonzero digits (key rows added).
 int run_keyswitch(const tfhe_amd_device_key* dk, const int32_t* ext, const int32_t* second, int count, int32_t* out,
-                  const int32_t* out_idx, char* scr, hipStream_t s, int path = -1) {
+                  const int32_t* out_idx, char* scr, hipStream_t s, int path = -1, int32_t* nz = nullptr) {
     const HostParams& P = dk->P;
     const int nin = P.k * P.N;
     const uint32_t prec_offset0 = 1u << (32 - (1 + P.ks_basebit * P.ks_t));
@@ -2035,7 +2055,7 @@ int run_keyswitch(const tfhe_amd_device_key* dk, const int32_t* ext, const int32
         uint32_t* abar = (uint32_t*)scr;
         int32_t* binit = (int32_t*)(scr + align256(sizeof(uint32_t) * (size_t)nin * cpad));
         hipLaunchKernelGGL(ks_prep_kernel, dim3(count), dim3(256), 0, s, ext, second, nin, 1 << 29, prec_offset0, abar,
-                           cpad, binit);
+                           cpad, binit, P.ks_t, P.ks_basebit, nz);
         // small batches leave most CUs idle: slice the input coefficients over
         // up to 32 workgroups per tile (an even number of coefficients each)
         const int gx = dk->npad / 64, gy = (count + 16 * KS_MMA_GT - 1) / (16 * KS_MMA_GT);
@@ -2055,7 +2075,7 @@ int run_keyswitch(const tfhe_amd_device_key* dk, const int32_t* ext, const int32
     int32_t* part = (int32_t*)(p + align256(sizeof(int32_t) * (size_t)k.cpad));
     const uint32_t prec_offset = 1u << (32 - (1 + P.ks_basebit * P.ks_t));
     hipLaunchKernelGGL(ks_prep_kernel, dim3(count), dim3(256), 0, s, ext, second, nin, 1 << 29, prec_offset, abar,
-                       k.cpad, binit);
+                       k.cpad, binit, P.ks_t, P.ks_basebit, nz);
     const int ips = (nin + k.splits - 1) / k.splits;
     dim3 grid(k.chunks, k.tiles, k.splits);
     if (k.waves == 4)
@@ -2083,7 +2103,7 @@ size_t ext_bytes(const HostParams& P, int count) {
 int gates_device_locked(Ctx& C, tfhe_amd_device_key* dk, int count, const int32_t* d_op, const int32_t* d_in0,
                         const int32_t* d_in1, const int32_t* d_in2, const int32_t* d_out, int32_t* d_pool,
                         int32_t* d_ext_out, int32_t mu, char* scr, hipStream_t s, bool allow_prof = true,
-                        int ks_path = -1) {
+                        int ks_path = -1, int32_t* d_nz = nullptr) {
     if (count <= 0) return count == 0 ? 0 : set_error("negative gate count");
     const HostParams& P = dk->P;
     const size_t eb = ext_bytes(P, count);
@@ -2163,7 +2183,7 @@ int gates_device_locked(Ctx& C, tfhe_amd_device_key* dk, int count, const int32_
         hipLaunchKernelGGL(sum_halves_kernel, dim3(count), dim3(256), 0, s, ext, second, P.k * P.N, mu, d_ext_out);
         HIP_TRY(hipGetLastError());
     } else {
-        rc = run_keyswitch(dk, ext, second, count, d_pool, d_out, scr + eb, s, ks_path);
+        rc = run_keyswitch(dk, ext, second, count, d_pool, d_out, scr + eb, s, ks_path, d_nz);
     }
     if (prof) {
         HIP_TRY(hipEventRecord(ev.e[2], s));
@@ -2512,7 +2532,7 @@ static void host_parallel(int n, Fn fn) {
 
 int engine_gates_host(const LweBootstrappingKeyFFT* bf, int32_t count, const int32_t* ops,
                       LweSample* const* out, const LweSample* const* in0, const LweSample* const* in1,
-                      const LweSample* const* in2, int32_t mu, bool keyswitch) {
+                      const LweSample* const* in2, int32_t mu, bool keyswitch, int32_t* nz) {
     if (count == 0) return 0;
     if (count < 0 || !ops || !out || !in0) return set_error("bad arguments to gate batch");
     tfhe_amd_device_key* dk = engine_key(bf);
@@ -2553,7 +2573,8 @@ int engine_gates_host(const LweBootstrappingKeyFFT* bf, int32_t count, const int
     const size_t out_bytes = (size_t)count * out_row * sizeof(int32_t);
     const size_t pool_bytes = align256(pool_in_bytes + (keyswitch ? out_bytes : 0));
     const size_t extout_bytes = keyswitch ? 0 : align256(out_bytes);
-    if (int rc = ensure_pinned(C, idx_bytes + pool_bytes + extout_bytes)) return rc;
+    const size_t nz_bytes = (keyswitch && nz) ? align256(sizeof(int32_t) * (size_t)count) : 0;
+    if (int rc = ensure_pinned(C, idx_bytes + pool_bytes + extout_bytes + nz_bytes)) return rc;
     char* h = (char*)C.h_pinned;
     memcpy(h, idx.data(), idx.size() * sizeof(int32_t));
     int32_t* hp = (int32_t*)(h + idx_bytes);
@@ -2563,8 +2584,8 @@ int engine_gates_host(const LweBootstrappingKeyFFT* bf, int32_t count, const int
             hp[(size_t)s * w + P.n] = uniq[s]->b;
         }
     });
-    // device layout: [idx][pool][ext out] then the engine's own scratch
-    const size_t dev_bytes = idx_bytes + pool_bytes + extout_bytes;
+    // device layout: [idx][pool][ext out][nonzero digit counts] then the engine's own scratch
+    const size_t dev_bytes = idx_bytes + pool_bytes + extout_bytes + nz_bytes;
     const size_t work = ext_bytes(P, count) + ks_scratch_bytes(P, count);
     Scratch* ws = nullptr;
     if (int rc = ensure_scratch(C, dev_bytes + work, C.stream, &ws)) return rc;
@@ -2574,12 +2595,16 @@ int engine_gates_host(const LweBootstrappingKeyFFT* bf, int32_t count, const int
     const int32_t* d_idx = (const int32_t*)d;
     int32_t* d_pool = (int32_t*)(d + idx_bytes);
     int32_t* d_extout = keyswitch ? nullptr : (int32_t*)(d + idx_bytes + pool_bytes);
+    int32_t* d_nz = nz_bytes ? (int32_t*)(d + idx_bytes + pool_bytes + extout_bytes) : nullptr;
     int rc = gates_device_locked(C, dk, count, d_idx, d_idx + count, d_idx + 2 * count, d_idx + 3 * count,
-                                 d_idx + 4 * count, d_pool, d_extout, mu, d + dev_bytes, C.stream);
+                                 d_idx + 4 * count, d_pool, d_extout, mu, d + dev_bytes, C.stream, true, -1, d_nz);
     if (rc) return rc;
     const void* src = keyswitch ? (const void*)(d_pool + (size_t)nin_slots * w) : (const void*)d_extout;
     HIP_TRY(hipMemcpyAsync(h + idx_bytes, src, out_bytes, hipMemcpyDeviceToHost, C.stream));
+    char* h_nz = h + idx_bytes + pool_bytes + extout_bytes;
+    if (d_nz) HIP_TRY(hipMemcpyAsync(h_nz, d_nz, sizeof(int32_t) * (size_t)count, hipMemcpyDeviceToHost, C.stream));
     HIP_TRY(hipStreamSynchronize(C.stream));
+    if (d_nz) memcpy(nz, h_nz, sizeof(int32_t) * (size_t)count);
     ws->busy = false;  // the workspace is idle again
     const int32_t* ho = (const int32_t*)(h + idx_bytes);
     auto scatter = [&](int b, int e) {
@@ -2639,12 +2664,13 @@ int engine_blind_rotate_host(tfhe_amd_device_key* dk, int32_t row0, int32_t n, c
     return 0;
 }
 
-int engine_keyswitch_host(tfhe_amd_device_key* dk, const int32_t* in, int32_t* out) {
+int engine_keyswitch_host(tfhe_amd_device_key* dk, const int32_t* in, int32_t* out, int32_t* nz) {
     Ctx& C = ctx();
     std::lock_guard<std::mutex> g(C.mu);
     const HostParams& P = dk->P;
     const int nx = P.k * P.N + 1, w = P.n + 1;
-    const size_t in_b = align256(sizeof(int32_t) * 2 * nx), out_b = align256(sizeof(int32_t) * w);
+    // device: [in][out + the nonzero digit count][keyswitch scratch]
+    const size_t in_b = align256(sizeof(int32_t) * 2 * nx), out_b = align256(sizeof(int32_t) * (w + 1));
     const size_t work = ks_scratch_bytes(P, 1);
     Scratch* ws = nullptr;
     if (int rc = ensure_scratch(C, in_b + out_b + work, C.stream, &ws)) return rc;
@@ -2653,12 +2679,15 @@ int engine_keyswitch_host(tfhe_amd_device_key* dk, const int32_t* in, int32_t* o
     memcpy(h, in, sizeof(int32_t) * nx);
     char* d = (char*)ws->d;
     HIP_TRY(hipMemcpyAsync(d, h, sizeof(int32_t) * nx, hipMemcpyHostToDevice, C.stream));
-    if (int rc = run_keyswitch(dk, (const int32_t*)d, nullptr, 1, (int32_t*)(d + in_b), nullptr, d + in_b + out_b, C.stream))
+    int32_t* d_out = (int32_t*)(d + in_b);
+    if (int rc = run_keyswitch(dk, (const int32_t*)d, nullptr, 1, d_out, nullptr, d + in_b + out_b, C.stream, -1,
+                               d_out + w))
         return rc;
-    HIP_TRY(hipMemcpyAsync(h, d + in_b, sizeof(int32_t) * w, hipMemcpyDeviceToHost, C.stream));
+    HIP_TRY(hipMemcpyAsync(h, d_out, sizeof(int32_t) * (w + 1), hipMemcpyDeviceToHost, C.stream));
     HIP_TRY(hipStreamSynchronize(C.stream));
     ws->busy = false;
     memcpy(out, h, sizeof(int32_t) * w);
+    if (nz) *nz = ((const int32_t*)h)[w];
     return 0;
 }
 

@@ -695,24 +695,32 @@ TFHE_AMD_EXPORT int32_t bootsSymDecrypt(const LweSample* sample, const TFheGateB
 // ----------------------------------------------------------------- gates ----
 
 namespace {
-// variance of a keyswitched sample: every nonzero digit adds one key row's
-// variance; the expected count is k*N*t*(base-1)/base
-double ks_variance(const LweBootstrappingKeyFFT* bf) {
+// current_variance of a keyswitched sample, as the reference keeps it: the
+// result starts as a noiseless trivial sample (variance 0) and every nonzero
+// digit subtracts one key row, adding that row's variance
+// (lwe-keyswitch-functions.cpp:114-129,228-238; lwe-functions.cpp:140-145).
+// The rows carry one variance (key generation and the key files give every
+// row the same), so a count of nz rows gives nz additions of it, in order:
+// ks_variances(bf)[nz] for nz = 0 .. k*N*t.
+std::vector<double> ks_variances(const LweBootstrappingKeyFFT* bf) {
     const LweKeySwitchKey* ks = bf->ks;
-    const double a = bf->in_out_params->alpha_min;
-    return (double)ks->n * ks->t * (ks->base - 1) / ks->base * a * a;
+    const double r = ks->ks0_raw[ks->base > 1 ? 1 : 0].current_variance;
+    std::vector<double> v((size_t)ks->n * ks->t + 1);
+    v[0] = 0.;
+    for (size_t c = 1; c < v.size(); c++) v[c] = v[c - 1] + r;
+    return v;
 }
 
 void one_gate(int op, LweSample* result, const LweSample* a, const LweSample* b, const LweSample* c,
               const TFheGateBootstrappingCloudKeySet* ck) {
     if (!ck) die("null cloud key");
-    int32_t o = op;
+    int32_t o = op, nz = 0;
     LweSample* outs[1] = {result};
     const LweSample* i0[1] = {a};
     const LweSample* i1[1] = {b};
     const LweSample* i2[1] = {c};
-    if (engine_gates_host(ck->bkFFT, 1, &o, outs, i0, i1, i2, modSwitchToTorus32(1, 8), true)) die(g_err.c_str());
-    result->current_variance = ks_variance(ck->bkFFT);
+    if (engine_gates_host(ck->bkFFT, 1, &o, outs, i0, i1, i2, modSwitchToTorus32(1, 8), true, &nz)) die(g_err.c_str());
+    result->current_variance = ks_variances(ck->bkFFT)[nz];
 }
 }  // namespace
 
@@ -752,9 +760,12 @@ TFHE_AMD_EXPORT int32_t bootsGate_batch(int32_t count, const int32_t* gate_op, L
     if (!ck) return set_error("null cloud key");
     for (int i = 0; i < count; i++)
         if (gate_op[i] < TFHE_AMD_NAND || gate_op[i] > TFHE_AMD_MUX) return set_error("gate %d: bad gate code %d", i, gate_op[i]);
-    if (engine_gates_host(ck->bkFFT, count, gate_op, out, in0, in1, in2, modSwitchToTorus32(1, 8), true)) return -1;
-    const double v = ks_variance(ck->bkFFT);
-    for (int i = 0; i < count; i++) out[i]->current_variance = v;
+    std::vector<int32_t> nz(count > 0 ? count : 0);
+    if (engine_gates_host(ck->bkFFT, count, gate_op, out, in0, in1, in2, modSwitchToTorus32(1, 8), true, nz.data()))
+        return -1;
+    const std::vector<double> var = ks_variances(ck->bkFFT);
+    // outputs named twice: the later gate wins, as for the ciphertexts
+    for (int i = 0; i < count; i++) out[i]->current_variance = var[nz[i]];
     return 0;
 }
 
@@ -765,8 +776,9 @@ TFHE_AMD_EXPORT void tfhe_bootstrap_FFT(LweSample* result, const LweBootstrappin
     int32_t op = OP_IDENTITY;
     LweSample* o[1] = {result};
     const LweSample* i0[1] = {x};
-    if (engine_gates_host(bk, 1, &op, o, i0, i0, i0, mu, true)) die(g_err.c_str());
-    result->current_variance = ks_variance(bk);
+    int32_t nz = 0;
+    if (engine_gates_host(bk, 1, &op, o, i0, i0, i0, mu, true, &nz)) die(g_err.c_str());
+    result->current_variance = ks_variances(bk)[nz];
 }
 
 // libtfhe/lwe-bootstrapping-functions-fft.cpp:169-200 (result has dimension k*N)
@@ -826,10 +838,11 @@ TFHE_AMD_EXPORT void lweKeySwitch(LweSample* result, const LweKeySwitchKey* ks, 
     std::vector<int32_t> in((size_t)nin + 1), out((size_t)n + 1);
     memcpy(in.data(), sample->a, sizeof(int32_t) * nin);
     in[nin] = sample->b;
-    if (engine_keyswitch_host(dk, in.data(), out.data())) die(g_err.c_str());
+    int32_t nz = 0;
+    if (engine_keyswitch_host(dk, in.data(), out.data(), &nz)) die(g_err.c_str());
     memcpy(result->a, out.data(), sizeof(int32_t) * n);
     result->b = out[n];
-    result->current_variance = ks_variance(bf);
+    result->current_variance = ks_variances(bf)[nz];
 }
 
 // --------------------------------------------- bulk helpers (no reference counterpart) ----
