@@ -1212,29 +1212,32 @@ __global__ __launch_bounds__(256, 2) void brp_kernel(BrArgs A, const c64* __rest
                         fp[1024 * q + 64 * a] = (int32_t)pacc[q * R + a];
                     }
                 wave_sync();
-                // rotated positions: base (own or partner's quarter 0 position, minus abar)
-                // plus a compile-time offset
-                const int bo = 512 * h + lane - abar, bp = 512 * (1 - h) + lane - abar;
+                // rotated positions x = base + compile-time offset, base = (own or partner's
+                // quarter 0 position - abar) mod 2N, in bytes: (X^abar ACC_c)[j] = +-ACC_c[x mod N]
+                // with the sign from bit log2(N) of x, which x < 3N keeps (x mod 2N has the same
+                // bits below it), so the read needs one mask and the sign one bit-field extract
+                const uint32_t bo4 = ((uint32_t)(512 * h + lane - abar) & (2 * N - 1)) * 4u;
+                const uint32_t bp4 = ((uint32_t)(512 * (1 - h) + lane - abar) & (2 * N - 1)) * 4u;
+                const char* fbc = (const char*)fbi;
                 uint32_t to[2 * R], tp[2 * R];
 #pragma unroll
                 for (int q = 0; q < 2; q++)
 #pragma unroll
                     for (int a = 0; a < R; a++) {
-                        const int io = (bo + 1024 * q + 64 * a) & (2 * N - 1);
-                        const int ip = (bp + 1024 * q + 64 * a) & (2 * N - 1);
-                        to[q * R + a] = (uint32_t)fbi[io & (N - 1)];
-                        tp[q * R + a] = (uint32_t)fbi[ip & (N - 1)];
+                        const uint32_t xo = bo4 + 4u * (1024 * q + 64 * a), xp = bp4 + 4u * (1024 * q + 64 * a);
+                        to[q * R + a] = *(const uint32_t*)(fbc + (xo & (4u * N - 1)));
+                        tp[q * R + a] = *(const uint32_t*)(fbc + (xp & (4u * N - 1)));
                     }
 #pragma unroll
                 for (int q = 0; q < 2; q++)
 #pragma unroll
                     for (int a = 0; a < R; a++) {
-                        const int io = (bo + 1024 * q + 64 * a) & (2 * N - 1);
-                        const int ip = (bp + 1024 * q + 64 * a) & (2 * N - 1);
-                        const uint32_t ro = (io & N) ? 0u - to[q * R + a] : to[q * R + a];
-                        const uint32_t rp = (ip & N) ? 0u - tp[q * R + a] : tp[q * R + a];
-                        to[q * R + a] = ro - acc[q * R + a] + offset;
-                        tp[q * R + a] = rp - pacc[q * R + a] + offset;
+                        const uint32_t xo = bo4 + 4u * (1024 * q + 64 * a), xp = bp4 + 4u * (1024 * q + 64 * a);
+                        // 0 or -1: the sign of the wrapped coefficient
+                        const uint32_t mo = (uint32_t)__builtin_amdgcn_sbfe((int32_t)xo, 2 + LOG2N2 - 1, 1);
+                        const uint32_t mp = (uint32_t)__builtin_amdgcn_sbfe((int32_t)xp, 2 + LOG2N2 - 1, 1);
+                        to[q * R + a] = ((to[q * R + a] ^ mo) - mo) - acc[q * R + a] + offset;
+                        tp[q * R + a] = ((tp[q * R + a] ^ mp) - mp) - pacc[q * R + a] + offset;
                     }
                 wave_sync();  // the reads are done before the transform reuses the buffer
                 // the truncating gadget decomposition, tgsw-functions.cpp:337-348
@@ -1400,21 +1403,27 @@ __global__ __launch_bounds__(256, 2) void brp_kernel(BrArgs A, const c64* __rest
         {
             // y_0 / 2 and y_1 / 2 -> u_{j'} = y0 + y1 (quarters 0, 2), u_{j'+512} =
             // (y0 - y1) e^{-i pi/4} (quarters 1, 3); ACC_c += result
-            // (lwe-bootstrapping-functions-fft.cpp:67-76) for the own and the partner's quarters
+            // (lwe-bootstrapping-functions-fft.cpp:67-76) for the own and the partner's quarters.
+            // h is the same on the whole wavefront: one branch, no per-value selects.
             const c64* o = s_fft[w ^ 1];
+            auto combine = [&](auto hc) {
+                constexpr int H = decltype(hc)::value;
 #pragma unroll
-            for (int a = 0; a < R; a++) {
-                const c64 ya = x[a], yb = o[a * 64 + lane];
-                const c64 y0 = h == 0 ? ya : yb, y1 = h == 0 ? yb : ya;
-                const c64 u0 = cadd(y0, y1);
-                const c64 dd = csub(y0, y1);
-                const c64 u1 = c64{P::SQ * (dd.re + dd.im), P::SQ * (dd.im - dd.re)};
-                const c64 uo = h == 0 ? u0 : u1, up = h == 0 ? u1 : u0;
-                acc[a] += (uint32_t)round_mod32(uo.re);
-                acc[R + a] += (uint32_t)round_mod32(uo.im);
-                pacc[a] = (uint32_t)acc_in[a * 64 + lane] + (uint32_t)round_mod32(up.re);
-                pacc[R + a] = (uint32_t)acc_in[(R + a) * 64 + lane] + (uint32_t)round_mod32(up.im);
-            }
+                for (int a = 0; a < R; a++) {
+                    const c64 ya = x[a], yb = o[a * 64 + lane];
+                    const c64 y0 = H == 0 ? ya : yb, y1 = H == 0 ? yb : ya;
+                    const c64 u0 = cadd(y0, y1);
+                    const c64 dd = csub(y0, y1);
+                    const c64 u1 = c64{P::SQ * (dd.re + dd.im), P::SQ * (dd.im - dd.re)};
+                    const c64 uo = H == 0 ? u0 : u1, up = H == 0 ? u1 : u0;
+                    acc[a] += (uint32_t)round_mod32(uo.re);
+                    acc[R + a] += (uint32_t)round_mod32(uo.im);
+                    pacc[a] = (uint32_t)acc_in[a * 64 + lane] + (uint32_t)round_mod32(up.re);
+                    pacc[R + a] = (uint32_t)acc_in[(R + a) * 64 + lane] + (uint32_t)round_mod32(up.im);
+                }
+            };
+            if (h == 0) combine(std::integral_constant<int, 0>{});
+            else combine(std::integral_constant<int, 1>{});
         }
     }
     // tLweExtractLweSampleIndex(.., 0, ..), libtfhe/lwe.cpp:41-56
