@@ -231,6 +231,15 @@ def bench_pbig(T, args):
     P = T.big_params()
     sk = T.SecretKey(P, seed=args.seed)
     sk.upload()
+    # kernel shapes per batch size for this key: the four-wavefront pair kernel and the
+    # two-wavefront kernel timed on a ladder of sizes; the crossover goes to automatic mode
+    tune = None
+    if not args.no_autotune:
+        _, pair_max, table = T.autotune(sk.cloud)
+        tune = {"pair_kernel_up_to": pair_max, "two_wave_kernel_above": pair_max,
+                "ms": table}
+        print(f"[bench] N=2048 autotune: pair kernel up to {pair_max} gates, two-wavefront above", file=sys.stderr,
+              flush=True)
     rng = np.random.default_rng(args.seed + 4)
     G = 32768
     ops = np.concatenate([np.full(G // 2, 10), rng.integers(0, 10, size=G // 2)]).astype(np.int32)
@@ -321,13 +330,16 @@ def bench_pbig(T, args):
            "scaled": dict(timed(scaled), launch_sizes={"min": int(scaled.min()), "median": int(np.median(scaled)),
                                                        "max": int(scaled.max())}),
            "batch_8192": timed([8192]),
-           "decrypt_errors": errors}
+           "decrypt_errors": errors,
+           "autotune": tune}
     if not args.no_compare_modes:
-        # the same 8192-gate launch on the pair kernel's radix-8 form (mode 8) and again on
-        # the default fused-butterfly form, alternating, in this run
-        cmp = {"radix8_transform": [], "fused_butterfly_transform": []}
+        # the same 8192-gate launch on the pair kernel (four wavefronts per bootstrap) on the
+        # radix-8 stages (mode 8) and on the fused butterflies (mode 4), and on the
+        # two-wavefront kernel (mode 6, the automatic choice at this size), alternating
+        cmp = {"pair_radix8_transform": [], "pair_fused_butterfly": [], "two_wave_fused_butterfly": []}
         for _ in range(2):
-            for mode, name in ((8, "radix8_transform"), (0, "fused_butterfly_transform")):
+            for mode, name in ((8, "pair_radix8_transform"), (4, "pair_fused_butterfly"),
+                               (6, "two_wave_fused_butterfly")):
                 T.set_launch_config(mode)
                 try:
                     cmp[name].append(timed([8192])["gates_per_s"])

@@ -115,10 +115,11 @@ def test_pbig_pair_kernel_matches_one_wave_kernel(pbig):
 
 @pytest.mark.parametrize("count", [1, 37, 600])
 def test_pbig_pair_kernel_forms_bitexact(pbig, count):
-    """The pair kernel on the fused-butterfly transform (the default, mode 4) and
-    on the radix-8 stages (mode 8) give identical ciphertexts, equal to the
-    oracle's (first gates) and decrypting to the plaintext gates; 600 gates with
-    MUX take more workgroups than the GPU holds at once."""
+    """The pair kernel on the fused-butterfly transform (the default, mode 4),
+    on the radix-8 stages (mode 8) and the two-wavefront kernel (mode 6) give
+    identical ciphertexts, equal to the oracle's (first gates) and decrypting to
+    the plaintext gates; 600 gates with MUX take more workgroups than the GPU
+    holds at once."""
     T, P, sk, keys = pbig
     rng = np.random.default_rng(47 + count)
     ops = rng.integers(0, 11, size=count).astype(np.int32)
@@ -127,13 +128,14 @@ def test_pbig_pair_kernel_forms_bitexact(pbig, count):
     a, b, c = (sk.encrypt(x) for x in bits)
     print(f"[test] N=2048 pair kernel forms, {count} gates", flush=True)
     got = {}
-    for mode in (4, 8):
+    for mode in (4, 8, 6):
         T.set_launch_config(mode)
         try:
             got[mode] = T.run_gates(sk.cloud, ops, a, b, c)
         finally:
             T.set_launch_config(0)
     assert np.array_equal(got[4], got[8])
+    assert np.array_equal(got[4], got[6])
     k = min(count, 6)
     assert np.array_equal(got[4][:k], O.gates(keys, ops[:k], a[:k], b[:k], c[:k], threads=16))
     exp = np.array([O.TRUTH[O.GATES[o]](x, y, z) for o, x, y, z in zip(ops, *bits)])
@@ -156,8 +158,9 @@ def n2048_l3(gpu):
 
 def test_n2048_l3_pair_kernel_bitexact(n2048_l3):
     """Every gate type (and extra MUX, odd batch) through the l = 3 pair kernel
-    (fused-butterfly and radix-8 forms): bit-exact against the oracle and the
-    one-wavefront kernel, and decrypting to the plaintext gate."""
+    (fused-butterfly and radix-8 forms) and the two-wavefront kernel: bit-exact
+    against the oracle and the one-wavefront kernel, and decrypting to the
+    plaintext gate."""
     T, P, sk, keys = n2048_l3
     rng = np.random.default_rng(46)
     ops = np.array(list(range(11)) + [10, 10], np.int32)
@@ -174,8 +177,45 @@ def test_n2048_l3_pair_kernel_bitexact(n2048_l3):
         radix8 = T.run_gates(sk.cloud, ops, a, b, c)
     finally:
         T.set_launch_config(0)
+    T.set_launch_config(6)
+    try:
+        two_wave = T.run_gates(sk.cloud, ops, a, b, c)
+    finally:
+        T.set_launch_config(0)
     assert np.array_equal(pair, one)
     assert np.array_equal(pair, radix8)
+    assert np.array_equal(pair, two_wave)
     assert np.array_equal(pair, O.gates(keys, ops, a, b, c, threads=16))
     exp = np.array([O.TRUTH[O.GATES[o]](x, y, z) for o, x, y, z in zip(ops, *bits)])
     assert np.array_equal(sk.decrypt(pair), exp)
+
+
+def test_pbig_autotune_chooses_kernel_shape(pbig):
+    """The autotuner on an N = 2048 key times the pair kernel (four wavefronts
+    per bootstrap) and the two-wavefront kernel over its ladder of batch sizes
+    and records the faster one per size; automatic mode then gives the same
+    ciphertexts as either kernel forced, on both sides of the crossover and
+    beyond the ladder."""
+    T, P, sk, keys = pbig
+    lat, pair_max, table = T.autotune(sk.cloud)
+    assert lat == 0
+    assert pair_max == 0 or pair_max in T.AUTOTUNE_SIZES
+    assert set(table) == set(T.AUTOTUNE_MODES_N2048)
+    assert all(v > 0 for row in table.values() for v in row.values())
+    # the crossover is where the measured ladder says it is
+    sizes = [s for s in T.AUTOTUNE_SIZES if s <= pair_max]
+    assert all(table["pair_four_wave"][s] <= table["two_wave"][s] for s in sizes)
+    rng = np.random.default_rng(48)
+    for count in (max(pair_max, 1), pair_max + 130, 1600):
+        ops = rng.integers(0, 11, size=count).astype(np.int32)
+        bits = rng.integers(0, 2, size=(3, count)).astype(np.int32)
+        a, b, c = (sk.encrypt(x) for x in bits)
+        auto = T.run_gates(sk.cloud, ops, a, b, c)
+        for mode in (4, 6):
+            T.set_launch_config(mode)
+            try:
+                assert np.array_equal(T.run_gates(sk.cloud, ops, a, b, c), auto), (count, mode)
+            finally:
+                T.set_launch_config(0)
+        exp = np.array([O.TRUTH[O.GATES[o]](x, y, z) for o, x, y, z in zip(ops, *bits)])
+        assert np.array_equal(sk.decrypt(auto), exp)

@@ -466,15 +466,22 @@ AUTOTUNE_SIZES = (64, 128, 256, 384, 512, 640, 768, 896, 1024, 1280, 1536)
 AUTOTUNE_MODES = ("digit_parallel", "two_wave_split", "one_per_wavefront_batched")
 
 
+AUTOTUNE_MODES_N2048 = ("pair_four_wave", "two_wave")
+
+
 def autotune(cloud):
-    """Measures the three blind-rotation kernels on this key over AUTOTUNE_SIZES
-    and sets the automatic mode's crossovers. Returns (latency_max,
-    small_batch_max, {mode name: {size: ms}})."""
+    """Measures the blind-rotation kernels of this key's ring size over
+    AUTOTUNE_SIZES and sets the automatic mode's crossovers. Returns
+    (latency_max, small_batch_max, {mode name: {size: ms}}); for an N = 2048
+    key small_batch_max is the pair kernel's crossover, latency_max 0, and the
+    table has AUTOTUNE_MODES_N2048."""
     lat, small = ctypes.c_int32(), ctypes.c_int32()
     ms = (ctypes.c_float * (3 * len(AUTOTUNE_SIZES)))()
     check(lib().tfhe_amd_autotune(cloud, ctypes.byref(lat), ctypes.byref(small), ms), "autotune")
+    big = cloud.contents.params.contents.tgsw_params.contents.tlwe_params.contents.N == 2048
+    names = AUTOTUNE_MODES_N2048 if big else AUTOTUNE_MODES
     table = {name: {sz: ms[m * len(AUTOTUNE_SIZES) + i] for i, sz in enumerate(AUTOTUNE_SIZES)}
-             for m, name in enumerate(AUTOTUNE_MODES)}
+             for m, name in enumerate(names)}
     return lat.value, small.value, table
 
 

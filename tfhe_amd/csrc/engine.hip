@@ -66,6 +66,10 @@ struct tfhe_amd_device_key {
     size_t bytes;
     uint64_t gen;  // unique per upload: a re-uploaded key never compares equal to a freed one
     int pins;      // holders besides the host key (servers); the memory is freed when the last one lets go
+    uint32_t pair_sizes;  // M = 1024, automatic mode: bit i set when the four-wavefront pair kernel
+                          // beats the two-wavefront kernel at the autotuner's ladder size i; a batch
+                          // takes the choice of the smallest ladder size that holds it, batches
+                          // beyond the ladder the two-wavefront kernel
     bool orphaned; // the host key has forgotten it (release or delete) while it was pinned
 };
 
@@ -1442,6 +1446,235 @@ __global__ __launch_bounds__(256, 2) void brp_kernel(BrArgs A, const c64* __rest
     }
 }
 
+// Blind rotation for N = 2048 with one bootstrap per 128-thread workgroup of
+// two wavefronts (four workgroups per CU, twice the bootstraps resident of
+// brp_kernel) on the fused-butterfly pair transform (FftCTPair; key and tables
+// as for brp_kernel<L, true>). Wavefront h runs half h of every transform, for
+// both accumulator components: per key row and input component it rotates and
+// decomposes the whole of ACC_c (its own quarters h, h + 2 and its partner's,
+// which the transform's radix-2 step across the pair needs) from the
+// accumulator in LDS, transforms each digit polynomial and multiply-accumulates
+// the bins of both output components against its half of the key rows; then
+// wavefront h takes over output component h (the pair swaps the halves of the
+// other component once through LDS), inverse-transforms both of its halves and
+// adds the result to ACC_h. Three workgroup barriers per key row.
+template <int L>
+__global__ __launch_bounds__(128, 2) void brq_kernel(BrArgs A, const c64* __restrict__ bkq, const c64* __restrict__ twq,
+                                                    const int32_t* __restrict__ mux_gate,
+                                                    const int32_t* __restrict__ n_jobs) {
+    using G = FftCT;
+    using P = FftPair;
+    constexpr int R = 8, N = 2048, KPL = 2 * L, LOG2N2 = 12;
+    constexpr int NMAX = BRP_NMAX;
+    __shared__ c64 s_fft[2][G::LDS_C64];  // transform buffer of wavefront h; the halves handed over at a row's end
+    __shared__ int32_t s_acc[2][N];       // ACC_0, ACC_1
+    __shared__ int16_t s_bara[NMAX];
+    __shared__ int32_t s_barb;
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int h = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int job = blockIdx.x;
+    if (job >= *n_jobs) return;  // whole block idle (uniform)
+    const int n = A.n;
+    c64* fb = s_fft[h];
+
+    // gate linear form and modulus switch (wavefront 1, while wavefront 0 loads its tables)
+    if (h == 1) {
+        const int gate = job < A.count ? job : mux_gate[job - A.count];
+        const int half = job < A.count ? 0 : 1;
+        const int op = A.op[gate];
+        int32_t k0, ca, cb, cc;
+        gate_coeffs(op, half, k0, ca, cb, cc);
+        const int32_t* a = A.pool + (size_t)A.in0[gate] * (n + 1);
+        const int32_t* b = (op == OP_IDENTITY) ? a : A.pool + (size_t)A.in1[gate] * (n + 1);
+        const int32_t* cp = (op == TFHE_AMD_MUX) ? A.pool + (size_t)A.in2[gate] * (n + 1) : a;
+        for (int i = lane; i <= n; i += 64) {
+            uint32_t v =
+                (uint32_t)ca * (uint32_t)a[i] + (uint32_t)cb * (uint32_t)b[i] + (uint32_t)cc * (uint32_t)cp[i];
+            if (i == n) v += (uint32_t)k0;
+            // modSwitchFromTorus32(v, 2N), libtfhe/numeric-functions.cpp:59-65
+            const uint32_t bar = (v + (1u << (31 - LOG2N2))) >> (32 - LOG2N2);
+            if (i < n) s_bara[i] = (int16_t)bar;
+            else s_barb = (int32_t)bar;
+        }
+    }
+    // this wavefront's forward tables, in registers for the whole kernel
+    const ct64* twl = (const ct64*)(twq + h * G::COUNT) + lane;
+    ct64 fwb[4], fwc[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        fwb[k] = twl[G::FB + 64 * k];
+        fwc[k] = twl[G::FC + 64 * k];
+    }
+    const uint32_t offset = A.offset;
+    const int Bgbit = A.Bgbit;
+    const int32_t dhalf = 1 << (Bgbit - 1);
+    __syncthreads();
+    {
+        // ACC = (0, X^{2N-barb} * [mu...mu]), lwe-bootstrapping-functions-fft.cpp:146-148
+        const int barb = s_barb;
+        for (int j = tid; j < N; j += 128) {
+            s_acc[0][j] = 0;
+            s_acc[1][j] = (((j + barb) & (2 * N - 1)) < N) ? A.mu : -A.mu;
+        }
+    }
+    __syncthreads();
+
+    // coefficient positions: own quarters h, h + 2 from jo = 512 h + lane, the partner's
+    // 1 - h, 3 - h from jp = 512 (1 - h) + lane, plus 1024 q + 64 a (compile-time)
+    const int jo = 512 * h + lane, jp = 512 * (1 - h) + lane;
+    for (int i = 0; i < n; i++) {
+        const int abar = s_bara[i];
+        c64 r0[R], r1[R];
+#pragma unroll
+        for (int s = 0; s < R; s++) r0[s] = r1[s] = c64{0.0, 0.0};
+#pragma unroll 1
+        for (int c = 0; c < 2; c++) {
+            // T = (X^abar - 1) * ACC_c + gadget offset: slot k = 8 (2 g + q) + a
+            // holds coefficient j = (g ? jp : jo) + 1024 q + 64 a. In bytes, x = 4 ((j0 - abar)
+            // mod 2N) + 4 (1024 q + 64 a) for the slot's base j0: the rotated coefficient is
+            // +-ACC_c[x / 4 mod N], its sign bit log2(N) of x / 4 (x / 4 < 3N)
+            uint32_t t[4 * R];
+            {
+                const char* ac = (const char*)s_acc[c];
+                // the same for both components: computed per component (opaque offset), since
+                // kept across the loop they would need ~100 registers
+                const int z = opaque_zero();
+                const uint32_t xo = ((uint32_t)(jo + z - abar) & (2 * N - 1)) * 4u;
+                const uint32_t xp = ((uint32_t)(jp + z - abar) & (2 * N - 1)) * 4u;
+                const int32_t* ao = s_acc[c] + jo;
+                const int32_t* ap = s_acc[c] + jp;
+#pragma unroll
+                for (int k = 0; k < 4 * R; k++) {
+                    const int g = k / (2 * R), q = (k / R) & 1, a = k % R;
+                    const uint32_t x = (g ? xp : xo) + 4u * (uint32_t)(1024 * q + 64 * a);
+                    const uint32_t m = (uint32_t)__builtin_amdgcn_sbfe((int32_t)x, 2 + LOG2N2 - 1, 1);
+                    const uint32_t rv = *(const uint32_t*)(ac + (x & (4u * N - 1)));
+                    const uint32_t av = (uint32_t)(g ? ap : ao)[1024 * q + 64 * a];
+                    t[k] = ((rv ^ m) - m) - av + offset;
+                }
+            }
+#pragma unroll 1
+            for (int p = 0; p < L; p++) {
+                // digit p (tgsw-functions.cpp:337-348): field p minus Bg/2. (With a signed field
+                // extract of the flipped word instead, this kernel's compiled form converted the
+                // field to double as unsigned.)
+                // x: u_{j'} = d_{j'} + i d_{j'+1024} of the own quarters, ov: of the partner's
+                const int shift = 32 - (p + 1) * Bgbit;
+                auto dig = [&](uint32_t v) { return (double)((int32_t)__builtin_amdgcn_ubfe(v, shift, Bgbit) - dhalf); };
+                c64 x[R], ov[R];
+#pragma unroll
+                for (int a = 0; a < R; a++) {
+                    x[a] = c64{dig(t[a]), dig(t[R + a])};
+                    ov[a] = c64{dig(t[2 * R + a]), dig(t[3 * R + a])};
+                }
+                wave_sync();  // the previous transform is done with the buffer
+                if (h == 0) {
+                    P::pair_fwd<0>(x, ov);
+                    FftCTPair::fwd1<0>(x, lane, fb);
+                } else {
+                    P::pair_fwd<1>(x, ov);
+                    FftCTPair::fwd1<1>(x, lane, fb);
+                }
+                wave_sync();
+                G::fwd2(x, lane, fb, fwb);
+                wave_sync();
+                G::fwd3(x, lane, fb);
+                // key row (i, c L + p), half h: output component 0 at q[s * 64], 1 at q[1024 + s * 64]
+                const c64* q = bkq + ((size_t)i * KPL + c * L + p) * 2048 + h * 512 + lane;
+                c64 kv[R];
+#pragma unroll
+                for (int s = 0; s < R; s++) kv[s] = q[s * 64];
+                wave_sync();
+                G::fwd4(x, lane, fb, fwc);
+#pragma unroll
+                for (int s = 0; s < R; s++) {
+                    cmac(r0[s], x[s], kv[s]);
+                    kv[s] = q[1024 + s * 64];
+                }
+#pragma unroll
+                for (int s = 0; s < R; s++) cmac(r1[s], x[s], kv[s]);
+            }
+        }
+        // wavefront h keeps output component h: it hands its half of the other component
+        // to the partner and takes the partner's half of its own
+        __syncthreads();  // both wavefronts are done with their transform buffers
+        {
+            c64* pb = s_fft[h ^ 1];
+            if (h == 0) {
+#pragma unroll
+                for (int s = 0; s < R; s++) pb[s * 64 + lane] = r1[s];
+            } else {
+#pragma unroll
+                for (int s = 0; s < R; s++) pb[s * 64 + lane] = r0[s];
+            }
+        }
+        __syncthreads();  // the halves handed over
+        {
+            // component c = h: its bins of half h (own, in registers) and of half 1 - h (from
+            // the partner, in this wavefront's buffer); inverse transform of each half with
+            // that half's untwist (tables read once per row at an opaque offset, so they are not
+            // hoisted out of the row loop), then u_{j'} = y0 + y1 (quarters 0, 2) and
+            // u_{j'+512} = (y0 - y1) e^{-i pi/4} (quarters 1, 3), ACC_c += result
+            // (lwe-bootstrapping-functions-fft.cpp:67-76)
+            c64 y0[R], y1[R];
+            if (h == 0) {
+#pragma unroll
+                for (int s = 0; s < R; s++) {
+                    y0[s] = r0[s];
+                    y1[s] = fb[s * 64 + lane];
+                }
+            } else {
+#pragma unroll
+                for (int s = 0; s < R; s++) {
+                    y0[s] = fb[s * 64 + lane];
+                    y1[s] = r1[s];
+                }
+            }
+            const int z = opaque_zero();
+            const ct64* tb = (const ct64*)twq + z + lane;  // half 0's tables; half 1's at + COUNT
+            ct64 wb[4], wa[4];
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                wb[k] = tb[G::IB + 64 * k];
+                wa[k] = tb[G::IA + 64 * k];
+            }
+#pragma unroll
+            for (int hh = 0; hh < 2; hh++) {
+                c64* x = hh ? y1 : y0;
+                c64 ut[R];
+#pragma unroll
+                for (int k = 0; k < R; k++) ut[k] = twq[hh * G::COUNT + G::UT + 64 * k + z + lane];
+                wave_sync();
+                G::inv1(x, lane, fb);
+                wave_sync();
+                G::inv2(x, lane, fb, wb);
+                wave_sync();
+                G::inv3(x, lane, fb);
+                wave_sync();
+                G::inv4(x, lane, fb, wa, ut);
+            }
+            int32_t* sa = s_acc[h] + lane;
+#pragma unroll
+            for (int a = 0; a < R; a++) {
+                const c64 lo = cadd(y0[a], y1[a]);
+                const c64 dd = csub(y0[a], y1[a]);
+                const c64 hi = c64{P::SQ * (dd.re + dd.im), P::SQ * (dd.im - dd.re)};
+                sa[64 * a] = (int32_t)((uint32_t)sa[64 * a] + (uint32_t)round_mod32(lo.re));
+                sa[1024 + 64 * a] = (int32_t)((uint32_t)sa[1024 + 64 * a] + (uint32_t)round_mod32(lo.im));
+                sa[512 + 64 * a] = (int32_t)((uint32_t)sa[512 + 64 * a] + (uint32_t)round_mod32(hi.re));
+                sa[1536 + 64 * a] = (int32_t)((uint32_t)sa[1536 + 64 * a] + (uint32_t)round_mod32(hi.im));
+            }
+        }
+        __syncthreads();  // ACC updated for the next row
+    }
+    // tLweExtractLweSampleIndex(.., 0, ..), libtfhe/lwe.cpp:41-56
+    int32_t* e = A.ext + (size_t)job * (N + 1);
+    for (int j = tid; j < N; j += 128) e[j == 0 ? 0 : N - j] = j == 0 ? s_acc[0][0] : (int32_t)(0u - (uint32_t)s_acc[0][j]);
+    if (tid == 0) e[N] = s_acc[1][0];
+}
+
 // Pair-of-wavefronts transform of every bootstrapping-key polynomial (N = 2048):
 // one 128-thread workgroup per polynomial, output [poly][h][slot][lane].
 __global__ __launch_bounds__(128) void bk_convert_pair_kernel(const int32_t* __restrict__ bk, c64* __restrict__ bkp,
@@ -2109,6 +2342,18 @@ size_t ext_bytes(const HostParams& P, int count) {
 // Core: gates on device-resident data. `scr` must hold ext_bytes(count) +
 // ks_scratch_bytes(count). d_ext_out (optional) receives the summed extracted
 // samples instead of running the keyswitch.
+// the autotuner's ladder of batch sizes (gates)
+constexpr int TUNE_SIZES[] = {64, 128, 256, 384, 512, 640, 768, 896, 1024, 1280, 1536};
+constexpr int TUNE_NS = sizeof(TUNE_SIZES) / sizeof(TUNE_SIZES[0]);
+
+// N = 2048, automatic mode: the pair kernel when it was the faster at the smallest
+// ladder size that holds the batch; beyond the ladder the two-wavefront kernel
+bool pair_kernel_chosen(const tfhe_amd_device_key* dk, int count) {
+    for (int si = 0; si < TUNE_NS; si++)
+        if (count <= TUNE_SIZES[si]) return (dk->pair_sizes >> si) & 1u;
+    return false;
+}
+
 int gates_device_locked(Ctx& C, tfhe_amd_device_key* dk, int count, const int32_t* d_op, const int32_t* d_in0,
                         const int32_t* d_in1, const int32_t* d_in2, const int32_t* d_out, int32_t* d_pool,
                         int32_t* d_ext_out, int32_t mu, char* scr, hipStream_t s, bool allow_prof = true,
@@ -2145,8 +2390,15 @@ int gates_device_locked(Ctx& C, tfhe_amd_device_key* dk, int count, const int32_
     // batches). N = 2048 uses the pair-transform kernel (4: fused butterflies, 8: radix-8 stages)
     // unless mode 1 is forced.
     int mode = C.br_mode;
-    if (dk->M != 512) mode = (mode == 1 || dk->P.n > BRP_NMAX || !dk->bkp) ? 1 : (mode == 8 || !dk->bkq) ? 8 : 4;
-    else if (mode == 8) mode = 2;
+    if (dk->M != 512) {
+        // N = 2048: 1 one-wavefront, 4 pair (four wavefronts), 6 two-wavefront, 8 pair on the
+        // radix-8 stages; otherwise automatic: the pair kernel up to pair_max gates
+        if (mode == 1 || dk->P.n > BRP_NMAX || !dk->bkp) mode = 1;
+        else if (mode == 8 || !dk->bkq) mode = 8;
+        else if (mode != 4 && mode != 6) mode = pair_kernel_chosen(dk, count) ? 4 : 6;
+    } else if (mode == 8 || mode == 6) {
+        mode = 2;
+    }
     else if (mode == 4) mode = 2;
     if (mode == 3 && dk->P.n > BR2_NMAX) mode = 2;
     if (mode == 0) {
@@ -2164,6 +2416,10 @@ int gates_device_locked(Ctx& C, tfhe_amd_device_key* dk, int count, const int32_
         const int blocks = 2 * count;  // one bootstrap per workgroup, at most 2*count jobs
         if (dk->P.l == 2) hipLaunchKernelGGL((brp_kernel<2, true>), dim3(blocks), dim3(256), 0, s, A, dk->bkq, dk->twq, mux_gate, n_jobs);
         else hipLaunchKernelGGL((brp_kernel<3, true>), dim3(blocks), dim3(256), 0, s, A, dk->bkq, dk->twq, mux_gate, n_jobs);
+    } else if (mode == 6) {
+        const int blocks = 2 * count;  // one bootstrap per two-wavefront workgroup, at most 2*count jobs
+        if (dk->P.l == 2) hipLaunchKernelGGL(brq_kernel<2>, dim3(blocks), dim3(128), 0, s, A, dk->bkq, dk->twq, mux_gate, n_jobs);
+        else hipLaunchKernelGGL(brq_kernel<3>, dim3(blocks), dim3(128), 0, s, A, dk->bkq, dk->twq, mux_gate, n_jobs);
     } else if (mode == 8) {
         const int blocks = 2 * count;  // the same kernel on the radix-8 stages (for comparison)
         if (dk->P.l == 2) hipLaunchKernelGGL(brp_kernel<2>, dim3(blocks), dim3(256), 0, s, A, dk->bkp, dk->twp, mux_gate, n_jobs);
@@ -2310,6 +2566,7 @@ static int upload_key(Ctx& C, const LweBootstrappingKeyFFT* bf, tfhe_amd_device_
     dk->gen = ++next_gen;  // under C.mu
     dk->pins = 0;
     dk->orphaned = false;
+    dk->pair_sizes = 0x1f;  // until the autotuner measures it: the pair kernel up to 512 gates
     dk->P = P;
     dk->M = M;
     dk->kpl = kpl;
@@ -2831,12 +3088,19 @@ TFHE_AMD_EXPORT int32_t tfhe_amd_autotune(const TFheGateBootstrappingCloudKeySet
     if (!dk) return -1;
     Ctx& C = ctx();
     std::lock_guard<std::mutex> g(C.mu);
-    static const int sizes[] = {64, 128, 256, 384, 512, 640, 768, 896, 1024, 1280, 1536};
-    constexpr int NS = sizeof(sizes) / sizeof(sizes[0]);
-    static const int modes[3] = {5, 3, 2};
-    if (dk->M != 512 || dk->P.n > BR4_ANY_NMAX) {
-        C.brl_max = 0;
-        C.br_small_max = 0;
+    const int* sizes = TUNE_SIZES;
+    constexpr int NS = TUNE_NS;
+    // N = 1024: the latency, two-wavefront and batched kernels (modes 5, 3, 2);
+    // N = 2048: the four-wavefront pair kernel and the two-wavefront kernel (modes 4, 6)
+    const bool pbig = dk->M == 1024;
+    static const int modes512[3] = {5, 3, 2}, modes1024[2] = {4, 6};
+    const int nm = pbig ? 2 : 3;
+    const int* modes = pbig ? modes1024 : modes512;
+    if (pbig ? (dk->P.n > BRP_NMAX || !dk->bkq) : (dk->M != 512 || dk->P.n > BR4_ANY_NMAX)) {
+        if (!pbig) {
+            C.brl_max = 0;
+            C.br_small_max = 0;
+        }
         if (latency_max) *latency_max = 0;
         if (small_batch_max) *small_batch_max = 0;
         return 0;
@@ -2878,9 +3142,9 @@ TFHE_AMD_EXPORT int32_t tfhe_amd_autotune(const TFheGateBootstrappingCloudKeySet
     if (int rc = ensure_scratch(C, ext_bytes(dk->P, maxB) + ks_scratch_bytes(dk->P, maxB), C.stream, &ws))
         return cleanup(rc);
     const int saved = C.br_mode;
-    float best[3][NS];
+    float best[3][NS] = {};
     int rc = 0;
-    for (int m = 0; m < 3 && !rc; m++) {
+    for (int m = 0; m < nm && !rc; m++) {
         C.br_mode = modes[m];
         for (int si = 0; si < NS && !rc; si++) {
             const int B = sizes[si];
@@ -2901,6 +3165,22 @@ TFHE_AMD_EXPORT int32_t tfhe_amd_autotune(const TFheGateBootstrappingCloudKeySet
     }
     C.br_mode = saved;
     if (rc) return cleanup(rc < 0 ? rc : set_error("autotune: event record failed"));
+    if (pbig) {
+        // per ladder size, the faster kernel; the crossover reported is the largest size up to
+        // which the pair kernel is the faster at every point
+        uint32_t mask = 0;
+        for (int si = 0; si < NS; si++)
+            if (best[0][si] <= best[1][si]) mask |= 1u << si;
+        dk->pair_sizes = mask;
+        int pm = 0;
+        for (int si = 0; si < NS && ((mask >> si) & 1u); si++) pm = sizes[si];
+        if (latency_max) *latency_max = 0;
+        if (small_batch_max) *small_batch_max = pm;
+        if (ms)
+            for (int m = 0; m < 3; m++)
+                for (int k = 0; k < NS; k++) ms[m * NS + k] = m < nm ? best[m][k] : 0.0f;
+        return cleanup(0);
+    }
     int si = 0, lat = 0, small = 0;
     for (; si < NS && best[0][si] <= best[1][si] && best[0][si] <= best[2][si]; si++) lat = sizes[si];
     small = lat;
