@@ -263,6 +263,27 @@ def test_wavefront_transform_emulation(fft_emu, N, bits):
     assert float(f[5]) < 1e-4
 
 
+@pytest.fixture(scope="module")
+def brq_emu(tmp_path_factory):
+    exe = str(tmp_path_factory.mktemp("emu") / "brq_emu")
+    src = os.path.join(os.path.dirname(os.path.abspath(__file__)), "brq_emu.cpp")
+    subprocess.run(["g++", "-O2", "-std=c++17", "-o", exe, src], check=True)
+    return exe
+
+
+@pytest.mark.parametrize("seed,abar", [(1, 0), (2, 1), (3, 511), (4, 777), (5, 2047), (6, 2048), (7, 3000),
+                                       (8, 4095)])
+def test_two_wave_kernel_row_emulation(brq_emu, seed, abar):
+    """One key row of the two-wavefront N = 2048 blind rotation, its per-lane
+    rotation (one mask and a sign bit-field per value), decomposition, pair
+    transform, multiply-accumulate, handover of the other component's half,
+    inverse halves and accumulator update, emulated lane by lane on the CPU:
+    equal to the exact external product mod 2^32 for rotations at and across
+    the wrap points (0, N - 1, N, 2N - 1)."""
+    out = subprocess.run([brq_emu, str(seed), str(abar)], capture_output=True, text=True).stdout.split()
+    assert out[:4] == ["abar", str(abar), "mismatches", "0"], out
+
+
 @pytest.mark.parametrize("N,kpl,bits,worst,exact", [
     (1, 6, 7, 0, True), (1, 6, 7, 1, True),          # P128, fused transform: typical and sign-aligned worst case
     (1024, 6, 7, 1, True), (-1024, 6, 7, 1, True),   # P128, radix-8 and half transforms, worst case
