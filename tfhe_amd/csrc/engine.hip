@@ -341,23 +341,26 @@ __global__ __launch_bounds__(1024) void mux_scan_kernel(const int32_t* __restric
 // N = 1024). Bootstraps share nothing, so each is its own workgroup: the
 // dispatcher refills a CU as soon as any one of them ends, and a batch of 512
 // gates still spreads over all 256 CUs. The transposes of the transform go
-// through the workgroup's 14 KB of LDS, and the transformed key and the stage-1
-// twiddles come straight from global memory (cached; the 4096 wavefronts of a
-// batch walk the key rows nearly together), so the LDS carries only the
-// transposes. The accumulator lives in registers in the transform's
-// coefficient layout; the negacyclic rotation is staged through the
-// wavefront's transform buffer as (ACC_c, -ACC_c, ACC_c), so every rotated
-// read is a plain load.
+// through the workgroup's LDS, and the transformed key and the twiddles come
+// straight from global memory (cached; the 4096 wavefronts of a batch walk the
+// key rows nearly together). The accumulator lives in registers in the
+// transform's coefficient layout; the negacyclic rotation is staged through
+// the wavefront's transform buffer, so every rotated read is a plain load.
 struct Br4Lds {
-    // the transform buffer, also holding (v, -v, v) for the rotation (3N int32)
+    // radix-8 form: one transform buffer, also holding (v, -v, v) for the rotation (3N int32)
     static constexpr int BUF_C64 = (Fft<512>::LDS_C64 > 3 * 1024 / 4) ? Fft<512>::LDS_C64 : 3 * 1024 / 4;
+    // fused-butterfly form: two transform buffers (the two inverse transforms of a
+    // row run interleaved), the first also holding the rotation's 2N words
+    static constexpr int CT_C64 = FftCT::LDS_C64;
+    static_assert(CT_C64 * 16 >= 2 * 1024 * 4, "rotation buffer");
 };
 
-// CT: the fused-butterfly transform (FftCT, key bkc, tables twc) instead of Fft<512>.
+// CT: the fused-butterfly transform (FftCT, key bkc, tables twc; LDS fb and fb2
+// of Br4Lds::CT_C64 entries) instead of Fft<512> (LDS fb of Br4Lds::BUF_C64).
 template <int L, bool CT = false>
 __device__ __forceinline__ void br4_body(BrArgs A, const int32_t* __restrict__ mux_gate,
                                          const int32_t* __restrict__ n_jobs, c64* __restrict__ fb,
-                                         int16_t* __restrict__ bara) {
+                                         int16_t* __restrict__ bara, c64* __restrict__ fb2) {
     constexpr int M = 512;
     using F = Fft<M>;
     using G = FftCT;
@@ -432,59 +435,158 @@ __device__ __forceinline__ void br4_body(BrArgs A, const int32_t* __restrict__ m
     for (int i = 0; i < n; i++) {
         const int abar = bara[i];
         c64 r0[R], r1[R];
+        if constexpr (CT) {
+            // digit p of component c: extraction, forward transform, products with key
+            // digit polynomial (i, c l + p) added into (r0, r1) (written, for the row's
+            // first digit, with the roundings of an addition into zero)
+            auto digit = [&](const int c, const int p, const uint32_t* tv, auto first_tag) {
+                constexpr bool FIRST = decltype(first_tag)::value;
+                const int r = c * L + p;
+                // digit p of the truncating gadget decomposition, tgsw-functions.cpp:337-348:
+                // ((T + offset) >> shift & (Bg - 1)) - Bg/2 = the signed field of the flipped word
+                const int shift = 32 - (p + 1) * Bgbit;
+                c64 x[R];
 #pragma unroll
-        for (int s = 0; s < R; s++) r0[s] = r1[s] = c64{0.0, 0.0};
-        uint32_t tv[2 * R];
-#pragma unroll 1
-        for (int r = 0; r < KPL; r++) {
-            const int c = r >= L ? 1 : 0, p = r - c * L;
-            if (p == 0) {
-                // T = (X^abar - 1) * ACC_c + gadget offset. The wave's buffer holds
-                // E = (ACC_c, -ACC_c, ACC_c), so (X^abar ACC_c)[j] = E[j + 2N - abar] with
-                // 0 < j + 2N - abar < 3N: one base address, constant offsets, no masks.
-                wave_sync();  // the previous transform is done with the buffer
-#pragma unroll
-                for (int q = 0; q < 2 * R; q++) {
-                    const int j = 64 * (q % R) + lane + (q / R) * M;
-                    const uint32_t v = c ? acc[1][q] : acc[0][q];
-                    fbi[j] = (int32_t)v;
-                    fbi[j + N] = (int32_t)(0u - v);
-                    fbi[j + 2 * N] = (int32_t)v;
+                for (int s = 0; s < R; s++) {
+                    const int32_t d0 = __builtin_amdgcn_sbfe((int32_t)tv[s], shift, Bgbit);
+                    const int32_t d1 = __builtin_amdgcn_sbfe((int32_t)tv[R + s], shift, Bgbit);
+                    x[s] = c64{(double)d0, (double)d1};
                 }
-                wave_sync();
-                const int32_t* er = fbi + lane + 2 * N - abar;
+                wave_sync();  // rotated reads or the previous transform done with the buffer
+                // key digit polynomial (i, r): component 0 at q[s * 64], component 1 at
+                // q[M + s * 64]. Component 0 is requested before the transform, half of
+                // component 1 before its last stage, the other half as the component-0
+                // products free their registers.
+                const c64* q = A.bkf + ((size_t)i * KPL + r) * 2 * M + lane;
+                c64 kv[R], kw[R / 2];
 #pragma unroll
-                for (int q = 0; q < 2 * R; q++) tv[q] = (uint32_t)er[64 * (q % R) + (q / R) * M];
-#pragma unroll
-                for (int q = 0; q < 2 * R; q++) tv[q] = (tv[q] - (c ? acc[1][q] : acc[0][q]) + offset) ^ flip;
-            }
-            // digit p of the truncating gadget decomposition, tgsw-functions.cpp:337-348:
-            // ((T + offset) >> shift & (Bg - 1)) - Bg/2 = the signed field of the flipped word
-            const int shift = 32 - (p + 1) * Bgbit;
-            c64 x[R];
-#pragma unroll
-            for (int s = 0; s < R; s++) {
-                const int32_t d0 = __builtin_amdgcn_sbfe((int32_t)tv[s], shift, Bgbit);
-                const int32_t d1 = __builtin_amdgcn_sbfe((int32_t)tv[R + s], shift, Bgbit);
-                x[s] = c64{(double)d0, (double)d1};
-            }
-            wave_sync();  // rotated reads (p = 0) or the previous transform done with the buffer
-            const c64* q = A.bkf + ((size_t)i * KPL + r) * 2 * M + lane;
-            c64 kv[R];
-            if constexpr (CT) {
+                for (int s = 0; s < R; s++) kv[s] = q[s * 64];
                 G::fwd1(x, lane, fb);
                 wave_sync();
                 G::fwd2(x, lane, fb, fwb);
                 wave_sync();
                 G::fwd3(x, lane, fb);
-                // key digit polynomial (i, r): component 0 at q[s * 64], component 1 at
-                // q[M + s * 64]; component 0 requested before the last stage, component 1
-                // as its products free the registers
 #pragma unroll
-                for (int s = 0; s < R; s++) kv[s] = q[s * 64];
+                for (int s = 0; s < R / 2; s++) kw[s] = q[M + s * 64];
                 wave_sync();
                 G::fwd4(x, lane, fb, fwc);
-            } else {
+#pragma unroll
+                for (int s = 0; s < R; s++) {
+                    if (FIRST) r0[s] = cmul_first(x[s], kv[s]);
+                    else cmac(r0[s], x[s], kv[s]);
+                    if (s < R / 2) kv[s] = q[M + (s + R / 2) * 64];
+                }
+#pragma unroll
+                for (int s = 0; s < R; s++) {
+                    const c64 k = s < R / 2 ? kw[s] : kv[s - R / 2];
+                    if (FIRST) r1[s] = cmul_first(x[s], k);
+                    else cmac(r1[s], x[s], k);
+                }
+            };
+#pragma unroll
+            for (int c = 0; c < 2; c++) {
+                // T = (X^abar - 1) * ACC_c + gadget offset. With E = (ACC_c, -ACC_c, ACC_c),
+                // (X^abar ACC_c)[j] = E[j + 2N - abar]; for abar <= N every read falls in
+                // E[N, 3N) = (-ACC_c, ACC_c), otherwise in E[0, 2N) = (ACC_c, -ACC_c), so
+                // the buffer holds those 2N words: one base address, no masks or selects.
+                uint32_t tv[2 * R];
+                {
+                    const bool low = abar <= N;
+                    const int vofs = low ? N : 0, nofs = low ? 0 : N;
+                    wave_sync();  // the previous transform is done with the buffer
+#pragma unroll
+                    for (int q = 0; q < 2 * R; q++) {
+                        const int j = 64 * (q % R) + lane + (q / R) * M;
+                        const uint32_t v = acc[c][q];
+                        fbi[j + nofs] = (int32_t)(0u - v);
+                        fbi[j + vofs] = (int32_t)v;
+                    }
+                    wave_sync();
+                    const int32_t* er = fbi + lane + (low ? N : 2 * N) - abar;
+#pragma unroll
+                    for (int q = 0; q < 2 * R; q++) tv[q] = (uint32_t)er[64 * (q % R) + (q / R) * M];
+#pragma unroll
+                    for (int q = 0; q < 2 * R; q++) tv[q] = (tv[q] - acc[c][q] + offset) ^ flip;
+                }
+                int p0 = 0;
+                if (c == 0) {
+                    digit(0, 0, tv, std::true_type{});
+                    p0 = 1;
+                }
+#pragma unroll 1
+                for (int p = p0; p < L; p++) digit(c, p, tv, std::false_type{});
+            }
+            // the inverse tables, loaded once per key row for both components (at an
+            // opaque offset, so not hoisted out of the row loop)
+            ct64 iwb[4], iwa[4];
+            c64 iut[R];
+            const ct64* twc = twc_base + opaque_zero() + lane;
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                iwb[k] = twc[G::IB + 64 * k];
+                iwa[k] = twc[G::IA + 64 * k];
+            }
+#pragma unroll
+            for (int k = 0; k < R; k++) iut[k] = ((const c64*)twc)[G::UT + 64 * k];
+            // inverse transforms and ACC += result (lwe-bootstrapping-functions-fft.cpp:67-76),
+            // the two components interleaved stage by stage, component 1 in the second buffer
+            wave_sync();
+            G::inv1(r0, lane, fb);
+            G::inv1(r1, lane, fb2);
+            wave_sync();
+            G::inv2(r0, lane, fb, iwb);
+            G::inv2(r1, lane, fb2, iwb);
+            wave_sync();
+            G::inv3(r0, lane, fb);
+            G::inv3(r1, lane, fb2);
+            wave_sync();
+            G::inv4(r0, lane, fb, iwa, iut);
+            G::inv4(r1, lane, fb2, iwa, iut);
+#pragma unroll
+            for (int s = 0; s < R; s++) {
+                acc[0][s] += (uint32_t)round_mod32(r0[s].re);
+                acc[0][R + s] += (uint32_t)round_mod32(r0[s].im);
+                acc[1][s] += (uint32_t)round_mod32(r1[s].re);
+                acc[1][R + s] += (uint32_t)round_mod32(r1[s].im);
+            }
+        } else {
+#pragma unroll
+            for (int s = 0; s < R; s++) r0[s] = r1[s] = c64{0.0, 0.0};
+            uint32_t tv[2 * R];
+#pragma unroll 1
+            for (int r = 0; r < KPL; r++) {
+                const int c = r >= L ? 1 : 0, p = r - c * L;
+                if (p == 0) {
+                    // T = (X^abar - 1) * ACC_c + gadget offset. The wave's buffer holds
+                    // E = (ACC_c, -ACC_c, ACC_c), so (X^abar ACC_c)[j] = E[j + 2N - abar] with
+                    // 0 < j + 2N - abar < 3N: one base address, constant offsets, no masks.
+                    wave_sync();  // the previous transform is done with the buffer
+#pragma unroll
+                    for (int q = 0; q < 2 * R; q++) {
+                        const int j = 64 * (q % R) + lane + (q / R) * M;
+                        const uint32_t v = c ? acc[1][q] : acc[0][q];
+                        fbi[j] = (int32_t)v;
+                        fbi[j + N] = (int32_t)(0u - v);
+                        fbi[j + 2 * N] = (int32_t)v;
+                    }
+                    wave_sync();
+                    const int32_t* er = fbi + lane + 2 * N - abar;
+#pragma unroll
+                    for (int q = 0; q < 2 * R; q++) tv[q] = (uint32_t)er[64 * (q % R) + (q / R) * M];
+#pragma unroll
+                    for (int q = 0; q < 2 * R; q++) tv[q] = (tv[q] - (c ? acc[1][q] : acc[0][q]) + offset) ^ flip;
+                }
+                const int shift = 32 - (p + 1) * Bgbit;
+                c64 x[R];
+#pragma unroll
+                for (int s = 0; s < R; s++) {
+                    const int32_t d0 = __builtin_amdgcn_sbfe((int32_t)tv[s], shift, Bgbit);
+                    const int32_t d1 = __builtin_amdgcn_sbfe((int32_t)tv[R + s], shift, Bgbit);
+                    x[s] = c64{(double)d0, (double)d1};
+                }
+                wave_sync();  // rotated reads (p = 0) or the previous transform done with the buffer
+                const c64* q = A.bkf + ((size_t)i * KPL + r) * 2 * M + lane;
+                c64 kv[R];
                 F::template fwd1<64>(x, lane, fb, tw1);
                 wave_sync();
                 F::fwd2(x, lane, fb, tw2);
@@ -494,46 +596,18 @@ __device__ __forceinline__ void br4_body(BrArgs A, const int32_t* __restrict__ m
                 for (int s = 0; s < R; s++) kv[s] = q[s * 64];
                 wave_sync();
                 F::fwd4(x, lane, fb);
+#pragma unroll
+                for (int s = 0; s < R; s++) {
+                    cmac(r0[s], x[s], kv[s]);
+                    kv[s] = q[M + s * 64];
+                }
+#pragma unroll
+                for (int s = 0; s < R; s++) cmac(r1[s], x[s], kv[s]);
             }
+            // inverse transforms and ACC += result (lwe-bootstrapping-functions-fft.cpp:67-76)
 #pragma unroll
-            for (int s = 0; s < R; s++) {
-                cmac(r0[s], x[s], kv[s]);
-                kv[s] = q[M + s * 64];
-            }
-#pragma unroll
-            for (int s = 0; s < R; s++) cmac(r1[s], x[s], kv[s]);
-        }
-        // inverse transforms and ACC += result (lwe-bootstrapping-functions-fft.cpp:67-76);
-        // FftCT: the inverse tables, loaded once per key row for both components
-        // (at an opaque offset, so not hoisted out of the row loop)
-        ct64 iwb[4], iwa[4];
-        c64 iut[R];
-        if constexpr (CT) {
-            const ct64* twc = twc_base + opaque_zero() + lane;
-#pragma unroll
-            for (int k = 0; k < 4; k++) {
-                iwb[k] = twc[G::IB + 64 * k];
-                iwa[k] = twc[G::IA + 64 * k];
-            }
-#pragma unroll
-            for (int k = 0; k < R; k++) iut[k] = ((const c64*)twc)[G::UT + 64 * k];
-        }
-#pragma unroll
-        for (int cc = 0; cc < 2; cc++) {
-            c64* x = cc ? r1 : r0;
-            if constexpr (CT) {
-                const ct64* wb = iwb;
-                const ct64* wa = iwa;
-                const c64* ut = iut;
-                wave_sync();
-                G::inv1(x, lane, fb);
-                wave_sync();
-                G::inv2(x, lane, fb, wb);
-                wave_sync();
-                G::inv3(x, lane, fb);
-                wave_sync();
-                G::inv4(x, lane, fb, wa, ut);
-            } else {
+            for (int cc = 0; cc < 2; cc++) {
+                c64* x = cc ? r1 : r0;
                 wave_sync();
                 F::inv1(x, lane, fb);
                 wave_sync();
@@ -542,11 +616,11 @@ __device__ __forceinline__ void br4_body(BrArgs A, const int32_t* __restrict__ m
                 F::inv3(x, lane, fb);
                 wave_sync();
                 F::template inv4<64>(x, lane, fb, tw1);
-            }
 #pragma unroll
-            for (int s = 0; s < R; s++) {
-                acc[cc][s] += (uint32_t)round_mod32(x[s].re);
-                acc[cc][R + s] += (uint32_t)round_mod32(x[s].im);
+                for (int s = 0; s < R; s++) {
+                    acc[cc][s] += (uint32_t)round_mod32(x[s].re);
+                    acc[cc][R + s] += (uint32_t)round_mod32(x[s].im);
+                }
             }
         }
     }
@@ -566,9 +640,14 @@ __device__ __forceinline__ void br4_body(BrArgs A, const int32_t* __restrict__ m
 template <int L, bool CT = false>
 __global__ __launch_bounds__(64, 8) void br4_kernel(BrArgs A, const int32_t* __restrict__ mux_gate,
                                                    const int32_t* __restrict__ n_jobs) {
-    __shared__ c64 fb[Br4Lds::BUF_C64];
     __shared__ int16_t bara[1024];
-    br4_body<L, CT>(A, mux_gate, n_jobs, fb, bara);
+    if constexpr (CT) {
+        __shared__ c64 fb[Br4Lds::CT_C64], fb2[Br4Lds::CT_C64];
+        br4_body<L, CT>(A, mux_gate, n_jobs, fb, bara, fb2);
+    } else {
+        __shared__ c64 fb[Br4Lds::BUF_C64];
+        br4_body<L, CT>(A, mux_gate, n_jobs, fb, bara, nullptr);
+    }
 }
 
 // One launch over several clients' batches (segments), each with its own
@@ -601,9 +680,9 @@ __global__ __launch_bounds__(64, 8) void br4_seg_kernel(BrSegTable T) {
     A.in0 += g.off;
     A.in1 += g.off;
     A.in2 += g.off;
-    __shared__ c64 fb[Br4Lds::BUF_C64];
+    __shared__ c64 fb[Br4Lds::CT_C64], fb2[Br4Lds::CT_C64];
     __shared__ int16_t bara[1024];
-    br4_body<L, true>(A, g.mux_gate, g.n_jobs, fb, bara);
+    br4_body<L, true>(A, g.mux_gate, g.n_jobs, fb, bara, fb2);
 }
 
 // largest LWE dimension the latency kernel's LDS table holds
