@@ -78,12 +78,12 @@ struct TLweSample { TorusPolynomial* a; TorusPolynomial* b; double current_varia
 /* include/tgsw.h:35-76 */
 struct TGswKey { const TGswParams* params; const TLweParams* tlwe_params; IntPolynomial* key; TLweKey tlwe_key; };
 struct TGswSample { TLweSample* all_sample; TLweSample** bloc_sample; int32_t k; int32_t l; };
-/* include/tgsw.h:78-96. There is no host transform-domain copy in this
- * library (all_samples/sample are NULL): the sample is row `row` of the
- * device-resident key of `owner`. */
+/* include/tgsw.h:78-96, the same layout. There is no host transform-domain copy
+ * in this library (all_samples/sample are NULL): the rows of a bootstrapping key
+ * live on the device, and an array of them is recognised by its address (the
+ * bkFFT array of a LweBootstrappingKeyFFT made by this library). */
 struct TGswSampleFFT {
     TLweSampleFFT* all_samples; TLweSampleFFT** sample; int32_t k; int32_t l;
-    const LweBootstrappingKeyFFT* owner; int32_t row;
 };
 /* include/lwekeyswitch.h:11-28 (ks[i][j][h] = ks0_raw[(i*t+j)*base+h]) */
 struct LweKeySwitchKey {
@@ -95,13 +95,14 @@ struct LweBootstrappingKey {
     const LweParams* in_out_params; const TGswParams* bk_params; const TLweParams* accum_params;
     const LweParams* extract_params; TGswSample* bk; LweKeySwitchKey* ks;
 };
-/* include/lwebootstrappingkey.h:36-59; dkey is filled on first use (or by
- * tfhe_amd_cloud_key_upload) and points at the HBM-resident key, built from
- * the coefficient-domain key bk_coef. */
+/* include/lwebootstrappingkey.h:36-59, the same layout. Made only by this
+ * library (new_LweBootstrappingKeyFFT, the key-set constructors and readers),
+ * which keeps beside it the coefficient-domain key it was built from and the
+ * handle of its HBM-resident copy (uploaded on first use or by
+ * tfhe_amd_cloud_key_upload). */
 struct LweBootstrappingKeyFFT {
     const LweParams* in_out_params; const TGswParams* bk_params; const TLweParams* accum_params;
     const LweParams* extract_params; const TGswSampleFFT* bkFFT; const LweKeySwitchKey* ks;
-    tfhe_amd_device_key* dkey; const LweBootstrappingKey* bk_coef;
 };
 /* include/tfhe_gate_bootstrapping_structures.h:26-62 */
 struct TFheGateBootstrappingCloudKeySet {

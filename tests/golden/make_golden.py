@@ -11,6 +11,7 @@ and every object of that format for a key set with n = 16 (seed 7) with the
 reference's outputs of 22 gates under it.
 
     python tests/golden/make_golden.py
+    python tests/golden/make_golden.py layout   # ref_struct_layout.txt only
 """
 import hashlib
 import json
@@ -117,5 +118,19 @@ def main():
         print(f, os.path.getsize(os.path.join(HERE, f)))
 
 
+def struct_layout():
+    """Sizes and field offsets of the gate API's structs as the reference's own
+    headers lay them out (tests/c/struct_layout.c built as C++ against
+    src/include), for the binary-layout test."""
+    exe = os.path.join(tempfile.mkdtemp(), "struct_layout_ref")
+    subprocess.run(["g++", "-std=c++11", "-x", "c++", "-DREFERENCE_HEADERS", "-I", "/root/reference/src/include",
+                    os.path.join(os.path.dirname(HERE), "c", "struct_layout.c"), "-o", exe], check=True)
+    out = subprocess.run([exe], check=True, capture_output=True, text=True).stdout
+    open(os.path.join(HERE, "ref_struct_layout.txt"), "w").write(out)
+
+
 if __name__ == "__main__":
+    if len(sys.argv) > 1 and sys.argv[1] == "layout":
+        struct_layout()
+        sys.exit(0)
     main()

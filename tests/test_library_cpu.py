@@ -118,3 +118,27 @@ def test_philox_known_answers(tmp_path):
     src = os.path.join(os.path.dirname(os.path.abspath(__file__)), "c", "philox_kat.cpp")
     subprocess.run(["g++", "-O2", "-std=c++17", "-o", exe, src], check=True)
     assert subprocess.run([exe], capture_output=True, text=True, check=True).stdout.strip() == "OK"
+
+
+def test_struct_layout_matches_reference_headers(tmp_path):
+    """Every public struct has the size and field offsets the reference's own
+    headers give it (include/*.h of tfhe: lweparams.h, tlwe.h, tgsw.h:78-96,
+    lwebootstrappingkey.h:36-59, ...), so a program compiled against those
+    headers finds every field where it expects it. The expected table was written
+    from the reference headers (tests/golden/make_golden.py layout); where they
+    are present it is rebuilt from them here too."""
+    src = os.path.join(ROOT, "tests", "c", "struct_layout.c")
+    exe = str(tmp_path / "struct_layout")
+    subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"), src, "-o", exe],
+                   check=True)
+    ours = subprocess.run([exe], check=True, capture_output=True, text=True).stdout
+    golden = open(os.path.join(ROOT, "tests", "golden", "ref_struct_layout.txt")).read()
+    assert ours == golden
+    ref_inc = "/root/reference/src/include"
+    if os.path.isdir(ref_inc):
+        exr = str(tmp_path / "struct_layout_ref")
+        subprocess.run(["g++", "-std=c++11", "-x", "c++", "-DREFERENCE_HEADERS", "-I", ref_inc, src, "-o", exr],
+                       check=True)
+        assert subprocess.run([exr], check=True, capture_output=True, text=True).stdout == golden
+    sizes = dict(line.split(" size ") for line in golden.splitlines() if " size " in line)
+    assert sizes["TGswSampleFFT"] == "24" and sizes["LweBootstrappingKeyFFT"] == "48"

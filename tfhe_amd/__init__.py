@@ -58,7 +58,7 @@ class TLweSample(ctypes.Structure):
 
 
 class TGswSampleFFT(ctypes.Structure):
-    _fields_ = [("all_samples", vp), ("sample", vp), ("k", i32), ("l", i32), ("owner", vp), ("row", i32)]
+    _fields_ = [("all_samples", vp), ("sample", vp), ("k", i32), ("l", i32)]
 
 
 class LweKeySwitchKey(ctypes.Structure):
@@ -69,7 +69,7 @@ class LweKeySwitchKey(ctypes.Structure):
 class LweBootstrappingKeyFFT(ctypes.Structure):
     _fields_ = [("in_out_params", vp), ("bk_params", ctypes.POINTER(TGswParams)), ("accum_params", vp),
                 ("extract_params", vp), ("bkFFT", ctypes.POINTER(TGswSampleFFT)),
-                ("ks", ctypes.POINTER(LweKeySwitchKey)), ("dkey", vp), ("bk_coef", vp)]
+                ("ks", ctypes.POINTER(LweKeySwitchKey))]
 
 
 class CloudKeySet(ctypes.Structure):

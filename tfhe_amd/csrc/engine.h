@@ -26,6 +26,15 @@ HostParams host_params_fft(const LweBootstrappingKeyFFT* bf);
 int set_error(const char* fmt, ...);
 [[noreturn]] void die(const char* what);
 
+// What the library keeps beside a transform-domain key it made (the public
+// struct has the reference's layout): the coefficient-domain key it was built
+// from and the handle of its device copy (null until uploaded). Dies on a
+// pointer this library did not make.
+const LweBootstrappingKey* bkfft_coef(const LweBootstrappingKeyFFT* bf);
+tfhe_amd_device_key*& bkfft_dkey(const LweBootstrappingKeyFFT* bf);
+// The transform-domain key whose bkFFT array holds `rows`, and the index of
+// `rows` in it; nullptr if no key made by this library holds it.
+const LweBootstrappingKeyFFT* bkfft_of_rows(const TGswSampleFFT* rows, int32_t* row);
 // Device key of a transform-domain key; uploads on first use. Returns nullptr on error.
 tfhe_amd_device_key* engine_key(const LweBootstrappingKeyFFT* bf);
 

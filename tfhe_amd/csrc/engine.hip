@@ -2634,7 +2634,7 @@ size_t engine_key_bytes(const tfhe_amd_device_key* dk) { return dk->bytes; }
 static int upload_key(Ctx& C, const LweBootstrappingKeyFFT* bf, tfhe_amd_device_key** out) {
     const HostParams P = host_params_fft(bf);
     if (int rc = check_supported(P)) return rc;
-    const LweBootstrappingKey* bk = bf->bk_coef;
+    const LweBootstrappingKey* bk = bkfft_coef(bf);
     if (!bk || !bk->bk || !bk->ks) return set_error("cloud key has no coefficient-domain bootstrapping key");
     const int N = P.N, k = P.k, kpl = (k + 1) * P.l, n = P.n, M = N / 2;
     const int base = 1 << P.ks_basebit, t = P.ks_t, nin = k * N;
@@ -2807,13 +2807,14 @@ static tfhe_amd_device_key* engine_key_impl(const LweBootstrappingKeyFFT* bf0, b
     std::lock_guard<std::mutex> g(C.mu);
     if (ensure_ready(C)) return nullptr;
     LweBootstrappingKeyFFT* bf = const_cast<LweBootstrappingKeyFFT*>(bf0);
-    if (!bf->dkey) {
+    tfhe_amd_device_key*& slot = bkfft_dkey(bf);
+    if (!slot) {
         tfhe_amd_device_key* dk = nullptr;
         if (upload_key(C, bf, &dk)) return nullptr;
-        bf->dkey = dk;
+        slot = dk;
     }
-    if (pin) bf->dkey->pins++;  // under the same lock as the lookup: no release in between
-    return bf->dkey;
+    if (pin) slot->pins++;  // under the same lock as the lookup: no release in between
+    return slot;
 }
 
 static void free_key(Ctx& C, tfhe_amd_device_key* dk) {
@@ -2836,12 +2837,12 @@ static void free_key(Ctx& C, tfhe_amd_device_key* dk) {
 }
 
 void engine_forget_host_key(const LweBootstrappingKeyFFT* bkfft) {
-    if (!bkfft || !bkfft->dkey) return;
+    if (!bkfft || !bkfft_dkey(bkfft)) return;
     Ctx& C = ctx();
     std::lock_guard<std::mutex> g(C.mu);
-    LweBootstrappingKeyFFT* bf = const_cast<LweBootstrappingKeyFFT*>(bkfft);
-    tfhe_amd_device_key* dk = bf->dkey;
-    bf->dkey = nullptr;  // the next use through this host key uploads a fresh copy
+    tfhe_amd_device_key*& slot = bkfft_dkey(bkfft);
+    tfhe_amd_device_key* dk = slot;
+    slot = nullptr;  // the next use through this host key uploads a fresh copy
     if (dk->pins > 0) dk->orphaned = true;  // a server still runs on it; the last unpin frees it
     else free_key(C, dk);
 }
@@ -3051,7 +3052,7 @@ TFHE_AMD_EXPORT void tfhe_amd_cloud_key_release(const TFheGateBootstrappingCloud
 }
 
 TFHE_AMD_EXPORT size_t tfhe_amd_cloud_key_device_bytes(const TFheGateBootstrappingCloudKeySet* ck) {
-    return (ck && ck->bkFFT && ck->bkFFT->dkey) ? ck->bkFFT->dkey->bytes : 0;
+    return (ck && ck->bkFFT && bkfft_dkey(ck->bkFFT)) ? bkfft_dkey(ck->bkFFT)->bytes : 0;
 }
 
 TFHE_AMD_EXPORT int32_t tfhe_amd_gates_device(const TFheGateBootstrappingCloudKeySet* ck, int32_t count,
@@ -3083,7 +3084,7 @@ TFHE_AMD_EXPORT int32_t tfhe_amd_gates_device_noalloc(const TFheGateBootstrappin
     std::lock_guard<std::mutex> g(C.mu);
     // an upload allocates: the key must already be on the device (read under the
     // lock a release takes)
-    tfhe_amd_device_key* dk = ck->bkFFT->dkey;
+    tfhe_amd_device_key* dk = bkfft_dkey(ck->bkFFT);
     if (!C.ready || !dk)
         return set_error("gates_device_noalloc: the cloud key is not on the device (call tfhe_amd_cloud_key_upload first)");
     if (count == 0) return 0;

@@ -2,6 +2,7 @@
 // reference gate API (include/tfhe_gate_bootstrapping_functions.h). Every
 // bootstrap and keyswitch is handed to the GPU engine (engine.hip); there is
 // no CPU implementation of the bootstrapping path in this library.
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -120,8 +121,23 @@ struct KsOwner {
 
 struct BkFftOwner {
     LweBootstrappingKeyFFT pub;  // must stay first
+    uint64_t magic = kBkFftMagic;
     std::vector<TGswSampleFFT> rows;
+    const LweBootstrappingKey* bk_coef = nullptr;
+    tfhe_amd_device_key* dkey = nullptr;
+    static constexpr uint64_t kBkFftMagic = 0x7466686542464654ull;
 };
+
+// every live transform-domain key this library made, for finding the key of a
+// row array (tfhe_blindRotate_FFT gets the rows, not the key)
+std::mutex g_bkfft_mu;
+std::vector<BkFftOwner*> g_bkfft;
+
+BkFftOwner* bkfft_owner(const LweBootstrappingKeyFFT* bf) {
+    auto* o = reinterpret_cast<BkFftOwner*>(const_cast<LweBootstrappingKeyFFT*>(bf));
+    if (!bf || o->magic != BkFftOwner::kBkFftMagic) die("LweBootstrappingKeyFFT not made by tfhe_amd");
+    return o;
+}
 
 KsOwner* new_ks(int32_t nin, int32_t t, int32_t basebit, const LweParams* out_params) {
     auto* o = new KsOwner();
@@ -166,9 +182,11 @@ BkFftOwner* new_bkfft(const LweBootstrappingKey* bk) {
     const int n = bk->in_out_params->n;
     o->rows.resize(n);
     o->pub = LweBootstrappingKeyFFT{bk->in_out_params, bk->bk_params, bk->accum_params, bk->extract_params,
-                                    o->rows.data(), bk->ks, nullptr, bk};
-    for (int i = 0; i < n; i++)
-        o->rows[i] = TGswSampleFFT{nullptr, nullptr, bk->accum_params->k, bk->bk_params->l, &o->pub, i};
+                                    o->rows.data(), bk->ks};
+    o->bk_coef = bk;
+    for (int i = 0; i < n; i++) o->rows[i] = TGswSampleFFT{nullptr, nullptr, bk->accum_params->k, bk->bk_params->l};
+    std::lock_guard<std::mutex> g(g_bkfft_mu);
+    g_bkfft.push_back(o);
     return o;
 }
 
@@ -176,9 +194,15 @@ void unregister_ks(const LweBootstrappingKeyFFT* bf);
 
 void delete_bkfft(const LweBootstrappingKeyFFT* bf) {
     if (!bf) return;
+    BkFftOwner* o = bkfft_owner(bf);
     unregister_ks(bf);
     engine_forget_host_key(bf);
-    delete reinterpret_cast<BkFftOwner*>(const_cast<LweBootstrappingKeyFFT*>(bf));
+    {
+        std::lock_guard<std::mutex> g(g_bkfft_mu);
+        g_bkfft.erase(std::remove(g_bkfft.begin(), g_bkfft.end(), o), g_bkfft.end());
+    }
+    o->magic = 0;
+    delete o;
 }
 
 void delete_bk(const LweBootstrappingKey* bk) {
@@ -235,6 +259,23 @@ void add_binary_product(const int32_t* s, const int32_t* a, int N, int32_t* b) {
 int32_t to_torus(double d) { return (int32_t)(int64_t)((d - (double)(int64_t)d) * 4294967296.0); }
 
 }  // namespace
+
+const LweBootstrappingKey* bkfft_coef(const LweBootstrappingKeyFFT* bf) { return bkfft_owner(bf)->bk_coef; }
+tfhe_amd_device_key*& bkfft_dkey(const LweBootstrappingKeyFFT* bf) { return bkfft_owner(bf)->dkey; }
+
+const LweBootstrappingKeyFFT* bkfft_of_rows(const TGswSampleFFT* rows, int32_t* row) {
+    if (!rows) return nullptr;
+    std::lock_guard<std::mutex> g(g_bkfft_mu);
+    for (BkFftOwner* o : g_bkfft) {
+        const TGswSampleFFT* b = o->rows.data();
+        if (rows >= b && rows < b + o->rows.size()) {
+            *row = (int32_t)(rows - b);
+            return &o->pub;
+        }
+    }
+    return nullptr;
+}
+
 }  // namespace tfhe_amd
 
 using namespace tfhe_amd;
@@ -790,9 +831,10 @@ TFHE_AMD_EXPORT void tfhe_bootstrap_woKS_FFT(LweSample* result, const LweBootstr
     result->current_variance = 0.;
 }
 
-static tfhe_amd_device_key* key_of_rows(const TGswSampleFFT* bk) {
-    if (!bk || !bk->owner) die("TGswSampleFFT does not belong to a tfhe_amd key");
-    tfhe_amd_device_key* dk = engine_key(bk->owner);
+static tfhe_amd_device_key* key_of_rows(const TGswSampleFFT* bk, int32_t* row) {
+    const LweBootstrappingKeyFFT* bf = bkfft_of_rows(bk, row);
+    if (!bf) die("TGswSampleFFT does not belong to a tfhe_amd key");
+    tfhe_amd_device_key* dk = engine_key(bf);
     if (!dk) die(g_err.c_str());
     return dk;
 }
@@ -800,11 +842,12 @@ static tfhe_amd_device_key* key_of_rows(const TGswSampleFFT* bk) {
 // libtfhe/lwe-bootstrapping-functions-fft.cpp:88-112
 TFHE_AMD_EXPORT void tfhe_blindRotate_FFT(TLweSample* accum, const TGswSampleFFT* bk, const int32_t* bara, const int32_t n,
                                           const TGswParams* bk_params) {
-    tfhe_amd_device_key* dk = key_of_rows(bk);
+    int32_t row = 0;
+    tfhe_amd_device_key* dk = key_of_rows(bk, &row);
     const int N = bk_params->tlwe_params->N, k = bk_params->tlwe_params->k;
     std::vector<int32_t> acc((size_t)(k + 1) * N);
     for (int c = 0; c <= k; c++) memcpy(&acc[(size_t)c * N], accum->a[c].coefsT, sizeof(int32_t) * N);
-    if (engine_blind_rotate_host(dk, bk->row, n, bara, acc.data(), nullptr)) die(g_err.c_str());
+    if (engine_blind_rotate_host(dk, row, n, bara, acc.data(), nullptr)) die(g_err.c_str());
     for (int c = 0; c <= k; c++) memcpy(accum->a[c].coefsT, &acc[(size_t)c * N], sizeof(int32_t) * N);
 }
 
@@ -812,7 +855,8 @@ TFHE_AMD_EXPORT void tfhe_blindRotate_FFT(TLweSample* accum, const TGswSampleFFT
 TFHE_AMD_EXPORT void tfhe_blindRotateAndExtract_FFT(LweSample* result, const TorusPolynomial* v, const TGswSampleFFT* bk,
                                                     const int32_t barb, const int32_t* bara, const int32_t n,
                                                     const TGswParams* bk_params) {
-    tfhe_amd_device_key* dk = key_of_rows(bk);
+    int32_t row = 0;
+    tfhe_amd_device_key* dk = key_of_rows(bk, &row);
     const int N = bk_params->tlwe_params->N, k = bk_params->tlwe_params->k;
     std::vector<int32_t> acc((size_t)(k + 1) * N, 0), ext((size_t)k * N + 1);
     // test vector X^{2N-barb} v in the b component
@@ -822,7 +866,7 @@ TFHE_AMD_EXPORT void tfhe_blindRotateAndExtract_FFT(LweSample* result, const Tor
         int idx = ((j - a) % (2 * N) + 2 * N) % (2 * N);
         tv[j] = idx < N ? v->coefsT[idx] : S(0u - U(v->coefsT[idx - N]));
     }
-    if (engine_blind_rotate_host(dk, bk->row, n, bara, acc.data(), ext.data())) die(g_err.c_str());
+    if (engine_blind_rotate_host(dk, row, n, bara, acc.data(), ext.data())) die(g_err.c_str());
     memcpy(result->a, ext.data(), sizeof(int32_t) * k * N);
     result->b = ext[(size_t)k * N];
     result->current_variance = 0.;
