@@ -503,23 +503,30 @@ TFHE_AMD_EXPORT void tfhe_amd_server_destroy(tfhe_amd_server* srv);
  * per CU) on the fused-butterfly transform, 6 = the two-wavefront kernel (four
  * bootstraps per CU, same transform), 8 = the pair kernel on the radix-8
  * stages (kept for comparison; modes 6 and 8 run mode 2 for N = 1024). All
- * modes give bit-identical results. In automatic mode, for N = 1024, batches
- * of at most latency_max gates use mode 5, then up to small_batch_max mode 3,
- * larger ones mode 2 (a negative value keeps the current one); for N = 2048,
+ * modes give bit-identical results. In automatic mode, for N = 1024, a key the
+ * autotuner has measured takes its per-size choice (below) unless crossovers
+ * have been set here since; otherwise batches of at most latency_max gates use
+ * mode 5, then up to small_batch_max mode 3, larger ones mode 2 (a negative
+ * value keeps the current crossover; setting either one makes the crossovers
+ * apply instead of the per-size choice until the next autotune); for N = 2048,
  * a batch takes the kernel (4 or 6) the autotuner measured the faster at the
  * smallest of its ladder sizes that holds the batch (until it runs: mode 4 up to
  * 512 gates), batches beyond the ladder mode 6. */
 TFHE_AMD_EXPORT void tfhe_amd_set_launch_config(int32_t br_mode, int32_t latency_max, int32_t small_batch_max);
 TFHE_AMD_EXPORT int32_t tfhe_amd_get_launch_config(int32_t* br_mode, int32_t* latency_max, int32_t* small_batch_max);
 /* Autotuner: for an N = 1024 key, times modes 5, 3 and 2 at batch sizes 64,
- * 128, 256, 384, 512, 640, 768, 896, 1024, 1280, 1536 (milliseconds per batch
- * including the keyswitch, into ms[3][11] in that mode order, if not NULL) and
- * sets latency_max to the largest size up to which mode 5 is the fastest at
- * every measured point, small_batch_max to the largest size up to which mode 3
- * then beats mode 2 at every further point. For an N = 2048 key, times modes 4
- * and 6 at the same sizes (ms rows 0 and 1; row 2 zero), records the faster one
- * per size for automatic mode, and returns in small_batch_max the largest size
- * up to which mode 4 is the faster at every point (latency_max 0).
+ * 128, 256, 384, 512, 640, 768, 896, 1024, 1280, 1536, 2048, 3072, 4096
+ * (milliseconds per batch including the keyswitch, into ms[3][14] in that mode
+ * order, if not NULL), records the fastest mode per size for automatic mode (a
+ * batch takes the choice of the smallest of these sizes that holds it, mode 2
+ * beyond them) and sets the crossovers: latency_max to the largest size up to
+ * which mode 5 is the fastest at every measured point, small_batch_max to the
+ * largest size up to which mode 3 then beats mode 2 at every further point
+ * (they apply instead of the per-size choice once tfhe_amd_set_launch_config
+ * sets crossovers). For an N = 2048 key, times modes 4 and 6 at the same sizes
+ * (ms rows 0 and 1; row 2 zero), records the faster one per size for automatic
+ * mode, and returns in small_batch_max the largest size up to which mode 4 is
+ * the faster at every point (latency_max 0).
  * Synchronous; allocates and frees its own buffers. */
 TFHE_AMD_EXPORT int32_t tfhe_amd_autotune(const TFheGateBootstrappingCloudKeySet* ck, int32_t* latency_max,
                                           int32_t* small_batch_max, float* ms);

@@ -462,7 +462,7 @@ def decrypt_device(sk, d_cts, d_bits=None, d_phases=None, count=None, stream=Non
                                         d_phases.ptr if d_phases else None, stream), "decrypt_device")
 
 
-AUTOTUNE_SIZES = (64, 128, 256, 384, 512, 640, 768, 896, 1024, 1280, 1536)
+AUTOTUNE_SIZES = (64, 128, 256, 384, 512, 640, 768, 896, 1024, 1280, 1536, 2048, 3072, 4096)
 AUTOTUNE_MODES = ("digit_parallel", "two_wave_split", "one_per_wavefront_batched")
 
 

@@ -11,7 +11,7 @@
 //   br4_seg_kernel     the same over several clients' batches in one launch,
 //                      each segment with its own key (serving)
 //   br2_kernel         blind rotation, two wavefronts per bootstrap (one per TLWE
-//                      component), key blocks shared through LDS
+//                      component), partial sums handed over through LDS
 //   brl_kernel         blind rotation for one gate: 4l wavefronts per bootstrap on
 //                      half transforms
 //   brp_kernel         blind rotation for N = 2048: four wavefronts per bootstrap,
@@ -70,6 +70,8 @@ struct tfhe_amd_device_key {
                           // beats the two-wavefront kernel at the autotuner's ladder size i; a batch
                           // takes the choice of the smallest ladder size that holds it, batches
                           // beyond the ladder the two-wavefront kernel
+    uint8_t br_choice[16];  // M = 512, automatic mode after the autotuner: the fastest kernel (mode 5,
+                            // 3 or 2) at ladder size i; 0 before tuning (the crossovers apply)
     bool orphaned; // the host key has forgotten it (release or delete) while it was pinned
 };
 
@@ -945,206 +947,200 @@ __global__ __launch_bounds__(128) void bk_convert_half_kernel(const int32_t* __r
 }
 
 // largest LWE dimension the two-wavefront kernel's LDS table holds
-constexpr int BR2_NMAX = 768;
+constexpr int BR2_NMAX = 1024;
 
-// Blind rotation with two wavefronts per bootstrap (M = 512): wavefront c of a
-// bootstrap owns TLWE component c of the accumulator (in registers). Per key
-// row it rotates and decomposes its own component, transforms its l digit
-// polynomials and accumulates partial products for both output components;
-// the two wavefronts then swap the partial sum meant for the other component
-// through LDS and each inverse-transforms its own component. Half the latency
-// of one wavefront per bootstrap; a workgroup carries two bootstraps (four
-// wavefronts) that share the key digit blocks in LDS.
+// Blind rotation with two wavefronts per bootstrap on the fused-butterfly
+// transform (M = 512): one bootstrap per 128-thread workgroup, wavefront c owns
+// TLWE component c of the accumulator (in registers). Per key row it rotates
+// and decomposes its own component (through its 2N-word buffer, as br4_kernel),
+// transforms its l digit polynomials and multiply-accumulates them against key
+// rows (c l + p) of both output components, read straight from global memory;
+// then it hands the partial sum of the other component to its partner through
+// LDS, takes the partner's partial sum of its own, inverse-transforms and
+// updates its component. Two workgroup barriers per row (between the two
+// wavefronts of one bootstrap only); four bootstraps per CU.
 template <int L>
-__global__ __launch_bounds__(256, 2) void br2_kernel(BrArgs A, const int32_t* __restrict__ mux_gate,
+__global__ __launch_bounds__(128, 2) void br2_kernel(BrArgs A, const int32_t* __restrict__ mux_gate,
                                                     const int32_t* __restrict__ n_jobs) {
     constexpr int M = 512;
-    using F = Fft<M>;
-    constexpr int R = F::R, N = 2 * M, KPL = 2 * L;
-    constexpr int LOG2N2 = 11;
-    constexpr int NMAX = BR2_NMAX;
-    __shared__ c64 s_fft[4][F::LDS_C64];
-    __shared__ c64 s_bk[2][2 * M];  // key digit blocks of rows (c*L + p), c = 0, 1
-    __shared__ c64 s_tw1[R * 64];
-    __shared__ int16_t s_bara[2][NMAX];
+    using G = FftCT;
+    constexpr int R = 8, N = 2 * M, KPL = 2 * L, LOG2N2 = 11;
+    __shared__ c64 s_fft[2][G::LDS_C64];  // transform and rotation buffer of wavefront c
+    __shared__ c64 s_swap[2][M];          // s_swap[d]: partial sum of output d from wavefront 1 - d
+    __shared__ int16_t s_bara[BR2_NMAX];
+    __shared__ int32_t s_barb;
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
-    const int w = tid >> 6;
-    const int q = w >> 1;  // bootstrap slot in the workgroup
-    const int c = w & 1;   // accumulator component owned by this wavefront
-    const int njobs = *n_jobs;
-    const int job = blockIdx.x * 2 + q;
-    if (blockIdx.x * 2 >= njobs) return;  // whole block idle (uniform)
-    const bool active = job < njobs;
+    const int c = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int job = blockIdx.x;
+    if (job >= *n_jobs) return;  // whole block idle (uniform)
     const int n = A.n;
-    int16_t* bara = s_bara[q];
-    c64* fb = s_fft[w];
+    c64* fb = s_fft[c];
     int32_t* fbi = (int32_t*)fb;
 
-    // acc[h*R + s] = ACC_c[64 s + lane + h M]
-    uint32_t acc[2 * R];
-#pragma unroll
-    for (int k = 0; k < 2 * R; k++) acc[k] = 0;
     if (c == 1) {
-        if (active) {
-            const int gate = job < A.count ? job : mux_gate[job - A.count];
-            const int half = job < A.count ? 0 : 1;
-            const int op = A.op[gate];
-            int32_t k0, ca, cb, cc;
-            gate_coeffs(op, half, k0, ca, cb, cc);
-            const int32_t* a = A.pool + (size_t)A.in0[gate] * (n + 1);
-            const int32_t* b = (op == OP_IDENTITY) ? a : A.pool + (size_t)A.in1[gate] * (n + 1);
-            const int32_t* cp = (op == TFHE_AMD_MUX) ? A.pool + (size_t)A.in2[gate] * (n + 1) : a;
-            uint32_t barb = 0;
-            for (int i = lane; i <= n; i += 64) {
-                uint32_t v =
-                    (uint32_t)ca * (uint32_t)a[i] + (uint32_t)cb * (uint32_t)b[i] + (uint32_t)cc * (uint32_t)cp[i];
-                if (i == n) v += (uint32_t)k0;
-                // modSwitchFromTorus32(v, 2N), libtfhe/numeric-functions.cpp:59-65
-                const uint32_t bar = (v + (1u << (31 - LOG2N2))) >> (32 - LOG2N2);
-                if (i < n) bara[i] = (int16_t)bar;
-                else barb = bar;
-            }
-            barb = __shfl(barb, n & 63);
-            // ACC_1 = X^{2N-barb} * [mu...mu], lwe-bootstrapping-functions-fft.cpp:146-148
-#pragma unroll
-            for (int h = 0; h < 2; h++)
-#pragma unroll
-                for (int s = 0; s < R; s++) {
-                    const int j = 64 * s + lane + h * M;
-                    acc[h * R + s] = (((j + barb) & (2 * N - 1)) < N) ? (uint32_t)A.mu : 0u - (uint32_t)A.mu;
-                }
-        } else {
-            for (int i = lane; i < n; i += 64) bara[i] = 0;
+        // gate linear form and modulus switch (wavefront 1, while wavefront 0 loads its tables)
+        const int gate = job < A.count ? job : mux_gate[job - A.count];
+        const int half = job < A.count ? 0 : 1;
+        const int op = A.op[gate];
+        int32_t k0, ca, cb, cc;
+        gate_coeffs(op, half, k0, ca, cb, cc);
+        const int32_t* a = A.pool + (size_t)A.in0[gate] * (n + 1);
+        const int32_t* b = (op == OP_IDENTITY) ? a : A.pool + (size_t)A.in1[gate] * (n + 1);
+        const int32_t* cp = (op == TFHE_AMD_MUX) ? A.pool + (size_t)A.in2[gate] * (n + 1) : a;
+        for (int i = lane; i <= n; i += 64) {
+            uint32_t v =
+                (uint32_t)ca * (uint32_t)a[i] + (uint32_t)cb * (uint32_t)b[i] + (uint32_t)cc * (uint32_t)cp[i];
+            if (i == n) v += (uint32_t)k0;
+            // modSwitchFromTorus32(v, 2N), libtfhe/numeric-functions.cpp:59-65
+            const uint32_t bar = (v + (1u << (31 - LOG2N2))) >> (32 - LOG2N2);
+            if (i < n) s_bara[i] = (int16_t)bar;
+            else s_barb = (int32_t)bar;
         }
     }
-    for (int k = tid; k < R * 64; k += 256) s_tw1[k] = A.tw[(k & 63) * (R + 8) + (k >> 6)];
-    const c64* tw1 = s_tw1 + lane;
-    const c64 tw2 = A.tw[lane * (R + 8) + R + 1];
+    // the forward tables stay in registers for the whole kernel (32 VGPRs)
+    const ct64* twc_base = (const ct64*)A.twc;
+    ct64 fwb[4], fwc[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        fwb[k] = twc_base[G::FB + 64 * k + lane];
+        fwc[k] = twc_base[G::FC + 64 * k + lane];
+    }
     const uint32_t offset = A.offset;
     const int Bgbit = A.Bgbit;
     // the top bit of every digit field: with it flipped, the signed field is the balanced digit
     uint32_t flip = 0;
-    for (int pp = 0; pp < L; pp++) flip |= 1u << (31 - pp * Bgbit);
+    for (int p = 0; p < L; p++) flip |= 1u << (31 - p * Bgbit);
     __syncthreads();
-
-    for (int i = 0; i < n; i++) {
-        const int abar = bara[i];
-        c64 p0[R], p1[R];
+    // ACC_c in registers: acc[h*R + s] = ACC_c[64 s + lane + h M];
+    // ACC = (0, X^{2N-barb} * [mu...mu]), lwe-bootstrapping-functions-fft.cpp:146-148
+    uint32_t acc[2 * R];
+    {
+        const int barb = s_barb;
 #pragma unroll
-        for (int s = 0; s < R; s++) p0[s] = p1[s] = c64{0.0, 0.0};
-        uint32_t tv[2 * R];
-#pragma unroll 1
-        for (int p = 0; p < L; p++) {
-            __syncthreads();  // previous MAC is done with s_bk
-            {
-                // key digit blocks (i, p) and (i, L + p) -> LDS, 8 x 16 B per thread
-                const c64* q0 = A.bkf + ((size_t)i * KPL + p) * 2 * M + tid;
-                const c64* q1 = A.bkf + ((size_t)i * KPL + L + p) * 2 * M + tid;
-                c64* d0 = s_bk[0] + (tid & ~63);
-                c64* d1 = s_bk[1] + (tid & ~63);
-#pragma unroll
-                for (int u = 0; u < 4; u++) {
-                    __builtin_amdgcn_global_load_lds((const void*)(q0 + u * 256),
-                                                     (__attribute__((address_space(3))) void*)(d0 + u * 256), 16, 0, 0);
-                    __builtin_amdgcn_global_load_lds((const void*)(q1 + u * 256),
-                                                     (__attribute__((address_space(3))) void*)(d1 + u * 256), 16, 0, 0);
-                }
-            }
-            c64 x[R];
-            if (active) {
-                if (p == 0) {
-                    // T = (X^abar - 1) * ACC_c + gadget offset, through this wavefront's LDS buffer
-#pragma unroll
-                    for (int h = 0; h < 2; h++)
-#pragma unroll
-                        for (int s = 0; s < R; s++) fbi[64 * s + lane + h * M] = (int32_t)acc[h * R + s];
-                    wave_sync();
-#pragma unroll
-                    for (int h = 0; h < 2; h++)
-#pragma unroll
-                        for (int s = 0; s < R; s++) {
-                            const int j = 64 * s + lane + h * M;
-                            const int idx = (j - abar) & (2 * N - 1);
-                            const uint32_t v = (uint32_t)fbi[idx & (N - 1)];
-                            const uint32_t rot = (idx & N) ? 0u - v : v;
-                            tv[h * R + s] = (rot - acc[h * R + s] + offset) ^ flip;
-                        }
-                    wave_sync();
-                }
-                // digit p of the truncating gadget decomposition, tgsw-functions.cpp:337-348
-                const int shift = 32 - (p + 1) * Bgbit;
-#pragma unroll
-                for (int s = 0; s < R; s++) {
-                    const int32_t d0v = __builtin_amdgcn_sbfe((int32_t)tv[s], shift, Bgbit);
-                    const int32_t d1v = __builtin_amdgcn_sbfe((int32_t)tv[R + s], shift, Bgbit);
-                    x[s] = c64{(double)d0v, (double)d1v};
-                }
-                F::template fwd1<64>(x, lane, fb, tw1);
-                wave_sync();
-                F::fwd2(x, lane, fb, tw2);
-                wave_sync();
-                F::fwd3(x, lane, fb);
-                wave_sync();
-                F::fwd4(x, lane, fb);
-            }
-            __syncthreads();  // key blocks landed
-            if (active) {
-                const c64* bk = s_bk[c];
-                // eight key values in flight at a time, then their multiply-accumulates
-                c64 kv[R];
-#pragma unroll
-                for (int s = 0; s < R; s++) kv[s] = bk[s * 64 + lane];
-#pragma unroll
-                for (int s = 0; s < R; s++) cmac(p0[s], x[s], kv[s]);
-#pragma unroll
-                for (int s = 0; s < R; s++) kv[s] = bk[M + s * 64 + lane];
-#pragma unroll
-                for (int s = 0; s < R; s++) cmac(p1[s], x[s], kv[s]);
-            }
-        }
-        // swap partial sums: wavefront c keeps output c, hands over output 1-c
-        if (active) {
-            if (c) {
-#pragma unroll
-                for (int s = 0; s < R; s++) fb[s * 64 + lane] = p0[s];
-            } else {
-#pragma unroll
-                for (int s = 0; s < R; s++) fb[s * 64 + lane] = p1[s];
-            }
-        }
-        __syncthreads();
-        c64 x[R];
-        if (active) {
-            const c64* other = s_fft[w ^ 1];
-            if (c) {
-#pragma unroll
-                for (int s = 0; s < R; s++) x[s] = cadd(p1[s], other[s * 64 + lane]);
-            } else {
-#pragma unroll
-                for (int s = 0; s < R; s++) x[s] = cadd(p0[s], other[s * 64 + lane]);
-            }
-        }
-        __syncthreads();  // the partner has read this wavefront's buffer
-        if (active) {
-            // inverse transform of output component c and ACC_c += result
-            F::inv1(x, lane, fb);
-            wave_sync();
-            F::inv2(x, lane, fb, tw2);
-            wave_sync();
-            F::inv3(x, lane, fb);
-            wave_sync();
-            F::template inv4<64>(x, lane, fb, tw1);
+        for (int h = 0; h < 2; h++)
 #pragma unroll
             for (int s = 0; s < R; s++) {
-                acc[s] += (uint32_t)round_mod32(x[s].re);
-                acc[R + s] += (uint32_t)round_mod32(x[s].im);
+                const int j = 64 * s + lane + h * M;
+                acc[h * R + s] =
+                    c == 0 ? 0u : ((((j + barb) & (2 * N - 1)) < N) ? (uint32_t)A.mu : 0u - (uint32_t)A.mu);
             }
+    }
+
+    for (int i = 0; i < n; i++) {
+        const int abar = s_bara[i];
+        c64 r0[R], r1[R];  // partial spectra of output components 0 and 1
+        // T = (X^abar - 1) * ACC_c + gadget offset, through this wavefront's buffer: for
+        // abar <= N every rotated read E[j + 2N - abar] of E = (ACC_c, -ACC_c, ACC_c) falls
+        // in E[N, 3N) = (-ACC_c, ACC_c), otherwise in E[0, 2N) = (ACC_c, -ACC_c)
+        uint32_t tv[2 * R];
+        {
+            const bool low = abar <= N;
+            const int vofs = low ? N : 0, nofs = low ? 0 : N;
+            wave_sync();  // the previous inverse transform is done with the buffer
+#pragma unroll
+            for (int q = 0; q < 2 * R; q++) {
+                const int j = 64 * (q % R) + lane + (q / R) * M;
+                fbi[j + nofs] = (int32_t)(0u - acc[q]);
+                fbi[j + vofs] = (int32_t)acc[q];
+            }
+            wave_sync();
+            const int32_t* er = fbi + lane + (low ? N : 2 * N) - abar;
+#pragma unroll
+            for (int q = 0; q < 2 * R; q++) tv[q] = (uint32_t)er[64 * (q % R) + (q / R) * M];
+#pragma unroll
+            for (int q = 0; q < 2 * R; q++) tv[q] = (tv[q] - acc[q] + offset) ^ flip;
+        }
+        // digit p of this component: extraction (tgsw-functions.cpp:337-348, the signed field
+        // of the flipped word), forward transform, products with key digit polynomial
+        // (i, c l + p) of both output components (written, for the first digit)
+        auto digit = [&](const int p, auto first_tag) {
+            constexpr bool FIRST = decltype(first_tag)::value;
+            const int shift = 32 - (p + 1) * Bgbit;
+            c64 x[R];
+#pragma unroll
+            for (int s = 0; s < R; s++) {
+                const int32_t d0 = __builtin_amdgcn_sbfe((int32_t)tv[s], shift, Bgbit);
+                const int32_t d1 = __builtin_amdgcn_sbfe((int32_t)tv[R + s], shift, Bgbit);
+                x[s] = c64{(double)d0, (double)d1};
+            }
+            wave_sync();  // rotated reads or the previous transform done with the buffer
+            const c64* q = A.bkf + ((size_t)i * KPL + c * L + p) * 2 * M + lane;
+            c64 kv[R], kw[R / 2];
+#pragma unroll
+            for (int s = 0; s < R; s++) kv[s] = q[s * 64];
+            G::fwd1(x, lane, fb);
+            wave_sync();
+            G::fwd2(x, lane, fb, fwb);
+            wave_sync();
+            G::fwd3(x, lane, fb);
+#pragma unroll
+            for (int s = 0; s < R / 2; s++) kw[s] = q[M + s * 64];
+            wave_sync();
+            G::fwd4(x, lane, fb, fwc);
+#pragma unroll
+            for (int s = 0; s < R; s++) {
+                if (FIRST) r0[s] = cmul_first(x[s], kv[s]);
+                else cmac(r0[s], x[s], kv[s]);
+                if (s < R / 2) kv[s] = q[M + (s + R / 2) * 64];
+            }
+#pragma unroll
+            for (int s = 0; s < R; s++) {
+                const c64 k = s < R / 2 ? kw[s] : kv[s - R / 2];
+                if (FIRST) r1[s] = cmul_first(x[s], k);
+                else cmac(r1[s], x[s], k);
+            }
+        };
+        digit(0, std::true_type{});
+#pragma unroll 1
+        for (int p = 1; p < L; p++) digit(p, std::false_type{});
+        // hand the other component's partial sum to the partner, take this one's
+        __syncthreads();  // the partner has read the previous row's hand-over
+        c64 x[R];
+        if (c == 0) {
+#pragma unroll
+            for (int s = 0; s < R; s++) s_swap[1][s * 64 + lane] = r1[s];
+        } else {
+#pragma unroll
+            for (int s = 0; s < R; s++) s_swap[0][s * 64 + lane] = r0[s];
+        }
+        __syncthreads();  // both hand-overs written
+        if (c == 0) {
+#pragma unroll
+            for (int s = 0; s < R; s++) x[s] = cadd(r0[s], s_swap[0][s * 64 + lane]);
+        } else {
+#pragma unroll
+            for (int s = 0; s < R; s++) x[s] = cadd(r1[s], s_swap[1][s * 64 + lane]);
+        }
+        // inverse transform of component c and ACC_c += result
+        // (lwe-bootstrapping-functions-fft.cpp:67-76); the inverse tables are loaded
+        // once per row at an opaque offset, so they are not hoisted out of the loop
+        const ct64* twc = twc_base + opaque_zero() + lane;
+        ct64 iwb[4], iwa[4];
+        c64 iut[R];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            iwb[k] = twc[G::IB + 64 * k];
+            iwa[k] = twc[G::IA + 64 * k];
+        }
+#pragma unroll
+        for (int k = 0; k < R; k++) iut[k] = ((const c64*)twc)[G::UT + 64 * k];
+        wave_sync();
+        G::inv1(x, lane, fb);
+        wave_sync();
+        G::inv2(x, lane, fb, iwb);
+        wave_sync();
+        G::inv3(x, lane, fb);
+        wave_sync();
+        G::inv4(x, lane, fb, iwa, iut);
+#pragma unroll
+        for (int s = 0; s < R; s++) {
+            acc[s] += (uint32_t)round_mod32(x[s].re);
+            acc[R + s] += (uint32_t)round_mod32(x[s].im);
         }
     }
-    if (!active) return;
-    // tLweExtractLweSampleIndex(.., 0, ..), libtfhe/lwe.cpp:41-56
+    // tLweExtractLweSampleIndex(.., 0, ..), libtfhe/lwe.cpp:41-56: a'_0 = a_0, a'_m = -a_{N-m}, b' = b_0
     int32_t* e = A.ext + (size_t)job * (N + 1);
     if (c == 0) {
 #pragma unroll
@@ -2185,6 +2181,7 @@ struct Ctx {
     int br_mode = 0;
     int brl_max = 256;        // automatic mode: batches up to this many gates use the latency kernel (4l wavefronts per bootstrap)
     int br_small_max = 1024;  // automatic mode: batches up to this many gates use two wavefronts per bootstrap
+    bool crossovers_set = false;  // set_launch_config gave crossovers: they apply instead of the per-size choice
     char* noalloc_scr = nullptr;  // this device's copy of `scratch` (hipGetSymbolAddress at set-up)
     // optional per-launch timing with HIP events on the launch stream
     bool prof = false;
@@ -2422,8 +2419,19 @@ size_t ext_bytes(const HostParams& P, int count) {
 // ks_scratch_bytes(count). d_ext_out (optional) receives the summed extracted
 // samples instead of running the keyswitch.
 // the autotuner's ladder of batch sizes (gates)
-constexpr int TUNE_SIZES[] = {64, 128, 256, 384, 512, 640, 768, 896, 1024, 1280, 1536};
+constexpr int TUNE_SIZES[] = {64, 128, 256, 384, 512, 640, 768, 896, 1024, 1280, 1536, 2048, 3072, 4096};
 constexpr int TUNE_NS = sizeof(TUNE_SIZES) / sizeof(TUNE_SIZES[0]);
+static_assert(TUNE_NS <= 16, "per-size choice tables");
+
+// N = 1024, automatic mode: the autotuner's choice for a batch of `count` gates
+// (that of the smallest ladder size that holds it; the batched kernel beyond the
+// ladder), or 0 when the key has not been tuned or explicit crossovers are set
+int tuned_choice(const Ctx& C, const tfhe_amd_device_key* dk, int count) {
+    if (C.crossovers_set || !dk->br_choice[0]) return 0;
+    for (int si = 0; si < TUNE_NS; si++)
+        if (count <= TUNE_SIZES[si]) return dk->br_choice[si];
+    return 2;
+}
 
 // N = 2048, automatic mode: the pair kernel when it was the faster at the smallest
 // ladder size that holds the batch; beyond the ladder the two-wavefront kernel
@@ -2479,7 +2487,8 @@ int gates_device_locked(Ctx& C, tfhe_amd_device_key* dk, int count, const int32_
         mode = 2;
     }
     else if (mode == 4) mode = 2;
-    if (mode == 3 && dk->P.n > BR2_NMAX) mode = 2;
+    if (mode == 3 && (dk->P.n > BR2_NMAX || !dk->bkc)) mode = 2;
+    if (mode == 0) mode = tuned_choice(C, dk, count);
     if (mode == 0) {
         if (count <= C.brl_max && dk->P.n <= BRL_NMAX) mode = 5;
         else mode = (count <= C.br_small_max && dk->P.n <= BR2_NMAX) ? 3 : 2;
@@ -2514,9 +2523,11 @@ int gates_device_locked(Ctx& C, tfhe_amd_device_key* dk, int count, const int32_
         if (dk->P.l == 3) hipLaunchKernelGGL(br4_kernel<3>, dim3(blocks), dim3(64), 0, s, A, mux_gate, n_jobs);
         else hipLaunchKernelGGL(br4_kernel<2>, dim3(blocks), dim3(64), 0, s, A, mux_gate, n_jobs);
     } else if (mode == 3) {
-        const int blocks = count;  // two bootstraps per workgroup, at most 2*count jobs
-        if (dk->P.l == 3) hipLaunchKernelGGL(br2_kernel<3>, dim3(blocks), dim3(256), 0, s, A, mux_gate, n_jobs);
-        else hipLaunchKernelGGL(br2_kernel<2>, dim3(blocks), dim3(256), 0, s, A, mux_gate, n_jobs);
+        const int blocks = 2 * count;  // one bootstrap per two-wavefront workgroup, at most 2*count jobs
+        A.bkf = dk->bkc;               // the fused-butterfly transform's key and tables
+        A.twc = dk->twc;
+        if (dk->P.l == 3) hipLaunchKernelGGL(br2_kernel<3>, dim3(blocks), dim3(128), 0, s, A, mux_gate, n_jobs);
+        else hipLaunchKernelGGL(br2_kernel<2>, dim3(blocks), dim3(128), 0, s, A, mux_gate, n_jobs);
     } else {
         launch_br(dk, A, 2 * count, s);
     }
@@ -2646,6 +2657,7 @@ static int upload_key(Ctx& C, const LweBootstrappingKeyFFT* bf, tfhe_amd_device_
     dk->pins = 0;
     dk->orphaned = false;
     dk->pair_sizes = 0x1f;  // until the autotuner measures it: the pair kernel up to 512 gates
+    memset(dk->br_choice, 0, sizeof(dk->br_choice));
     dk->P = P;
     dk->M = M;
     dk->kpl = kpl;
@@ -3144,6 +3156,7 @@ TFHE_AMD_EXPORT void tfhe_amd_set_launch_config(int32_t br_mode, int32_t latency
     C.br_mode = br_mode;
     if (latency_max >= 0) C.brl_max = latency_max;
     if (small_batch_max >= 0) C.br_small_max = small_batch_max;
+    if (latency_max >= 0 || small_batch_max >= 0) C.crossovers_set = true;
 }
 
 TFHE_AMD_EXPORT int32_t tfhe_amd_get_launch_config(int32_t* br_mode, int32_t* latency_max, int32_t* small_batch_max) {
@@ -3261,12 +3274,20 @@ TFHE_AMD_EXPORT int32_t tfhe_amd_autotune(const TFheGateBootstrappingCloudKeySet
                 for (int k = 0; k < NS; k++) ms[m * NS + k] = m < nm ? best[m][k] : 0.0f;
         return cleanup(0);
     }
+    // per ladder size, the fastest kernel (automatic mode); the crossovers reported are the
+    // largest size up to which the latency kernel is the fastest at every point and, after
+    // it, the largest size up to which the two-wavefront kernel beats the batched one
+    for (int k = 0; k < NS; k++) {
+        const int m = (best[0][k] <= best[1][k] && best[0][k] <= best[2][k]) ? 0 : (best[1][k] < best[2][k] ? 1 : 2);
+        dk->br_choice[k] = (uint8_t)modes[m];
+    }
     int si = 0, lat = 0, small = 0;
     for (; si < NS && best[0][si] <= best[1][si] && best[0][si] <= best[2][si]; si++) lat = sizes[si];
     small = lat;
     for (; si < NS && best[1][si] < best[2][si]; si++) small = sizes[si];
     C.brl_max = lat;
     C.br_small_max = small;
+    C.crossovers_set = false;
     if (latency_max) *latency_max = lat;
     if (small_batch_max) *small_batch_max = small;
     if (ms)
