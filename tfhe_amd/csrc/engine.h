@@ -63,13 +63,16 @@ void engine_forget_secret_key(const LweKey* k);
 
 // For the level scheduler: device scratch a batch of `count` gates needs, the
 // engine lock, its stream, and a batch launch with caller-provided scratch
-// (call with engine_mutex() held; `capturing` disables event timing).
+// (call with engine_mutex() held; `capturing` disables event timing; `shared`:
+// other launches are running on the device, so automatic mode does not pick the
+// latency kernel, which takes a whole CU per bootstrap).
 size_t engine_batch_scratch_bytes(const tfhe_amd_device_key* dk, int count);
 std::mutex& engine_mutex();
 void* engine_stream();
 int engine_launch_batch_locked(tfhe_amd_device_key* dk, int count, const int32_t* d_op, const int32_t* d_in0,
                                const int32_t* d_in1, const int32_t* d_in2, const int32_t* d_out, int32_t* d_pool,
-                               char* scr, void* stream, bool capturing);
+                               char* scr, void* stream, bool capturing,
+                               bool shared = false);
 // Several clients' batches (segments) in one blind-rotation launch, each segment
 // with its own key (same parameters, N = 1024), then each segment's keyswitch.
 // d_idx holds [5][total] (op, in0, in1, in2, out) with segment g at

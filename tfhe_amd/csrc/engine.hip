@@ -2444,7 +2444,7 @@ bool pair_kernel_chosen(const tfhe_amd_device_key* dk, int count) {
 int gates_device_locked(Ctx& C, tfhe_amd_device_key* dk, int count, const int32_t* d_op, const int32_t* d_in0,
                         const int32_t* d_in1, const int32_t* d_in2, const int32_t* d_out, int32_t* d_pool,
                         int32_t* d_ext_out, int32_t mu, char* scr, hipStream_t s, bool allow_prof = true,
-                        int ks_path = -1, int32_t* d_nz = nullptr) {
+                        int ks_path = -1, int32_t* d_nz = nullptr, bool shared = false) {
     if (count <= 0) return count == 0 ? 0 : set_error("negative gate count");
     const HostParams& P = dk->P;
     const size_t eb = ext_bytes(P, count);
@@ -2488,10 +2488,15 @@ int gates_device_locked(Ctx& C, tfhe_amd_device_key* dk, int count, const int32_
     }
     else if (mode == 4) mode = 2;
     if (mode == 3 && (dk->P.n > BR2_NMAX || !dk->bkc)) mode = 2;
-    if (mode == 0) mode = tuned_choice(C, dk, count);
     if (mode == 0) {
-        if (count <= C.brl_max && dk->P.n <= BRL_NMAX) mode = 5;
-        else mode = (count <= C.br_small_max && dk->P.n <= BR2_NMAX) ? 3 : 2;
+        mode = tuned_choice(C, dk, count);
+        if (mode == 0) {
+            if (count <= C.brl_max && dk->P.n <= BRL_NMAX) mode = 5;
+            else mode = (count <= C.br_small_max && dk->P.n <= BR2_NMAX) ? 3 : 2;
+        }
+        // beside other launches the latency kernel (one bootstrap per CU) would hold CUs
+        // that the others' bootstraps could share; the two-wavefront kernel fits four per CU
+        if (shared && mode == 5) mode = dk->P.n <= BR2_NMAX ? 3 : 2;
     }
     if (mode == 5 && (dk->M != 512 || dk->P.n > BRL_NMAX)) mode = 3;
     if (mode == 5 && !dk->bkh) mode = 3;
@@ -2564,9 +2569,9 @@ void* engine_stream() {
 
 int engine_launch_batch_locked(tfhe_amd_device_key* dk, int count, const int32_t* d_op, const int32_t* d_in0,
                                const int32_t* d_in1, const int32_t* d_in2, const int32_t* d_out, int32_t* d_pool,
-                               char* scr, void* stream, bool capturing) {
+                               char* scr, void* stream, bool capturing, bool shared) {
     return gates_device_locked(ctx(), dk, count, d_op, d_in0, d_in1, d_in2, d_out, d_pool, nullptr, 1 << 29, scr,
-                               (hipStream_t)stream, !capturing);
+                               (hipStream_t)stream, !capturing, -1, nullptr, shared);
 }
 
 bool engine_can_segment(const tfhe_amd_device_key* dk) {

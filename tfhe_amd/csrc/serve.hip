@@ -326,10 +326,14 @@ int tfhe_amd_server::launch_client(int c, bool packing, bool* progress) {
     int32_t* d = C.d_idx + (size_t)ring * 5 * max_batch;
     if (hipMemcpyAsync(d, h, sizeof(int32_t) * 5 * (size_t)count, hipMemcpyHostToDevice, s) != hipSuccess)
         return set_error("server: index upload failed");
+    // other launches still on the device (any client's or a packed one): the kernel choice
+    // then leaves out the latency kernel, which takes a whole CU per bootstrap
+    size_t others = pack_inflight.size();
+    for (const Client* o : worker_clients) others += o->inflight.size();
     {
         std::lock_guard<std::mutex> g(engine_mutex());
         if (int rc = engine_launch_batch_locked(C.dk, count, d, d + count, d + 2 * count, d + 3 * count,
-                                                d + 4 * count, arena, scratch[si], s, true))
+                                                d + 4 * count, arena, scratch[si], s, true, others > 0))
             return rc;
     }
     Launch L;
