@@ -128,6 +128,14 @@ __device__ __forceinline__ int opaque_zero() {
     return z;
 }
 
+// a + b + c mod 2^32 in one instruction (c wave-uniform); the compiler splits the
+// sum into two adds when it sees -x + uniform
+__device__ __forceinline__ uint32_t add3_u32(uint32_t a, uint32_t b, uint32_t c) {
+    uint32_t r;
+    asm("v_add3_u32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "s"(c));
+    return r;
+}
+
 // round to nearest, result mod 2^32 (valid for |v| < 2^51)
 __device__ __forceinline__ int32_t round_mod32(double v) {
     const double t = v + 6755399441055744.0;  // 1.5 * 2^52
@@ -496,19 +504,20 @@ __device__ __forceinline__ void br4_body(BrArgs A, const int32_t* __restrict__ m
                     const bool low = abar <= N;
                     const int vofs = low ? N : 0, nofs = low ? 0 : N;
                     wave_sync();  // the previous transform is done with the buffer
+                    uint32_t nv[2 * R];  // -ACC_c, for the buffer and for T (one three-input add)
 #pragma unroll
                     for (int q = 0; q < 2 * R; q++) {
                         const int j = 64 * (q % R) + lane + (q / R) * M;
-                        const uint32_t v = acc[c][q];
-                        fbi[j + nofs] = (int32_t)(0u - v);
-                        fbi[j + vofs] = (int32_t)v;
+                        nv[q] = 0u - acc[c][q];
+                        fbi[j + nofs] = (int32_t)nv[q];
+                        fbi[j + vofs] = (int32_t)acc[c][q];
                     }
                     wave_sync();
                     const int32_t* er = fbi + lane + (low ? N : 2 * N) - abar;
 #pragma unroll
                     for (int q = 0; q < 2 * R; q++) tv[q] = (uint32_t)er[64 * (q % R) + (q / R) * M];
 #pragma unroll
-                    for (int q = 0; q < 2 * R; q++) tv[q] = (tv[q] - acc[c][q] + offset) ^ flip;
+                    for (int q = 0; q < 2 * R; q++) tv[q] = add3_u32(tv[q], nv[q], offset) ^ flip;
                 }
                 int p0 = 0;
                 if (c == 0) {
@@ -1039,10 +1048,12 @@ __global__ __launch_bounds__(128, 2) void br2_kernel(BrArgs A, const int32_t* __
             const bool low = abar <= N;
             const int vofs = low ? N : 0, nofs = low ? 0 : N;
             wave_sync();  // the previous inverse transform is done with the buffer
+            uint32_t nv[2 * R];  // -ACC_c, for the buffer and for T (one three-input add)
 #pragma unroll
             for (int q = 0; q < 2 * R; q++) {
                 const int j = 64 * (q % R) + lane + (q / R) * M;
-                fbi[j + nofs] = (int32_t)(0u - acc[q]);
+                nv[q] = 0u - acc[q];
+                fbi[j + nofs] = (int32_t)nv[q];
                 fbi[j + vofs] = (int32_t)acc[q];
             }
             wave_sync();
@@ -1050,7 +1061,7 @@ __global__ __launch_bounds__(128, 2) void br2_kernel(BrArgs A, const int32_t* __
 #pragma unroll
             for (int q = 0; q < 2 * R; q++) tv[q] = (uint32_t)er[64 * (q % R) + (q / R) * M];
 #pragma unroll
-            for (int q = 0; q < 2 * R; q++) tv[q] = (tv[q] - acc[q] + offset) ^ flip;
+            for (int q = 0; q < 2 * R; q++) tv[q] = add3_u32(tv[q], nv[q], offset) ^ flip;
         }
         // digit p of this component: extraction (tgsw-functions.cpp:337-348, the signed field
         // of the flipped word), forward transform, products with key digit polynomial
