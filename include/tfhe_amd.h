@@ -514,6 +514,11 @@ TFHE_AMD_EXPORT void tfhe_amd_server_destroy(tfhe_amd_server* srv);
  * 512 gates), batches beyond the ladder mode 6. */
 TFHE_AMD_EXPORT void tfhe_amd_set_launch_config(int32_t br_mode, int32_t latency_max, int32_t small_batch_max);
 TFHE_AMD_EXPORT int32_t tfhe_amd_get_launch_config(int32_t* br_mode, int32_t* latency_max, int32_t* small_batch_max);
+/* The blind-rotation kernel (mode number as above, never 0) that a batch of
+ * `count` gates under this key runs with the current settings, for a launch
+ * with nothing else on the device; uploads the key if needed; -1 on error.
+ * No counterpart in the reference: for tests and tuning reports. */
+TFHE_AMD_EXPORT int32_t tfhe_amd_kernel_for(const TFheGateBootstrappingCloudKeySet* ck, int32_t count);
 /* Autotuner: for an N = 1024 key, times modes 5, 3 and 2 at batch sizes 64,
  * 128, 256, 384, 512, 640, 768, 896, 1024, 1280, 1536, 2048, 3072, 4096
  * (milliseconds per batch including the keyswitch, into ms[3][14] in that mode

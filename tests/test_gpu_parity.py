@@ -351,6 +351,16 @@ def test_autotune_crossovers_and_auto_mode(p128):
         assert 0 <= lat <= small <= max(T.AUTOTUNE_SIZES)
         assert all(v > 0 for t in table.values() for v in t.values())
         assert T.get_launch_config()[1:] == (lat, small)
+        # automatic mode takes the fastest measured kernel of the smallest ladder size that
+        # holds the batch (modes 5, 3, 2 in the table's order), the batched kernel beyond it
+        names = list(T.AUTOTUNE_MODES)
+        for k, size in enumerate(T.AUTOTUNE_SIZES):
+            t5, t3, t2 = (table[nm][size] for nm in names)
+            want = 5 if (t5 <= t3 and t5 <= t2) else (3 if t3 < t2 else 2)
+            assert T.kernel_for(sk.cloud, size) == want, size
+            if k:
+                assert T.kernel_for(sk.cloud, T.AUTOTUNE_SIZES[k - 1] + 1) == want, size
+        assert T.kernel_for(sk.cloud, max(T.AUTOTUNE_SIZES) + 1) == 2
         rng = np.random.default_rng(31)
         for count in sorted({max(1, lat), lat + 1, small, small + 1}):
             ops = rng.integers(0, 11, size=count).astype(np.int32)
@@ -362,6 +372,9 @@ def test_autotune_crossovers_and_auto_mode(p128):
             finally:
                 T.set_launch_config(0)
             assert np.array_equal(auto, forced), count
+        # explicit crossovers replace the per-size choice
+        T.set_launch_config(0, 100, 900)
+        assert [T.kernel_for(sk.cloud, c) for c in (1, 100, 101, 900, 901)] == [5, 5, 3, 3, 2]
     finally:
         T.set_launch_config(0, 256, 1024)
 

@@ -205,8 +205,14 @@ def test_pbig_autotune_chooses_kernel_shape(pbig):
     # the crossover is where the measured ladder says it is
     sizes = [s for s in T.AUTOTUNE_SIZES if s <= pair_max]
     assert all(table["pair_four_wave"][s] <= table["two_wave"][s] for s in sizes)
+    # automatic mode: the faster kernel of the smallest ladder size that holds the batch,
+    # the two-wavefront kernel beyond the ladder
+    for size in T.AUTOTUNE_SIZES:
+        want = 4 if table["pair_four_wave"][size] <= table["two_wave"][size] else 6
+        assert T.kernel_for(sk.cloud, size) == want, size
+    assert T.kernel_for(sk.cloud, max(T.AUTOTUNE_SIZES) + 1) == 6
     rng = np.random.default_rng(48)
-    for count in (max(pair_max, 1), pair_max + 130, 1600):
+    for count in (max(pair_max, 1), pair_max + 130, max(T.AUTOTUNE_SIZES) + 100):
         ops = rng.integers(0, 11, size=count).astype(np.int32)
         bits = rng.integers(0, 2, size=(3, count)).astype(np.int32)
         a, b, c = (sk.encrypt(x) for x in bits)

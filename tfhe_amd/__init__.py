@@ -154,6 +154,7 @@ _SIGS = {
     "tfhe_amd_set_launch_config": (None, [i32, i32, i32]),
     "tfhe_amd_set_keyswitch_path": (None, [i32]),
     "tfhe_amd_get_launch_config": (i32, [i32p, i32p, i32p]),
+    "tfhe_amd_kernel_for": (i32, [CKP, i32]),
     "tfhe_amd_autotune": (i32, [CKP, i32p, i32p, f32p]),
     "tfhe_amd_profile_enable": (None, [i32]),
     "tfhe_amd_profile_read": (i32, [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double), i32p]),
@@ -483,6 +484,14 @@ def autotune(cloud):
     table = {name: {sz: ms[m * len(AUTOTUNE_SIZES) + i] for i, sz in enumerate(AUTOTUNE_SIZES)}
              for m, name in enumerate(names)}
     return lat.value, small.value, table
+
+
+def kernel_for(cloud, count):
+    """The blind-rotation kernel (mode) a batch of `count` gates runs with now."""
+    m = lib().tfhe_amd_kernel_for(cloud, int(count))
+    if m < 0:
+        check(m, "kernel_for")
+    return m
 
 
 def set_launch_config(mode, latency_max=-1, small_batch_max=-1):

@@ -2452,6 +2452,36 @@ bool pair_kernel_chosen(const tfhe_amd_device_key* dk, int count) {
     return false;
 }
 
+// The blind-rotation kernel (mode) a batch of `count` gates runs with: the forced mode, or
+// in automatic mode the autotuner's per-size choice or the crossovers (shared: other launches
+// are on the device, so not the latency kernel), mapped to a kernel the key supports.
+int resolve_mode(const Ctx& C, const tfhe_amd_device_key* dk, int count, bool shared) {
+    int mode = C.br_mode;
+    if (dk->M != 512) {
+        // N = 2048: 1 one-wavefront, 4 pair (four wavefronts), 6 two-wavefront, 8 pair on the
+        // radix-8 stages; otherwise automatic: the autotuner's (or the default) per-size choice
+        if (mode == 1 || dk->P.n > BRP_NMAX || !dk->bkp) return 1;
+        if (mode == 8 || !dk->bkq) return 8;
+        if (mode != 4 && mode != 6) mode = pair_kernel_chosen(dk, count) ? 4 : 6;
+        return mode;
+    }
+    if (mode == 8 || mode == 6 || mode == 4) mode = 2;
+    if (mode == 0) {
+        mode = tuned_choice(C, dk, count);
+        if (mode == 0) {
+            if (count <= C.brl_max && dk->P.n <= BRL_NMAX) mode = 5;
+            else mode = (count <= C.br_small_max && dk->P.n <= BR2_NMAX) ? 3 : 2;
+        }
+        // beside other launches the latency kernel (one bootstrap per CU) would hold CUs
+        // that the others' bootstraps could share; the two-wavefront kernel fits four per CU
+        if (shared && mode == 5) mode = 3;
+    }
+    if (mode == 5 && (dk->P.n > BRL_NMAX || !dk->bkh)) mode = 3;
+    if (mode == 3 && (dk->P.n > BR2_NMAX || !dk->bkc)) mode = 2;
+    if (mode == 2 && !dk->bkc) mode = 7;
+    return mode;
+}
+
 int gates_device_locked(Ctx& C, tfhe_amd_device_key* dk, int count, const int32_t* d_op, const int32_t* d_in0,
                         const int32_t* d_in1, const int32_t* d_in2, const int32_t* d_out, int32_t* d_pool,
                         int32_t* d_ext_out, int32_t mu, char* scr, hipStream_t s, bool allow_prof = true,
@@ -2487,31 +2517,7 @@ int gates_device_locked(Ctx& C, tfhe_amd_device_key* dk, int count, const int32_
     // (5 for the smallest batches, 3 while the batch leaves CUs idle, 2 for large
     // batches). N = 2048 uses the pair-transform kernel (4: fused butterflies, 8: radix-8 stages)
     // unless mode 1 is forced.
-    int mode = C.br_mode;
-    if (dk->M != 512) {
-        // N = 2048: 1 one-wavefront, 4 pair (four wavefronts), 6 two-wavefront, 8 pair on the
-        // radix-8 stages; otherwise automatic: the pair kernel up to pair_max gates
-        if (mode == 1 || dk->P.n > BRP_NMAX || !dk->bkp) mode = 1;
-        else if (mode == 8 || !dk->bkq) mode = 8;
-        else if (mode != 4 && mode != 6) mode = pair_kernel_chosen(dk, count) ? 4 : 6;
-    } else if (mode == 8 || mode == 6) {
-        mode = 2;
-    }
-    else if (mode == 4) mode = 2;
-    if (mode == 3 && (dk->P.n > BR2_NMAX || !dk->bkc)) mode = 2;
-    if (mode == 0) {
-        mode = tuned_choice(C, dk, count);
-        if (mode == 0) {
-            if (count <= C.brl_max && dk->P.n <= BRL_NMAX) mode = 5;
-            else mode = (count <= C.br_small_max && dk->P.n <= BR2_NMAX) ? 3 : 2;
-        }
-        // beside other launches the latency kernel (one bootstrap per CU) would hold CUs
-        // that the others' bootstraps could share; the two-wavefront kernel fits four per CU
-        if (shared && mode == 5) mode = dk->P.n <= BR2_NMAX ? 3 : 2;
-    }
-    if (mode == 5 && (dk->M != 512 || dk->P.n > BRL_NMAX)) mode = 3;
-    if (mode == 5 && !dk->bkh) mode = 3;
-    if (mode == 2 && !dk->bkc) mode = 7;
+    const int mode = resolve_mode(C, dk, count, shared);
     if (mode == 5) {
         const int blocks = 2 * count;  // one bootstrap per workgroup, at most 2*count jobs
         if (dk->P.l == 3) hipLaunchKernelGGL(brl_kernel<3>, dim3(blocks), dim3(768), 0, s, A, dk->bkh, dk->twh, mux_gate, n_jobs);
@@ -3173,6 +3179,16 @@ TFHE_AMD_EXPORT void tfhe_amd_set_launch_config(int32_t br_mode, int32_t latency
     if (latency_max >= 0) C.brl_max = latency_max;
     if (small_batch_max >= 0) C.br_small_max = small_batch_max;
     if (latency_max >= 0 || small_batch_max >= 0) C.crossovers_set = true;
+}
+
+TFHE_AMD_EXPORT int32_t tfhe_amd_kernel_for(const TFheGateBootstrappingCloudKeySet* ck, int32_t count) {
+    if (!ck || !ck->bkFFT) return set_error("kernel_for: null cloud key");
+    if (count <= 0) return set_error("kernel_for: the gate count must be positive");
+    tfhe_amd_device_key* dk = engine_key(ck->bkFFT);
+    if (!dk) return -1;
+    Ctx& C = ctx();
+    std::lock_guard<std::mutex> g(C.mu);
+    return resolve_mode(C, dk, count, false);
 }
 
 TFHE_AMD_EXPORT int32_t tfhe_amd_get_launch_config(int32_t* br_mode, int32_t* latency_max, int32_t* small_batch_max) {
