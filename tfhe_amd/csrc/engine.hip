@@ -1609,7 +1609,7 @@ __global__ __launch_bounds__(128, 2) void brq_kernel(BrArgs A, const c64* __rest
 
     // coefficient positions: own quarters h, h + 2 from jo = 512 h + lane, the partner's
     // 1 - h, 3 - h from jp = 512 (1 - h) + lane, plus 1024 q + 64 a (compile-time)
-    const int jo = 512 * h + lane, jp = 512 * (1 - h) + lane;
+    const int jo = 512 * h + lane;
     for (int i = 0; i < n; i++) {
         const int abar = s_bara[i];
         c64 r0[R], r1[R];
@@ -1623,23 +1623,29 @@ __global__ __launch_bounds__(128, 2) void brq_kernel(BrArgs A, const c64* __rest
             // +-ACC_c[x / 4 mod N], its sign bit log2(N) of x / 4 (x / 4 < 3N)
             uint32_t t[4 * R];
             {
+                // each wavefront forms T for its own quarters (slots k < 2R) and hands them to
+                // its partner through its transform buffer, idle at this point, taking the
+                // partner's (slots 2R + k: the partner's own quarters are this one's jp)
                 const char* ac = (const char*)s_acc[c];
-                // the same for both components: computed per component (opaque offset), since
-                // kept across the loop they would need ~100 registers
                 const int z = opaque_zero();
                 const uint32_t xo = ((uint32_t)(jo + z - abar) & (2 * N - 1)) * 4u;
-                const uint32_t xp = ((uint32_t)(jp + z - abar) & (2 * N - 1)) * 4u;
                 const int32_t* ao = s_acc[c] + jo;
-                const int32_t* ap = s_acc[c] + jp;
+                uint32_t* tb = (uint32_t*)fb;
 #pragma unroll
-                for (int k = 0; k < 4 * R; k++) {
-                    const int g = k / (2 * R), q = (k / R) & 1, a = k % R;
-                    const uint32_t x = (g ? xp : xo) + 4u * (uint32_t)(1024 * q + 64 * a);
+                for (int k = 0; k < 2 * R; k++) {
+                    const int q = k / R, a = k % R;
+                    const uint32_t x = xo + 4u * (uint32_t)(1024 * q + 64 * a);
                     const uint32_t m = (uint32_t)__builtin_amdgcn_sbfe((int32_t)x, 2 + LOG2N2 - 1, 1);
                     const uint32_t rv = *(const uint32_t*)(ac + (x & (4u * N - 1)));
-                    const uint32_t av = (uint32_t)(g ? ap : ao)[1024 * q + 64 * a];
+                    const uint32_t av = (uint32_t)ao[1024 * q + 64 * a];
                     t[k] = ((rv ^ m) - m) - av + offset;
+                    tb[k * 64 + lane] = t[k];
                 }
+                __syncthreads();  // both halves of T written
+                const uint32_t* tp = (const uint32_t*)s_fft[h ^ 1];
+#pragma unroll
+                for (int k = 0; k < 2 * R; k++) t[2 * R + k] = tp[k * 64 + lane];
+                __syncthreads();  // the partner has read this wavefront's half
             }
 #pragma unroll 1
             for (int p = 0; p < L; p++) {
