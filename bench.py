@@ -625,10 +625,13 @@ def main():
 
     mode_cmp = None
     if not args.no_compare_modes:
-        # the batched kernel on both transforms, alternating, same batch (gates/s)
-        mode_cmp = {"radix8_transform": [], "fused_butterfly_transform": []}
+        # the batched kernel on both transforms, and on the fused-butterfly transform without
+        # the row-synchronised eight-bootstrap workgroups, alternating, same batch (gates/s)
+        mode_cmp = {"radix8_transform": [], "fused_butterfly_one_bootstrap_workgroups": [],
+                    "fused_butterfly_transform": []}
         for _ in range(2):
-            for mode, name in ((7, "radix8_transform"), (2, "fused_butterfly_transform")):
+            for mode, name in ((7, "radix8_transform"), (9, "fused_butterfly_one_bootstrap_workgroups"),
+                               (2, "fused_butterfly_transform")):
                 T.lib().tfhe_amd_set_launch_config(mode, -1, -1)
                 step()
                 T.synchronize()
@@ -638,7 +641,7 @@ def main():
                 T.synchronize()
                 mode_cmp[name].append(round(B * args.steps / (time.perf_counter() - t1), 1))
         T.lib().tfhe_amd_set_launch_config(0, -1, -1)
-        print(f"[bench] batched kernel, both transforms: {mode_cmp}", file=sys.stderr, flush=True)
+        print(f"[bench] batched kernel variants: {mode_cmp}", file=sys.stderr, flush=True)
 
     # one gate per launch (same kernels, count = 1, synchronised each time)
     lat = []

@@ -496,9 +496,12 @@ TFHE_AMD_EXPORT void tfhe_amd_server_destroy(tfhe_amd_server* srv);
 /* Blind-rotation kernel choice. mode: 0 = automatic, 1 = one wavefront per
  * bootstrap, 2 = one wavefront per bootstrap with the key and twiddles read
  * through the vector memory path (the batched kernel, on the fused-butterfly
- * transform), 7 = the same kernel on the radix-8 transform (kept for
- * comparison), 3 = two wavefronts per bootstrap, 5 = 4l wavefronts per
- * bootstrap (lowest latency; modes 2, 3, 5, 7 for N = 1024); for N = 2048:
+ * transform: the batch's whole device rounds on workgroups of eight such
+ * bootstraps held together by a barrier per key row, the rest on workgroups of
+ * one), 9 = the batched kernel on workgroups of one only, 7 = the same on the
+ * radix-8 transform (9 and 7 kept for comparison), 3 = two wavefronts per
+ * bootstrap, 5 = 4l wavefronts per bootstrap (lowest latency; modes 2, 3, 5, 7,
+ * 9 for N = 1024); for N = 2048:
  * 4 = the pair-transform kernel (four wavefronts per bootstrap, two bootstraps
  * per CU) on the fused-butterfly transform, 6 = the two-wavefront kernel (four
  * bootstraps per CU, same transform), 8 = the pair kernel on the radix-8

@@ -86,6 +86,32 @@ def test_ragged_batch_in_the_batched_kernel(key):
     assert np.array_equal(got[sel], O.gates(keys, ops[sel], a[sel], b[sel], c[sel], threads=8))
 
 
+def test_row_synchronised_rounds_match_one_bootstrap_workgroups(key):
+    """4133 gates with MUX gates: mode 2 runs the whole device rounds (4096 jobs on
+    256 CUs) on the eight-bootstrap workgroups with a barrier per key row and the
+    rest, from job 4096 on, on one-bootstrap workgroups; mode 9 runs every job on
+    the latter. Bit-identical, and gates from both parts equal the oracle."""
+    T, P, sk, keys = key
+    rng = np.random.default_rng(81)
+    count = 4133
+    ops = rng.integers(0, 11, size=count).astype(np.int32)
+    ops[[0, 2047, 4095, 4100]] = 10
+    bits = rng.integers(0, 2, size=(3, count)).astype(np.int32)
+    a, b, c = (sk.encrypt(x) for x in bits)
+    got = {}
+    try:
+        for mode in (2, 9):
+            T.set_launch_config(mode)
+            assert T.kernel_for(sk.cloud, count) == mode
+            got[mode] = T.run_gates(sk.cloud, ops, a, b, c)
+    finally:
+        T.set_launch_config(0)
+    assert np.array_equal(got[2], got[9])
+    assert np.array_equal(sk.decrypt(got[2]), _truth(ops, bits))
+    sel = [0, 2047, 4095, 4100, 4132]
+    assert np.array_equal(got[2][sel], O.gates(keys, ops[sel], a[sel], b[sel], c[sel], threads=8))
+
+
 def test_very_large_launch(key):
     """131,072 gates in one launch over a pool of 4096 inputs (so every index and
     workspace size is well past 2^16 gates)."""
