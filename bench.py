@@ -79,7 +79,7 @@ def parse():
 def profiled_traffic():
     """HBM bytes per batch launch of the blind-rotation kernel, from the
     calibrated rocprofv3 FETCH_SIZE/WRITE_SIZE passes summarised in profiles/."""
-    path = os.path.join(ROOT, "profiles", "br4_kernel_p128_batch4096.md")
+    path = os.path.join(ROOT, "profiles", "br4w_kernel_p128_batch4096.md")
     try:
         for line in open(path):
             if line.startswith("traffic_bytes_per_launch:"):
@@ -734,7 +734,7 @@ def main():
                      "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(achieved_tflops / FP64_PEAK_TFLOPS, 4) if achieved_tflops else None,
                      "traffic": profiled_traffic() if B == 4096 else None,
-                     "kernel": "br4_kernel (blind rotation, FP64); traffic from profiles/br4_kernel_p128_batch4096.md",
+                     "kernel": "br4w_kernel (blind rotation, FP64); traffic from profiles/br4w_kernel_p128_batch4096.md",
                      "note": "the blind rotation runs on the FP64 vector ALUs (no matrix-core instruction); "
                              "MI355X FP64 vector peak 78.6 TFLOP/s"},
         "hbm_views": hbm,
