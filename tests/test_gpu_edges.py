@@ -86,16 +86,21 @@ def test_ragged_batch_in_the_batched_kernel(key):
     assert np.array_equal(got[sel], O.gates(keys, ops[sel], a[sel], b[sel], c[sel], threads=8))
 
 
-def test_row_synchronised_rounds_match_one_bootstrap_workgroups(key):
-    """4133 gates with MUX gates: mode 2 runs the whole device rounds (4096 jobs on
-    256 CUs) on the eight-bootstrap workgroups with a barrier per key row and the
-    rest, from job 4096 on, on one-bootstrap workgroups; mode 9 runs every job on
-    the latter. Bit-identical, and gates from both parts equal the oracle."""
+@pytest.mark.parametrize("count,mux_all", [(8300, False), (4133, False), (2048, True)])
+def test_row_synchronised_rounds_match_one_bootstrap_workgroups(key, count, mux_all):
+    """Mode 2 runs whole device rounds (2048 jobs on 256 CUs) on eight-bootstrap
+    workgroups with a barrier per key row and the rest on one-bootstrap workgroups;
+    mode 9 runs every job on the latter. 8300 gates with MUX gates (about 9050 jobs:
+    three rounds on the former, the last whole round and the remainder on the latter),
+    4133 gates (about 4500 jobs: too few whole rounds to split, all on the latter) and
+    2048 MUX gates (4096 jobs, two whole rounds, second bootstraps included):
+    bit-identical, and gates from every part equal the oracle."""
     T, P, sk, keys = key
-    rng = np.random.default_rng(81)
-    count = 4133
-    ops = rng.integers(0, 11, size=count).astype(np.int32)
-    ops[[0, 2047, 4095, 4100]] = 10
+    rng = np.random.default_rng(81 + count)
+    ops = np.full(count, 10, np.int32) if mux_all else rng.integers(0, 11, size=count).astype(np.int32)
+    sel = {2048: [0, 1023, 2047], 4133: [0, 2047, 4095, 4100, 4132],
+           8300: [0, 2047, 6143, 6144, 8299]}[count]
+    ops[sel] = 10
     bits = rng.integers(0, 2, size=(3, count)).astype(np.int32)
     a, b, c = (sk.encrypt(x) for x in bits)
     got = {}
@@ -108,7 +113,6 @@ def test_row_synchronised_rounds_match_one_bootstrap_workgroups(key):
         T.set_launch_config(0)
     assert np.array_equal(got[2], got[9])
     assert np.array_equal(sk.decrypt(got[2]), _truth(ops, bits))
-    sel = [0, 2047, 4095, 4100, 4132]
     assert np.array_equal(got[2][sel], O.gates(keys, ops[sel], a[sel], b[sel], c[sel], threads=8))
 
 

@@ -105,7 +105,7 @@ struct BrArgs {
     int32_t n;
     int32_t Bgbit;
     uint32_t offset;  // gadget offset, libtfhe/tgsw.cpp:21-27
-    int32_t job0;     // batched kernel: workgroup x runs job job0 + x
+    int32_t split_round;  // batched kernel: > 0 when br4w_kernel ran the first br4w_split(n_jobs, split_round) jobs
 };
 
 // Orders LDS accesses between the lanes of one wavefront. The blind-rotation
@@ -368,20 +368,20 @@ struct Br4Lds {
 
 // CT: the fused-butterfly transform (FftCT, key bkc, tables twc; LDS fb and fb2
 // of Br4Lds::CT_C64 entries) instead of Fft<512> (LDS fb of Br4Lds::BUF_C64).
-// The calling wavefront runs job `job`. ROW_SYNC (fused-butterfly form only): a
+// The calling wavefront (lane `lane`) runs job `job`. ROW_SYNC (fused-butterfly form only): a
 // workgroup barrier after every key row, which holds the workgroup's wavefronts
 // (bootstraps) together so that they read each key row at about the same time.
 template <int L, bool CT = false, bool ROW_SYNC = false>
 __device__ __forceinline__ void br4_body(BrArgs A, const int32_t* __restrict__ mux_gate,
                                          const int32_t* __restrict__ n_jobs, c64* __restrict__ fb,
-                                         int16_t* __restrict__ bara, c64* __restrict__ fb2, const int job) {
+                                         int16_t* __restrict__ bara, c64* __restrict__ fb2, const int job,
+                                         const int lane) {
     constexpr int M = 512;
     using F = Fft<M>;
     using G = FftCT;
     constexpr int R = F::R, N = 2 * M, KPL = 2 * L;
     constexpr int LOG2N2 = 11;
 
-    const int lane = threadIdx.x & 63;
     if (job >= *n_jobs) return;
     const int n = A.n;
     int32_t* fbi = (int32_t*)fb;
@@ -653,20 +653,39 @@ __device__ __forceinline__ void br4_body(BrArgs A, const int32_t* __restrict__ m
     if (lane == 0) e[N] = (int32_t)acc[1][0];
 }
 
+// Jobs [0, split) of a batch of n jobs that br4w_kernel runs, for device rounds of
+// `round` jobs: all of them when n is whole rounds; otherwise every whole round but the
+// last, which runs with the remainder on one-bootstrap workgroups, when that leaves at
+// least BR4W_MIN_ROUNDS rounds to br4w_kernel, else none. The one-bootstrap workgroups
+// start the remainder's bootstraps as soon as bootstraps of the round before end, where a
+// kernel boundary holds them until the slowest CU is done: that costs about a tenth of a
+// round, each round on br4w_kernel saves a twenty-fifth (4,470 jobs: 25.9 ms split after
+// two rounds, 24.4 ms after one, 23.7 ms all on one-bootstrap workgroups).
+constexpr int BR4W_MIN_ROUNDS = 3;
+__device__ __forceinline__ int br4w_split(int n, int round) {
+    const int k = n / round;
+    if (n == k * round) return n;
+    return k - 1 >= BR4W_MIN_ROUNDS ? (k - 1) * round : 0;
+}
+
+// Workgroup x runs job x, or job split + x after br4w_kernel ran jobs [0, split)
+// (A.split_round > 0).
 template <int L, bool CT = false>
 __global__ __launch_bounds__(64, 8) void br4_kernel(BrArgs A, const int32_t* __restrict__ mux_gate,
                                                    const int32_t* __restrict__ n_jobs) {
     __shared__ int16_t bara[1024];
+    const int job = (A.split_round > 0 ? br4w_split(*n_jobs, A.split_round) : 0) + (int)blockIdx.x;
     if constexpr (CT) {
         __shared__ c64 fb[Br4Lds::CT_C64], fb2[Br4Lds::CT_C64];
-        br4_body<L, CT>(A, mux_gate, n_jobs, fb, bara, fb2, A.job0 + (int)blockIdx.x);
+        br4_body<L, CT>(A, mux_gate, n_jobs, fb, bara, fb2, job, (int)threadIdx.x);
     } else {
         __shared__ c64 fb[Br4Lds::BUF_C64];
-        br4_body<L, CT>(A, mux_gate, n_jobs, fb, bara, nullptr, A.job0 + (int)blockIdx.x);
+        br4_body<L, CT>(A, mux_gate, n_jobs, fb, bara, nullptr, job, (int)threadIdx.x);
     }
 }
 
-// The batched kernel's rounds that fill the device: BR4W_WAVES bootstraps per
+// The batched kernel's rounds that fill the device (jobs [0, br4w_split(n_jobs,
+// round)), `round` = resident workgroups x BR4W_WAVES): BR4W_WAVES bootstraps per
 // workgroup (one per wavefront, each with its own LDS, 160 KB together: one
 // workgroup per CU), workgroup x running jobs BR4W_WAVES x + w, with a workgroup
 // barrier after every key row. Held together, the eight bootstraps of a CU read
@@ -678,11 +697,14 @@ __global__ __launch_bounds__(64, 8) void br4_kernel(BrArgs A, const int32_t* __r
 constexpr int BR4W_WAVES = 8;
 template <int L>
 __global__ __launch_bounds__(64 * BR4W_WAVES, 1) void br4w_kernel(BrArgs A, const int32_t* __restrict__ mux_gate,
-                                                                 const int32_t* __restrict__ n_jobs) {
+                                                                 const int32_t* __restrict__ n_jobs, int round) {
     __shared__ int16_t bara[BR4W_WAVES][1024];
     __shared__ c64 fb[BR4W_WAVES][Br4Lds::CT_C64], fb2[BR4W_WAVES][Br4Lds::CT_C64];
     const int w = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
-    br4_body<L, true, true>(A, mux_gate, n_jobs, fb[w], bara[w], fb2[w], (int)blockIdx.x * BR4W_WAVES + w);
+    const int job = (int)blockIdx.x * BR4W_WAVES + w;
+    // uniform per workgroup (the split is a multiple of BR4W_WAVES)
+    if (job >= br4w_split(*n_jobs, round)) return;
+    br4_body<L, true, true>(A, mux_gate, n_jobs, fb[w], bara[w], fb2[w], job, (int)threadIdx.x & 63);
 }
 
 // One launch over several clients' batches (segments), each with its own
@@ -717,7 +739,7 @@ __global__ __launch_bounds__(64, 8) void br4_seg_kernel(BrSegTable T) {
     A.in2 += g.off;
     __shared__ c64 fb[Br4Lds::CT_C64], fb2[Br4Lds::CT_C64];
     __shared__ int16_t bara[1024];
-    br4_body<L, true>(A, g.mux_gate, g.n_jobs, fb, bara, fb2, (int)blockIdx.x);
+    br4_body<L, true>(A, g.mux_gate, g.n_jobs, fb, bara, fb2, (int)blockIdx.x, (int)threadIdx.x);
 }
 
 // largest LWE dimension the latency kernel's LDS table holds
@@ -2603,17 +2625,18 @@ int gates_device_locked(Ctx& C, tfhe_amd_device_key* dk, int count, const int32_
     } else if (mode == 2 || mode == 9) {
         A.bkf = dk->bkc;  // the fused-butterfly transform's key and tables
         A.twc = dk->twc;
-        // jobs [0, j8): whole rounds of the device on the row-synchronised eight-bootstrap
-        // workgroups; the rest (at most 2*count jobs in all) on one-bootstrap workgroups
-        // (mode 9: every job on these)
+        // jobs [0, br4w_split(n_jobs, round)) (on the device, which knows the job count):
+        // whole rounds of the device on the row-synchronised eight-bootstrap workgroups; the
+        // rest (at most 2*count jobs in all) on one-bootstrap workgroups (mode 9: every job
+        // on these). Workgroups past the split or the job count leave at once.
         const int round = mode == 2 ? br4w_round(C, dk->P.l) : 0;
-        const int j8 = round > 0 ? count / round * round : 0;
-        if (j8 > 0) {
-            if (dk->P.l == 3) hipLaunchKernelGGL(br4w_kernel<3>, dim3(j8 / BR4W_WAVES), dim3(64 * BR4W_WAVES), 0, s, A, mux_gate, n_jobs);
-            else hipLaunchKernelGGL(br4w_kernel<2>, dim3(j8 / BR4W_WAVES), dim3(64 * BR4W_WAVES), 0, s, A, mux_gate, n_jobs);
+        const int j8max = round > 0 ? 2 * count / round * round : 0;
+        if (j8max > 0) {
+            if (dk->P.l == 3) hipLaunchKernelGGL(br4w_kernel<3>, dim3(j8max / BR4W_WAVES), dim3(64 * BR4W_WAVES), 0, s, A, mux_gate, n_jobs, round);
+            else hipLaunchKernelGGL(br4w_kernel<2>, dim3(j8max / BR4W_WAVES), dim3(64 * BR4W_WAVES), 0, s, A, mux_gate, n_jobs, round);
+            A.split_round = round;
         }
-        A.job0 = j8;
-        const int blocks = 2 * count - j8;
+        const int blocks = 2 * count;
         if (dk->P.l == 3) hipLaunchKernelGGL((br4_kernel<3, true>), dim3(blocks), dim3(64), 0, s, A, mux_gate, n_jobs);
         else hipLaunchKernelGGL((br4_kernel<2, true>), dim3(blocks), dim3(64), 0, s, A, mux_gate, n_jobs);
     } else if (mode == 7) {
