@@ -86,34 +86,40 @@ def test_ragged_batch_in_the_batched_kernel(key):
     assert np.array_equal(got[sel], O.gates(keys, ops[sel], a[sel], b[sel], c[sel], threads=8))
 
 
-@pytest.mark.parametrize("count,mux_all", [(8300, False), (4133, False), (2048, True)])
-def test_row_synchronised_rounds_match_one_bootstrap_workgroups(key, count, mux_all):
+@pytest.mark.parametrize("mode,count,kind", [(2, 8300, "mixed"), (2, 4133, "mixed"), (2, 2048, "mux"),
+                                             (3, 1024, "two_input"), (3, 512, "mux"), (3, 1100, "mixed")])
+def test_whole_round_workgroups_match_one_bootstrap_workgroups(key, mode, count, kind):
     """Mode 2 runs whole device rounds (2048 jobs on 256 CUs) on eight-bootstrap
-    workgroups with a barrier per key row and the rest on one-bootstrap workgroups;
-    mode 9 runs every job on the latter. 8300 gates with MUX gates (about 9050 jobs:
-    three rounds on the former, the last whole round and the remainder on the latter),
-    4133 gates (about 4500 jobs: too few whole rounds to split, all on the latter) and
-    2048 MUX gates (4096 jobs, two whole rounds, second bootstraps included):
-    bit-identical, and gates from every part equal the oracle."""
+    workgroups with a barrier per key row, mode 3 (1024 jobs) on workgroups of four
+    two-wavefront bootstraps, and the rest on one-bootstrap workgroups; mode 9 runs
+    every job on the batched kernel's one-bootstrap workgroups. Mode 2: 8300 gates with
+    MUX gates (about 9050 jobs: three rounds split off), 4133 (about 4500 jobs: too few
+    whole rounds to split) and 2048 MUX gates (4096 jobs, two whole rounds, second
+    bootstraps included); mode 3: 1024 two-input gates and 512 MUX gates (one whole
+    round each) and 1100 gates with MUX gates (not whole rounds). Bit-identical, and
+    gates from every part equal the oracle."""
     T, P, sk, keys = key
     rng = np.random.default_rng(81 + count)
-    ops = np.full(count, 10, np.int32) if mux_all else rng.integers(0, 11, size=count).astype(np.int32)
-    sel = {2048: [0, 1023, 2047], 4133: [0, 2047, 4095, 4100, 4132],
-           8300: [0, 2047, 6143, 6144, 8299]}[count]
-    ops[sel] = 10
+    if kind == "mux":
+        ops = np.full(count, 10, np.int32)
+    else:
+        ops = rng.integers(0, 11 if kind == "mixed" else 10, size=count).astype(np.int32)
+    sel = sorted({0, count // 2, count - 1} | ({2047, 6143, 6144} if count == 8300 else set()))
+    if kind == "mixed":
+        ops[sel] = 10
     bits = rng.integers(0, 2, size=(3, count)).astype(np.int32)
     a, b, c = (sk.encrypt(x) for x in bits)
     got = {}
     try:
-        for mode in (2, 9):
-            T.set_launch_config(mode)
-            assert T.kernel_for(sk.cloud, count) == mode
-            got[mode] = T.run_gates(sk.cloud, ops, a, b, c)
+        for m in (mode, 9):
+            T.set_launch_config(m)
+            assert T.kernel_for(sk.cloud, count) == m
+            got[m] = T.run_gates(sk.cloud, ops, a, b, c)
     finally:
         T.set_launch_config(0)
-    assert np.array_equal(got[2], got[9])
-    assert np.array_equal(sk.decrypt(got[2]), _truth(ops, bits))
-    assert np.array_equal(got[2][sel], O.gates(keys, ops[sel], a[sel], b[sel], c[sel], threads=8))
+    assert np.array_equal(got[mode], got[9])
+    assert np.array_equal(sk.decrypt(got[mode]), _truth(ops, bits))
+    assert np.array_equal(got[mode][sel], O.gates(keys, ops[sel], a[sel], b[sel], c[sel], threads=8))
 
 
 def test_very_large_launch(key):

@@ -105,7 +105,7 @@ struct BrArgs {
     int32_t n;
     int32_t Bgbit;
     uint32_t offset;  // gadget offset, libtfhe/tgsw.cpp:21-27
-    int32_t split_round;  // batched kernel: > 0 when br4w_kernel ran the first br4w_split(n_jobs, split_round) jobs
+    int32_t split_round;  // batched kernel: > 0 when br4w_kernel ran the first round_split(n_jobs, split_round) jobs
 };
 
 // Orders LDS accesses between the lanes of one wavefront. The blind-rotation
@@ -653,16 +653,17 @@ __device__ __forceinline__ void br4_body(BrArgs A, const int32_t* __restrict__ m
     if (lane == 0) e[N] = (int32_t)acc[1][0];
 }
 
-// Jobs [0, split) of a batch of n jobs that br4w_kernel runs, for device rounds of
-// `round` jobs: all of them when n is whole rounds; otherwise every whole round but the
-// last, which runs with the remainder on one-bootstrap workgroups, when that leaves at
-// least BR4W_MIN_ROUNDS rounds to br4w_kernel, else none. The one-bootstrap workgroups
+// Jobs [0, split) of a batch of n jobs that the several-bootstrap workgroups of
+// br4w_kernel or br2_kernel<L, 4> run, for device rounds of `round` jobs: all of them when
+// n is whole rounds; otherwise every whole round but the last, which runs with the
+// remainder on one-bootstrap workgroups, when that leaves at least BR4W_MIN_ROUNDS rounds
+// to the former, else none. The one-bootstrap workgroups
 // start the remainder's bootstraps as soon as bootstraps of the round before end, where a
 // kernel boundary holds them until the slowest CU is done: that costs about a tenth of a
 // round, each round on br4w_kernel saves a twenty-fifth (4,470 jobs: 25.9 ms split after
 // two rounds, 24.4 ms after one, 23.7 ms all on one-bootstrap workgroups).
 constexpr int BR4W_MIN_ROUNDS = 3;
-__device__ __forceinline__ int br4w_split(int n, int round) {
+__device__ __forceinline__ int round_split(int n, int round) {
     const int k = n / round;
     if (n == k * round) return n;
     return k - 1 >= BR4W_MIN_ROUNDS ? (k - 1) * round : 0;
@@ -674,7 +675,7 @@ template <int L, bool CT = false>
 __global__ __launch_bounds__(64, 8) void br4_kernel(BrArgs A, const int32_t* __restrict__ mux_gate,
                                                    const int32_t* __restrict__ n_jobs) {
     __shared__ int16_t bara[1024];
-    const int job = (A.split_round > 0 ? br4w_split(*n_jobs, A.split_round) : 0) + (int)blockIdx.x;
+    const int job = (A.split_round > 0 ? round_split(*n_jobs, A.split_round) : 0) + (int)blockIdx.x;
     if constexpr (CT) {
         __shared__ c64 fb[Br4Lds::CT_C64], fb2[Br4Lds::CT_C64];
         br4_body<L, CT>(A, mux_gate, n_jobs, fb, bara, fb2, job, (int)threadIdx.x);
@@ -684,7 +685,7 @@ __global__ __launch_bounds__(64, 8) void br4_kernel(BrArgs A, const int32_t* __r
     }
 }
 
-// The batched kernel's rounds that fill the device (jobs [0, br4w_split(n_jobs,
+// The batched kernel's rounds that fill the device (jobs [0, round_split(n_jobs,
 // round)), `round` = resident workgroups x BR4W_WAVES): BR4W_WAVES bootstraps per
 // workgroup (one per wavefront, each with its own LDS, 160 KB together: one
 // workgroup per CU), workgroup x running jobs BR4W_WAVES x + w, with a workgroup
@@ -703,7 +704,7 @@ __global__ __launch_bounds__(64 * BR4W_WAVES, 1) void br4w_kernel(BrArgs A, cons
     const int w = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
     const int job = (int)blockIdx.x * BR4W_WAVES + w;
     // uniform per workgroup (the split is a multiple of BR4W_WAVES)
-    if (job >= br4w_split(*n_jobs, round)) return;
+    if (job >= round_split(*n_jobs, round)) return;
     br4_body<L, true, true>(A, mux_gate, n_jobs, fb[w], bara[w], fb2[w], job, (int)threadIdx.x & 63);
 }
 
@@ -1005,31 +1006,51 @@ __global__ __launch_bounds__(128) void bk_convert_half_kernel(const int32_t* __r
 constexpr int BR2_NMAX = 1024;
 
 // Blind rotation with two wavefronts per bootstrap on the fused-butterfly
-// transform (M = 512): one bootstrap per 128-thread workgroup, wavefront c owns
-// TLWE component c of the accumulator (in registers). Per key row it rotates
-// and decomposes its own component (through its 2N-word buffer, as br4_kernel),
-// transforms its l digit polynomials and multiply-accumulates them against key
-// rows (c l + p) of both output components, read straight from global memory;
-// then it hands the partial sum of the other component to its partner through
-// LDS, takes the partner's partial sum of its own, inverse-transforms and
-// updates its component. Two workgroup barriers per row (between the two
-// wavefronts of one bootstrap only); four bootstraps per CU.
-template <int L>
-__global__ __launch_bounds__(128, 2) void br2_kernel(BrArgs A, const int32_t* __restrict__ mux_gate,
-                                                    const int32_t* __restrict__ n_jobs) {
+// transform (M = 512): wavefront c owns TLWE component c of the accumulator (in
+// registers). Per key row it rotates and decomposes its own component (through
+// its 2N-word buffer, as br4_kernel), transforms its l digit polynomials and
+// multiply-accumulates them against key rows (c l + p) of both output
+// components, read straight from global memory; then it hands the partial sum
+// of the other component to its partner through LDS, takes the partner's partial
+// sum of its own, inverse-transforms and updates its component. Two workgroup
+// barriers per row; four bootstraps per CU. B = 1: one bootstrap per 128-thread
+// workgroup, workgroup x running job x (or job split + x after the B = 4 kernel
+// ran jobs [0, split), A.split_round > 0). B = 4: whole device rounds (jobs
+// [0, round_split(n_jobs, round))), four bootstraps per 512-thread workgroup,
+// pair b of workgroup x running job 4 x + b with its own LDS (one workgroup per
+// CU): the barriers then hold the CU's four bootstraps together, so their key
+// reads share the CU's L1 (one whole round of 1,024 two-input gates: 176.6k ->
+// 188.7k gates/s; two per workgroup 177.5k).
+template <int L, int B = 1>
+__global__ __launch_bounds__(128 * B, 2 / (B > 1 ? 2 : 1)) void br2_kernel(BrArgs A, const int32_t* __restrict__ mux_gate,
+                                                                          const int32_t* __restrict__ n_jobs, int round) {
     constexpr int M = 512;
     using G = FftCT;
     constexpr int R = 8, N = 2 * M, KPL = 2 * L, LOG2N2 = 11;
-    __shared__ c64 s_fft[2][G::LDS_C64];  // transform and rotation buffer of wavefront c
-    __shared__ c64 s_swap[2][M];          // s_swap[d]: partial sum of output d from wavefront 1 - d
-    __shared__ int16_t s_bara[BR2_NMAX];
-    __shared__ int32_t s_barb;
+    __shared__ c64 s_fft_b[B][2][G::LDS_C64];  // transform and rotation buffer of wavefront c
+    __shared__ c64 s_swap_b[B][2][M];          // s_swap[d]: partial sum of output d from wavefront 1 - d
+    __shared__ int16_t s_bara_b[B][BR2_NMAX];
+    __shared__ int32_t s_barb_b[B];
+    const int pb = B == 1 ? 0 : __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 7);  // this bootstrap's pair
+    auto& s_fft = s_fft_b[pb];
+    auto& s_swap = s_swap_b[pb];
+    auto& s_bara = s_bara_b[pb];
+    int32_t& s_barb = s_barb_b[pb];
 
-    const int tid = threadIdx.x;
+    const int tid = B == 1 ? (int)threadIdx.x : (int)threadIdx.x & 127;
     const int lane = tid & 63;
     const int c = __builtin_amdgcn_readfirstlane(tid >> 6);
-    const int job = blockIdx.x;
-    if (job >= *n_jobs) return;  // whole block idle (uniform)
+    int job;
+    if constexpr (B == 1) {
+        job = (A.split_round > 0 ? round_split(*n_jobs, A.split_round) : 0) + (int)blockIdx.x;
+    } else {
+        job = (int)blockIdx.x * B + pb;
+        // uniform per workgroup (the split is a multiple of B)
+        if (job >= round_split(*n_jobs, round)) return;
+    }
+    // uniform per pair; its wavefronts end, and the workgroup's barriers then wait for the
+    // other pairs only
+    if (job >= *n_jobs) return;
     const int n = A.n;
     c64* fb = s_fft[c];
     int32_t* fbi = (int32_t*)fb;
@@ -2259,7 +2280,8 @@ struct Ctx {
     int br_small_max = 1024;  // automatic mode: batches up to this many gates use two wavefronts per bootstrap
     bool crossovers_set = false;  // set_launch_config gave crossovers: they apply instead of the per-size choice
     char* noalloc_scr = nullptr;  // this device's copy of `scratch` (hipGetSymbolAddress at set-up)
-    int br4w_jobs[2] = {-1, -1};  // bootstraps in one whole round of br4w_kernel<l> (l = 2, 3), once queried
+    int round_jobs[2][2] = {{-1, -1}, {-1, -1}};  // bootstraps in one whole round of br4w_kernel<l> [0][l == 3]
+                                                   // and br2_kernel<l, 4> [1][l == 3], once queried
     // optional per-launch timing with HIP events on the launch stream
     bool prof = false;
     struct Ev { hipEvent_t e[3]; };
@@ -2526,17 +2548,23 @@ bool pair_kernel_chosen(const tfhe_amd_device_key* dk, int count) {
 // The blind-rotation kernel (mode) a batch of `count` gates runs with: the forced mode, or
 // in automatic mode the autotuner's per-size choice or the crossovers (shared: other launches
 // are on the device, so not the latency kernel), mapped to a kernel the key supports.
-// Bootstraps that one whole round of br4w_kernel<l> holds on the device
-// (resident workgroups x BR4W_WAVES; 0 if the query fails).
-int br4w_round(Ctx& C, int l) {
-    int& r = C.br4w_jobs[l == 3];
+// Bootstraps that one whole round of the several-bootstrap workgroups holds on the
+// device (resident workgroups x bootstraps per workgroup; 0 if the query fails):
+// which = 0 br4w_kernel<l>, 1 br2_kernel<l, 4>.
+int whole_round(Ctx& C, int which, int l) {
+    int& r = C.round_jobs[which][l == 3];
     if (r < 0) {
         int per_cu = 0, cus = 0;
-        const hipError_t e1 = l == 3
-            ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, br4w_kernel<3>, 64 * BR4W_WAVES, 0)
-            : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, br4w_kernel<2>, 64 * BR4W_WAVES, 0);
+        hipError_t e1;
+        if (which == 0)
+            e1 = l == 3 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, br4w_kernel<3>, 64 * BR4W_WAVES, 0)
+                        : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, br4w_kernel<2>, 64 * BR4W_WAVES, 0);
+        else
+            e1 = l == 3 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, br2_kernel<3, 4>, 512, 0)
+                        : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, br2_kernel<2, 4>, 512, 0);
         const hipError_t e2 = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, C.device);
-        r = (e1 == hipSuccess && e2 == hipSuccess && per_cu > 0 && cus > 0) ? per_cu * cus * BR4W_WAVES : 0;
+        r = (e1 == hipSuccess && e2 == hipSuccess && per_cu > 0 && cus > 0) ? per_cu * cus * (which == 0 ? BR4W_WAVES : 4)
+                                                                          : 0;
     }
     return r;
 }
@@ -2625,11 +2653,11 @@ int gates_device_locked(Ctx& C, tfhe_amd_device_key* dk, int count, const int32_
     } else if (mode == 2 || mode == 9) {
         A.bkf = dk->bkc;  // the fused-butterfly transform's key and tables
         A.twc = dk->twc;
-        // jobs [0, br4w_split(n_jobs, round)) (on the device, which knows the job count):
+        // jobs [0, round_split(n_jobs, round)) (on the device, which knows the job count):
         // whole rounds of the device on the row-synchronised eight-bootstrap workgroups; the
         // rest (at most 2*count jobs in all) on one-bootstrap workgroups (mode 9: every job
         // on these). Workgroups past the split or the job count leave at once.
-        const int round = mode == 2 ? br4w_round(C, dk->P.l) : 0;
+        const int round = mode == 2 ? whole_round(C, 0, dk->P.l) : 0;
         const int j8max = round > 0 ? 2 * count / round * round : 0;
         if (j8max > 0) {
             if (dk->P.l == 3) hipLaunchKernelGGL(br4w_kernel<3>, dim3(j8max / BR4W_WAVES), dim3(64 * BR4W_WAVES), 0, s, A, mux_gate, n_jobs, round);
@@ -2644,11 +2672,20 @@ int gates_device_locked(Ctx& C, tfhe_amd_device_key* dk, int count, const int32_
         if (dk->P.l == 3) hipLaunchKernelGGL(br4_kernel<3>, dim3(blocks), dim3(64), 0, s, A, mux_gate, n_jobs);
         else hipLaunchKernelGGL(br4_kernel<2>, dim3(blocks), dim3(64), 0, s, A, mux_gate, n_jobs);
     } else if (mode == 3) {
-        const int blocks = 2 * count;  // one bootstrap per two-wavefront workgroup, at most 2*count jobs
-        A.bkf = dk->bkc;               // the fused-butterfly transform's key and tables
+        A.bkf = dk->bkc;  // the fused-butterfly transform's key and tables
         A.twc = dk->twc;
-        if (dk->P.l == 3) hipLaunchKernelGGL(br2_kernel<3>, dim3(blocks), dim3(128), 0, s, A, mux_gate, n_jobs);
-        else hipLaunchKernelGGL(br2_kernel<2>, dim3(blocks), dim3(128), 0, s, A, mux_gate, n_jobs);
+        // whole device rounds on four-bootstrap workgroups, the rest on one-bootstrap ones
+        // (split on the device as for mode 2; at most 2*count jobs in all)
+        const int round = whole_round(C, 1, dk->P.l);
+        const int jmax = round > 0 ? 2 * count / round * round : 0;
+        if (jmax > 0) {
+            if (dk->P.l == 3) hipLaunchKernelGGL((br2_kernel<3, 4>), dim3(jmax / 4), dim3(512), 0, s, A, mux_gate, n_jobs, round);
+            else hipLaunchKernelGGL((br2_kernel<2, 4>), dim3(jmax / 4), dim3(512), 0, s, A, mux_gate, n_jobs, round);
+            A.split_round = round;
+        }
+        const int blocks = 2 * count;
+        if (dk->P.l == 3) hipLaunchKernelGGL((br2_kernel<3, 1>), dim3(blocks), dim3(128), 0, s, A, mux_gate, n_jobs, 0);
+        else hipLaunchKernelGGL((br2_kernel<2, 1>), dim3(blocks), dim3(128), 0, s, A, mux_gate, n_jobs, 0);
     } else {
         launch_br(dk, A, 2 * count, s);
     }
