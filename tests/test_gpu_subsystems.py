@@ -155,6 +155,31 @@ def test_80bit_preset_every_kernel(p80, mode):
         assert sk.decrypt(got[g:g + 1])[0] == O.TRUTH[O.GATES[op]](bits[0][g], bits[1][g], bits[2][g])
 
 
+@pytest.mark.parametrize("mode,count", [(2, 2048), (3, 1024)])
+def test_80bit_preset_whole_round_workgroups(p80, mode, count):
+    """One whole device round of two-input gates under the 80-bit preset (l = 2): mode 2
+    runs it on eight-bootstrap workgroups, mode 3 on four-bootstrap ones, mode 9 on
+    one-bootstrap workgroups; bit-identical, decrypting correctly, samples equal the
+    oracle."""
+    T, P, sk, keys = p80
+    rng = np.random.default_rng(90 + mode)
+    ops = rng.integers(0, 10, size=count).astype(np.int32)
+    bits = rng.integers(0, 2, size=(2, count)).astype(np.int32)
+    a, b = (sk.encrypt(x) for x in bits)
+    got = {}
+    try:
+        for m in (mode, 9):
+            T.set_launch_config(m)
+            got[m] = T.run_gates(sk.cloud, ops, a, b)
+    finally:
+        T.set_launch_config(0)
+    assert np.array_equal(got[mode], got[9])
+    exp = np.array([O.TRUTH[O.GATES[o]](x, y, 0) for o, x, y in zip(ops, bits[0], bits[1])])
+    assert np.array_equal(sk.decrypt(got[mode]), exp)
+    sel = [0, count // 2, count - 1]
+    assert np.array_equal(got[mode][sel], O.gates(keys, ops[sel], a[sel], b[sel], threads=8))
+
+
 @pytest.mark.parametrize("path", [0, 1])
 def test_80bit_preset_keyswitch(p80, path):
     T, P, sk, keys = p80
