@@ -693,7 +693,7 @@ __global__ __launch_bounds__(64, 8) void br4_kernel(BrArgs A, const int32_t* __r
 // each key row at about the same time, so most of their key reads hit the CU's
 // L1 (a quarter of the one-wavefront workgroups' L2 requests, half their fabric
 // reads) and the device holds a higher clock under the load (2340 against 2010
-// MHz): 2.5% more gates/s over whole rounds. Partial rounds stay on br4_kernel,
+// MHz): 2.5-4% more gates/s over whole rounds. Partial rounds stay on br4_kernel,
 // which spreads them over every CU.
 constexpr int BR4W_WAVES = 8;
 template <int L>
