@@ -87,15 +87,18 @@ def test_ragged_batch_in_the_batched_kernel(key):
 
 
 @pytest.mark.parametrize("mode,count,kind", [(2, 8300, "mixed"), (2, 4133, "mixed"), (2, 2048, "mux"),
+                                             (2, 1280, "two_input"), (2, 1536, "two_input"),
+                                             (2, 1792, "two_input"), (2, 3072, "two_input"),
                                              (3, 1024, "two_input"), (3, 512, "mux"), (3, 1100, "mixed")])
 def test_whole_round_workgroups_match_one_bootstrap_workgroups(key, mode, count, kind):
-    """Mode 2 runs whole device rounds (2048 jobs on 256 CUs) on eight-bootstrap
-    workgroups with a barrier per key row, mode 3 (1024 jobs) on workgroups of four
-    two-wavefront bootstraps, and the rest on one-bootstrap workgroups; mode 9 runs
-    every job on the batched kernel's one-bootstrap workgroups. Mode 2: 8300 gates with
-    MUX gates (about 9050 jobs: three rounds split off), 4133 (about 4500 jobs: too few
-    whole rounds to split) and 2048 MUX gates (4096 jobs, two whole rounds, second
-    bootstraps included); mode 3: 1024 two-input gates and 512 MUX gates (one whole
+    """Mode 2 runs whole device rounds on workgroups of 5 to 8 bootstraps (the batch's
+    bootstraps per CU when that is 5 to 8, else 8) with a barrier per key row, mode 3 (1024 jobs) on workgroups of four two-wavefront bootstraps, and the
+    rest on one-bootstrap workgroups; mode 9 runs every job on the batched kernel's
+    one-bootstrap workgroups. Mode 2 on 256 CUs: 8300 gates with MUX gates (about 9050
+    jobs: three rounds of eight split off), 4133 (about 4500 jobs: too few whole rounds
+    to split), 2048 MUX gates (4096 jobs, two whole rounds of eight, second bootstraps
+    included), 1280 / 1536 / 1792 two-input gates (one round of five / six / seven) and
+    3072 (one and a half rounds of eight: all on one-bootstrap workgroups); mode 3: 1024 two-input gates and 512 MUX gates (one whole
     round each) and 1100 gates with MUX gates (not whole rounds). Bit-identical, and
     gates from every part equal the oracle."""
     T, P, sk, keys = key

@@ -155,11 +155,11 @@ def test_80bit_preset_every_kernel(p80, mode):
         assert sk.decrypt(got[g:g + 1])[0] == O.TRUTH[O.GATES[op]](bits[0][g], bits[1][g], bits[2][g])
 
 
-@pytest.mark.parametrize("mode,count", [(2, 2048), (3, 1024)])
+@pytest.mark.parametrize("mode,count", [(2, 2048), (2, 1536), (3, 1024)])
 def test_80bit_preset_whole_round_workgroups(p80, mode, count):
     """One whole device round of two-input gates under the 80-bit preset (l = 2): mode 2
-    runs it on eight-bootstrap workgroups, mode 3 on four-bootstrap ones, mode 9 on
-    one-bootstrap workgroups; bit-identical, decrypting correctly, samples equal the
+    runs it on eight- or six-bootstrap workgroups, mode 3 on four-bootstrap ones, mode 9
+    on one-bootstrap workgroups; bit-identical, decrypting correctly, samples equal the
     oracle."""
     T, P, sk, keys = p80
     rng = np.random.default_rng(90 + mode)

@@ -686,26 +686,59 @@ __global__ __launch_bounds__(64, 8) void br4_kernel(BrArgs A, const int32_t* __r
 }
 
 // The batched kernel's rounds that fill the device (jobs [0, round_split(n_jobs,
-// round)), `round` = resident workgroups x BR4W_WAVES): BR4W_WAVES bootstraps per
-// workgroup (one per wavefront, each with its own LDS, 160 KB together: one
-// workgroup per CU), workgroup x running jobs BR4W_WAVES x + w, with a workgroup
-// barrier after every key row. Held together, the eight bootstraps of a CU read
-// each key row at about the same time, so most of their key reads hit the CU's
-// L1 (a quarter of the one-wavefront workgroups' L2 requests, half their fabric
-// reads) and the device holds a higher clock under the load (2340 against 2010
-// MHz): 2.5-4% more gates/s over whole rounds. Partial rounds stay on br4_kernel,
-// which spreads them over every CU.
-constexpr int BR4W_WAVES = 8;
-template <int L>
-__global__ __launch_bounds__(64 * BR4W_WAVES, 1) void br4w_kernel(BrArgs A, const int32_t* __restrict__ mux_gate,
-                                                                 const int32_t* __restrict__ n_jobs, int round) {
-    __shared__ int16_t bara[BR4W_WAVES][1024];
-    __shared__ c64 fb[BR4W_WAVES][Br4Lds::CT_C64], fb2[BR4W_WAVES][Br4Lds::CT_C64];
+// round)), `round` = resident workgroups x W): W bootstraps per workgroup (one per
+// wavefront, each with its own 20 KB of LDS; W >= 5 leaves room for one workgroup
+// per CU), workgroup x running jobs W x + w, with a workgroup barrier after every
+// key row. Held together, the bootstraps of a CU read each key row at about the
+// same time, so most of their key reads hit the CU's L1 (with eight: a quarter of
+// the one-wavefront workgroups' L2 requests, half their fabric reads) and the device
+// holds a higher clock under the load (2340 against 2010 MHz): 2.5-4% more gates/s
+// over whole rounds of eight, and a batch of 1536 gates, one round of six per CU,
+// 21% more than on one-bootstrap workgroups (which leave it three quarters of a
+// round). Partial rounds stay on br4_kernel, which spreads them over every CU.
+constexpr int BR4W_MIN = 5, BR4W_MAX = 8;
+template <int L, int W>
+__global__ __launch_bounds__(64 * W, 1) void br4w_kernel(BrArgs A, const int32_t* __restrict__ mux_gate,
+                                                        const int32_t* __restrict__ n_jobs, int round) {
+    static_assert(W >= BR4W_MIN && W <= BR4W_MAX, "bootstraps per workgroup");
+    __shared__ int16_t bara[W][1024];
+    __shared__ c64 fb[W][Br4Lds::CT_C64], fb2[W][Br4Lds::CT_C64];
     const int w = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
-    const int job = (int)blockIdx.x * BR4W_WAVES + w;
-    // uniform per workgroup (the split is a multiple of BR4W_WAVES)
+    const int job = (int)blockIdx.x * W + w;
+    // uniform per workgroup (the split is a multiple of W)
     if (job >= round_split(*n_jobs, round)) return;
     br4_body<L, true, true>(A, mux_gate, n_jobs, fb[w], bara[w], fb2[w], job, (int)threadIdx.x & 63);
+}
+
+// The bootstraps per workgroup of br4w_kernel for a batch of `count` gates on `cus` CUs:
+// the bootstraps per CU the batch needs when that is 5 to 8 (one round, whole when the
+// batch has no MUX gates), else BR4W_MAX. Several rounds of fewer than eight lose to the
+// one-bootstrap workgroups (3072 gates as two rounds of six: 192.0k against 199.1k
+// gates/s; one round of 5 / 6 / 7 per CU: +10% / +24% / +11%).
+int br4w_waves(int count, int cus) {
+    const int q = (count + cus - 1) / cus;
+    return q >= BR4W_MIN && q <= BR4W_MAX ? q : BR4W_MAX;
+}
+
+template <int L>
+void launch_br4w(int w, int blocks, hipStream_t s, const BrArgs& A, const int32_t* mux_gate, const int32_t* n_jobs,
+                 int round) {
+    switch (w) {
+    case 5: hipLaunchKernelGGL((br4w_kernel<L, 5>), dim3(blocks), dim3(64 * 5), 0, s, A, mux_gate, n_jobs, round); break;
+    case 6: hipLaunchKernelGGL((br4w_kernel<L, 6>), dim3(blocks), dim3(64 * 6), 0, s, A, mux_gate, n_jobs, round); break;
+    case 7: hipLaunchKernelGGL((br4w_kernel<L, 7>), dim3(blocks), dim3(64 * 7), 0, s, A, mux_gate, n_jobs, round); break;
+    default: hipLaunchKernelGGL((br4w_kernel<L, 8>), dim3(blocks), dim3(64 * 8), 0, s, A, mux_gate, n_jobs, round); break;
+    }
+}
+
+template <int L>
+hipError_t br4w_occupancy(int w, int* per_cu) {
+    switch (w) {
+    case 5: return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, br4w_kernel<L, 5>, 64 * 5, 0);
+    case 6: return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, br4w_kernel<L, 6>, 64 * 6, 0);
+    case 7: return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, br4w_kernel<L, 7>, 64 * 7, 0);
+    default: return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, br4w_kernel<L, 8>, 64 * 8, 0);
+    }
 }
 
 // One launch over several clients' batches (segments), each with its own
@@ -2280,8 +2313,10 @@ struct Ctx {
     int br_small_max = 1024;  // automatic mode: batches up to this many gates use two wavefronts per bootstrap
     bool crossovers_set = false;  // set_launch_config gave crossovers: they apply instead of the per-size choice
     char* noalloc_scr = nullptr;  // this device's copy of `scratch` (hipGetSymbolAddress at set-up)
-    int round_jobs[2][2] = {{-1, -1}, {-1, -1}};  // bootstraps in one whole round of br4w_kernel<l> [0][l == 3]
-                                                   // and br2_kernel<l, 4> [1][l == 3], once queried
+    int round_jobs[5][2] = {{-1, -1}, {-1, -1}, {-1, -1}, {-1, -1}, {-1, -1}};  // bootstraps in one whole
+                            // round of br2_kernel<l, 4> [0][l == 3] and br4w_kernel<l, W> [W - 4][l == 3],
+                            // once queried
+    int cus = -1;           // CUs of the device, once queried
     // optional per-launch timing with HIP events on the launch stream
     bool prof = false;
     struct Ev { hipEvent_t e[3]; };
@@ -2545,29 +2580,39 @@ bool pair_kernel_chosen(const tfhe_amd_device_key* dk, int count) {
     return false;
 }
 
-// The blind-rotation kernel (mode) a batch of `count` gates runs with: the forced mode, or
-// in automatic mode the autotuner's per-size choice or the crossovers (shared: other launches
-// are on the device, so not the latency kernel), mapped to a kernel the key supports.
+// CUs of the device (0 if the query fails).
+int device_cus(Ctx& C) {
+    if (C.cus < 0) {
+        int cus = 0;
+        C.cus = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, C.device) == hipSuccess && cus > 0
+                    ? cus : 0;
+    }
+    return C.cus;
+}
+
 // Bootstraps that one whole round of the several-bootstrap workgroups holds on the
-// device (resident workgroups x bootstraps per workgroup; 0 if the query fails):
-// which = 0 br4w_kernel<l>, 1 br2_kernel<l, 4>.
-int whole_round(Ctx& C, int which, int l) {
-    int& r = C.round_jobs[which][l == 3];
+// device (resident workgroups x bootstraps per workgroup; 0 if a query fails):
+// w = 4 br2_kernel<l, 4> (four two-wavefront bootstraps per workgroup), w = 5..8
+// br4w_kernel<l, w>.
+int whole_round(Ctx& C, int w, int l) {
+    int& r = C.round_jobs[w - 4][l == 3];
     if (r < 0) {
-        int per_cu = 0, cus = 0;
+        int per_cu = 0;
         hipError_t e1;
-        if (which == 0)
-            e1 = l == 3 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, br4w_kernel<3>, 64 * BR4W_WAVES, 0)
-                        : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, br4w_kernel<2>, 64 * BR4W_WAVES, 0);
-        else
+        if (w == 4)
             e1 = l == 3 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, br2_kernel<3, 4>, 512, 0)
                         : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, br2_kernel<2, 4>, 512, 0);
-        const hipError_t e2 = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, C.device);
-        r = (e1 == hipSuccess && e2 == hipSuccess && per_cu > 0 && cus > 0) ? per_cu * cus * (which == 0 ? BR4W_WAVES : 4)
-                                                                          : 0;
+        else
+            e1 = l == 3 ? br4w_occupancy<3>(w, &per_cu) : br4w_occupancy<2>(w, &per_cu);
+        const int cus = device_cus(C);
+        r = (e1 == hipSuccess && per_cu > 0 && cus > 0) ? per_cu * cus * w : 0;
     }
     return r;
 }
+
+// The blind-rotation kernel (mode) a batch of `count` gates runs with: the forced mode, or
+// in automatic mode the autotuner's per-size choice or the crossovers (shared: other launches
+// are on the device, so not the latency kernel), mapped to a kernel the key supports.
 
 int resolve_mode(const Ctx& C, const tfhe_amd_device_key* dk, int count, bool shared) {
     int mode = C.br_mode;
@@ -2654,14 +2699,15 @@ int gates_device_locked(Ctx& C, tfhe_amd_device_key* dk, int count, const int32_
         A.bkf = dk->bkc;  // the fused-butterfly transform's key and tables
         A.twc = dk->twc;
         // jobs [0, round_split(n_jobs, round)) (on the device, which knows the job count):
-        // whole rounds of the device on the row-synchronised eight-bootstrap workgroups; the
+        // whole rounds of the device on the row-synchronised workgroups of w bootstraps; the
         // rest (at most 2*count jobs in all) on one-bootstrap workgroups (mode 9: every job
         // on these). Workgroups past the split or the job count leave at once.
-        const int round = mode == 2 ? whole_round(C, 0, dk->P.l) : 0;
-        const int j8max = round > 0 ? 2 * count / round * round : 0;
-        if (j8max > 0) {
-            if (dk->P.l == 3) hipLaunchKernelGGL(br4w_kernel<3>, dim3(j8max / BR4W_WAVES), dim3(64 * BR4W_WAVES), 0, s, A, mux_gate, n_jobs, round);
-            else hipLaunchKernelGGL(br4w_kernel<2>, dim3(j8max / BR4W_WAVES), dim3(64 * BR4W_WAVES), 0, s, A, mux_gate, n_jobs, round);
+        const int w = device_cus(C) > 0 ? br4w_waves(count, device_cus(C)) : BR4W_MAX;
+        const int round = mode == 2 ? whole_round(C, w, dk->P.l) : 0;
+        const int jmax = round > 0 ? 2 * count / round * round : 0;
+        if (jmax > 0) {
+            if (dk->P.l == 3) launch_br4w<3>(w, jmax / w, s, A, mux_gate, n_jobs, round);
+            else launch_br4w<2>(w, jmax / w, s, A, mux_gate, n_jobs, round);
             A.split_round = round;
         }
         const int blocks = 2 * count;
@@ -2676,7 +2722,7 @@ int gates_device_locked(Ctx& C, tfhe_amd_device_key* dk, int count, const int32_
         A.twc = dk->twc;
         // whole device rounds on four-bootstrap workgroups, the rest on one-bootstrap ones
         // (split on the device as for mode 2; at most 2*count jobs in all)
-        const int round = whole_round(C, 1, dk->P.l);
+        const int round = whole_round(C, 4, dk->P.l);
         const int jmax = round > 0 ? 2 * count / round * round : 0;
         if (jmax > 0) {
             if (dk->P.l == 3) hipLaunchKernelGGL((br2_kernel<3, 4>), dim3(jmax / 4), dim3(512), 0, s, A, mux_gate, n_jobs, round);
