@@ -89,6 +89,7 @@ def test_ragged_batch_in_the_batched_kernel(key):
 @pytest.mark.parametrize("mode,count,kind", [(2, 8300, "mixed"), (2, 4133, "mixed"), (2, 2048, "mux"),
                                              (2, 1280, "two_input"), (2, 1536, "two_input"),
                                              (2, 1792, "two_input"), (2, 3072, "two_input"),
+                                             (2, 1100, "two_input"), (2, 1901, "two_input"), (2, 1300, "mixed"),
                                              (3, 1024, "two_input"), (3, 512, "mux"), (3, 1100, "mixed")])
 def test_whole_round_workgroups_match_one_bootstrap_workgroups(key, mode, count, kind):
     """Mode 2 runs whole device rounds on workgroups of 5 to 8 bootstraps (the batch's
@@ -98,7 +99,10 @@ def test_whole_round_workgroups_match_one_bootstrap_workgroups(key, mode, count,
     jobs: three rounds of eight split off), 4133 (about 4500 jobs: too few whole rounds
     to split), 2048 MUX gates (4096 jobs, two whole rounds of eight, second bootstraps
     included), 1280 / 1536 / 1792 two-input gates (one round of five / six / seven) and
-    3072 (one and a half rounds of eight: all on one-bootstrap workgroups); mode 3: 1024 two-input gates and 512 MUX gates (one whole
+    3072 (one and a half rounds of eight: all on one-bootstrap workgroups), and partial
+    single rounds that still give every CU w - 1 or more: 1100 two-input gates (five per
+    workgroup), 1901 (eight, the last workgroup with five) and 1300 gates with MUX gates
+    (six); mode 3: 1024 two-input gates and 512 MUX gates (one whole
     round each) and 1100 gates with MUX gates (not whole rounds). Bit-identical, and
     gates from every part equal the oracle."""
     T, P, sk, keys = key
