@@ -90,7 +90,8 @@ def test_ragged_batch_in_the_batched_kernel(key):
                                              (2, 1280, "two_input"), (2, 1536, "two_input"),
                                              (2, 1792, "two_input"), (2, 3072, "two_input"),
                                              (2, 1100, "two_input"), (2, 1901, "two_input"), (2, 1300, "mixed"),
-                                             (3, 1024, "two_input"), (3, 512, "mux"), (3, 1100, "mixed")])
+                                             (3, 1024, "two_input"), (3, 512, "mux"), (3, 1100, "mixed"),
+                                             (3, 1000, "two_input"), (3, 900, "mixed")])
 def test_whole_round_workgroups_match_one_bootstrap_workgroups(key, mode, count, kind):
     """Mode 2 runs whole device rounds on workgroups of 5 to 8 bootstraps (the batch's
     bootstraps per CU when that is 5 to 8, else 8) with a barrier per key row, mode 3 (1024 jobs) on workgroups of four two-wavefront bootstraps, and the
@@ -103,7 +104,9 @@ def test_whole_round_workgroups_match_one_bootstrap_workgroups(key, mode, count,
     single rounds that still give every CU w - 1 or more: 1100 two-input gates (five per
     workgroup), 1901 (eight, the last workgroup with five) and 1300 gates with MUX gates
     (six); mode 3: 1024 two-input gates and 512 MUX gates (one whole
-    round each) and 1100 gates with MUX gates (not whole rounds). Bit-identical, and
+    round each), 1100 gates with MUX gates (not whole rounds), 1000 two-input gates
+    (one partial round of four per workgroup) and 900 gates with MUX gates (about 980
+    jobs: the same). Bit-identical, and
     gates from every part equal the oracle."""
     T, P, sk, keys = key
     rng = np.random.default_rng(81 + count)

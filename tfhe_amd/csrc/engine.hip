@@ -105,8 +105,8 @@ struct BrArgs {
     int32_t n;
     int32_t Bgbit;
     uint32_t offset;  // gadget offset, libtfhe/tgsw.cpp:21-27
-    int32_t split_round;  // batched kernel: > 0 when br4w_kernel ran the first
-    int32_t split_width;  //   round_split(n_jobs, split_round, split_width) jobs
+    int32_t split_round;    // > 0 when the several-bootstrap workgroups ran the first
+    int32_t split_partial;  //   round_split(n_jobs, split_round, split_partial) jobs
 };
 
 // Orders LDS accesses between the lanes of one wavefront. The blind-rotation
@@ -662,14 +662,14 @@ __device__ __forceinline__ void br4_body(BrArgs A, const int32_t* __restrict__ m
 // start the remainder's bootstraps as soon as bootstraps of the round before end, where a
 // kernel boundary holds them until the slowest CU is done: that costs about a tenth of a
 // round, each round on br4w_kernel saves a twenty-fifth (4,470 jobs: 25.9 ms split after
-// two rounds, 24.4 ms after one, 23.7 ms all on one-bootstrap workgroups). With w > 0
-// (br4w_kernel's width), a batch of less than one round that still gives every CU w - 1
-// or more bootstraps (n > round - round / w) runs there as one partial round.
+// two rounds, 24.4 ms after one, 23.7 ms all on one-bootstrap workgroups). A batch of
+// less than one round with more than `partial` jobs (partial > 0) runs there as one
+// partial round.
 constexpr int BR4W_MIN_ROUNDS = 3;
-__device__ __forceinline__ int round_split(int n, int round, int w) {
+__device__ __forceinline__ int round_split(int n, int round, int partial) {
     const int k = n / round;
     if (n == k * round) return n;
-    if (k == 0 && w > 0 && n > round - round / w) return n;
+    if (k == 0 && partial > 0 && n > partial) return n;
     return k - 1 >= BR4W_MIN_ROUNDS ? (k - 1) * round : 0;
 }
 
@@ -679,7 +679,7 @@ template <int L, bool CT = false>
 __global__ __launch_bounds__(64, 8) void br4_kernel(BrArgs A, const int32_t* __restrict__ mux_gate,
                                                    const int32_t* __restrict__ n_jobs) {
     __shared__ int16_t bara[1024];
-    const int job = (A.split_round > 0 ? round_split(*n_jobs, A.split_round, A.split_width) : 0) + (int)blockIdx.x;
+    const int job = (A.split_round > 0 ? round_split(*n_jobs, A.split_round, A.split_partial) : 0) + (int)blockIdx.x;
     if constexpr (CT) {
         __shared__ c64 fb[Br4Lds::CT_C64], fb2[Br4Lds::CT_C64];
         br4_body<L, CT>(A, mux_gate, n_jobs, fb, bara, fb2, job, (int)threadIdx.x);
@@ -711,7 +711,7 @@ __global__ __launch_bounds__(64 * W, 1) void br4w_kernel(BrArgs A, const int32_t
     const int job = (int)blockIdx.x * W + w;
     // uniform per workgroup except in a partial round's last workgroup, whose wavefronts
     // past the job count end (the barrier then waits for the others only)
-    if (job >= round_split(*n_jobs, round, W)) return;
+    if (job >= round_split(*n_jobs, round, A.split_partial)) return;
     br4_body<L, true, true>(A, mux_gate, n_jobs, fb[w], bara[w], fb2[w], job, (int)threadIdx.x & 63);
 }
 
@@ -1080,11 +1080,11 @@ __global__ __launch_bounds__(128 * B, 2 / (B > 1 ? 2 : 1)) void br2_kernel(BrArg
     const int c = __builtin_amdgcn_readfirstlane(tid >> 6);
     int job;
     if constexpr (B == 1) {
-        job = (A.split_round > 0 ? round_split(*n_jobs, A.split_round, 0) : 0) + (int)blockIdx.x;
+        job = (A.split_round > 0 ? round_split(*n_jobs, A.split_round, A.split_partial) : 0) + (int)blockIdx.x;
     } else {
         job = (int)blockIdx.x * B + pb;
-        // uniform per workgroup (the split is a multiple of B)
-        if (job >= round_split(*n_jobs, round, 0)) return;
+        // uniform per workgroup except in a partial round's last workgroup
+        if (job >= round_split(*n_jobs, round, A.split_partial)) return;
     }
     // uniform per pair; its wavefronts end, and the workgroup's barriers then wait for the
     // other pairs only
@@ -2711,10 +2711,11 @@ int gates_device_locked(Ctx& C, tfhe_amd_device_key* dk, int count, const int32_
         const int round = mode == 2 ? whole_round(C, w, dk->P.l) : 0;
         const int jmax = round > 0 ? 2 * count / round * round : 0;
         if (jmax > 0) {
+            // a partial round on these when it gives every CU w - 1 or more bootstraps
+            A.split_round = round;
+            A.split_partial = round - round / w;
             if (dk->P.l == 3) launch_br4w<3>(w, jmax / w, s, A, mux_gate, n_jobs, round);
             else launch_br4w<2>(w, jmax / w, s, A, mux_gate, n_jobs, round);
-            A.split_round = round;
-            A.split_width = w;
         }
         const int blocks = 2 * count;
         if (dk->P.l == 3) hipLaunchKernelGGL((br4_kernel<3, true>), dim3(blocks), dim3(64), 0, s, A, mux_gate, n_jobs);
@@ -2731,9 +2732,12 @@ int gates_device_locked(Ctx& C, tfhe_amd_device_key* dk, int count, const int32_
         const int round = whole_round(C, 4, dk->P.l);
         const int jmax = round > 0 ? 2 * count / round * round : 0;
         if (jmax > 0) {
+            // a partial round on these when it leaves at most an eighth of the slots empty
+            // (one round of 1000 gates +3.9%; 900 even, 800 -3.3%)
+            A.split_round = round;
+            A.split_partial = round - round / 8;
             if (dk->P.l == 3) hipLaunchKernelGGL((br2_kernel<3, 4>), dim3(jmax / 4), dim3(512), 0, s, A, mux_gate, n_jobs, round);
             else hipLaunchKernelGGL((br2_kernel<2, 4>), dim3(jmax / 4), dim3(512), 0, s, A, mux_gate, n_jobs, round);
-            A.split_round = round;
         }
         const int blocks = 2 * count;
         if (dk->P.l == 3) hipLaunchKernelGGL((br2_kernel<3, 1>), dim3(blocks), dim3(128), 0, s, A, mux_gate, n_jobs, 0);
