@@ -2514,11 +2514,16 @@ int run_keyswitch(const tfhe_amd_device_key* dk, const int32_t* ext, const int32
         hipLaunchKernelGGL(ks_prep_kernel, dim3(count), dim3(256), 0, s, ext, second, nin, 1 << 29, prec_offset0, abar,
                            cpad, binit, P.ks_t, P.ks_basebit, nz);
         // small batches leave most CUs idle: slice the input coefficients over
-        // up to 32 workgroups per tile (an even number of coefficients each)
+        // up to 32 workgroups per tile (an even number of coefficients each). Tiles of 128
+        // gates: up to 1024 workgroups, at least 128 coefficients per slice (4096 gates:
+        // four slices, 0.374 -> 0.331 ms; 2048: 0.206 -> 0.189; 256: 0.078 -> 0.056; 1024
+        // gates keep eight, sixteen ran 0.117 -> 0.125); tiles of 64: up to 512, at least
+        // 16 (one gate: 32 slices)
         const int gt = count > 64 ? KS_MMA_GT_LARGE : KS_MMA_GT_SMALL;
         const int gx = dk->npad / 64, gy = (count + 16 * gt - 1) / (16 * gt);
+        const int target = gt == KS_MMA_GT_LARGE ? 1024 : 512, min_coefs = gt == KS_MMA_GT_LARGE ? 128 : 16;
         int slices = 1;
-        while (gx * gy * slices < 512 && slices < 32 && nin / (2 * slices) >= 16) slices *= 2;
+        while (gx * gy * slices < target && slices < 32 && nin / (2 * slices) >= min_coefs) slices *= 2;
         const int ips = (nin / slices + 1) & ~1;
         if (slices > 1) hipLaunchKernelGGL(ks_out_init_kernel, dim3(count), dim3(256), 0, s, binit, P.n, out, out_idx);
         if (gt == KS_MMA_GT_LARGE)
