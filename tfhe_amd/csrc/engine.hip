@@ -2214,6 +2214,16 @@ __global__ __launch_bounds__(256, 2) void ks_mma_kernel(const int8_t* __restrict
     const int q = lane >> 4, r16 = lane & 15;
     const int col = (blockIdx.x * 4 + w) * 16 + r16;
     const int g0 = blockIdx.y * GT * 16;
+    // one-hot words of the four 2-bit digits in a byte: s_onehot[x][jj] = 1 << 8 d_jj
+    __shared__ v4i s_onehot[256];
+    {
+        const uint32_t x = threadIdx.x;
+        v4i o;
+#pragma unroll
+        for (int jj = 0; jj < 4; jj++) o[jj] = 1 << (((x >> (6 - 2 * jj)) & 3u) << 3);
+        s_onehot[x] = o;
+    }
+    __syncthreads();
     v4i acc[GT][4];
 #pragma unroll
     for (int gt = 0; gt < GT; gt++)
@@ -2246,10 +2256,7 @@ __global__ __launch_bounds__(256, 2) void ks_mma_kernel(const int8_t* __restrict
         if (i0 + 2 < iend) load(i0 + 2);
 #pragma unroll
         for (int gt = 0; gt < GT; gt++) {
-            const uint32_t x = (a_cur[gt] >> sh) & 255u;
-            v4i av;
-#pragma unroll
-            for (int jj = 0; jj < 4; jj++) av[jj] = 1 << (((x >> (6 - 2 * jj)) & 3u) << 3);
+            const v4i av = s_onehot[(a_cur[gt] >> sh) & 255u];
 #pragma unroll
             for (int b = 0; b < 4; b++)
                 acc[gt][b] = __builtin_amdgcn_mfma_i32_16x16x64_i8(av, b_cur[b], acc[gt][b], 0, 0, 0);
