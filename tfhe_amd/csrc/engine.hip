@@ -2340,6 +2340,8 @@ Ctx& ctx() {
     return c;
 }
 
+int whole_round(Ctx& C, int w, int l);
+
 int ensure_ready(Ctx& C) {
     if (C.ready) return 0;
     int ndev = 0;
@@ -2350,6 +2352,10 @@ int ensure_ready(Ctx& C) {
     // the device address of this GPU's copy of the static workspace (the device is current)
     HIP_TRY(hipGetSymbolAddress((void**)&C.noalloc_scr, HIP_SYMBOL(scratch)));
     if (!C.noalloc_scr) return set_error("hipGetSymbolAddress(scratch) returned a null address");
+    // the round sizes of the several-bootstrap workgroups (occupancy queries), here rather
+    // than at a first launch that may be made under stream capture
+    for (int l = 2; l <= 3; l++)
+        for (int w = 4; w <= BR4W_MAX; w++) whole_round(C, w, l);
     C.ready = true;
     return 0;
 }
